@@ -1,0 +1,69 @@
+#!/usr/bin/env bash
+# Build the reference FORGE program (Fortran 90, /root/reference) for one
+# compile-time configuration, plus the project-written harness
+# oracle/ref_harness.f90 that calls the reference modules and dumps arrays.
+#
+# The reference is configured only through `parameter`s in Setup.f90
+# (Setup.f90:6-74); this recipe edits those values with sed in a scratch copy
+# that lives under oracle/_ref/ only while the compiler runs and is deleted on
+# exit.  Only the executables (and nothing else) stay in oracle/_ref/<case>/.
+#
+# Two edits beyond Setup parameters are needed because flang rejects constant
+# out-of-bounds subscripts inside the dead numS=2 branches that ifort accepts:
+#   Main.f90:58      nOccStates(numS), nPartOccStates(numS)  -> (2)
+#   Main.f90:635-636 KineticEnergyIn(2), FockEnergyIn(2)     -> (numS)
+# The first only enlarges arrays that the numS=1 path indexes at 1; the second
+# only changes the never-executed numS=2 output branch.
+# compile.sh:1's file order is also wrong (Geometry.f90:5 uses TightBinding),
+# so the modules are compiled in dependency order here.
+#
+# Usage: [WRITEFOCK=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
+#   XI_NM is xi*0.246 (gate parameter in nm).  Toolchain: amdflang (ROCm) and
+#   MKL LP64 sequential from /opt/conda/lib.
+set -euo pipefail
+CASE=$1; NTHETA=$2; NUMK=$3; NFILL=$4; ITMAX=$5; NREAD=$6; XINM=$7; NLAYERS=${8:-2}
+WRITEFOCK=${WRITEFOCK:-0}
+REF=/root/reference
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref/$CASE
+FC=${FC:-/opt/rocm/bin/amdflang}
+MKL=${MKL_DIR:-/opt/conda/lib}
+mkdir -p "$OUT"
+SCR=$(mktemp -d "$HERE/_ref/.scratch.XXXXXX")
+trap 'rm -rf "$SCR"' EXIT
+for f in LapackRoutines Setup TightBinding HartreeFock Geometry Main; do
+  cp "$REF/$f.f90" "$SCR/$f.f90"
+done
+if [ "$NLAYERS" = "3" ]; then ROT='[-1,+1,-1]'; else ROT='[-1,+1]'; fi
+sed -i \
+  -e "s/^integer(dp), parameter :: nlayers = .*/integer(dp), parameter :: nlayers = $NLAYERS/" \
+  -e "s/^integer(dp), parameter :: ntheta = .*/integer(dp), parameter :: ntheta = $NTHETA/" \
+  -e "s/^integer(dp), parameter :: RotateLayers(nlayers) = .*/integer(dp), parameter :: RotateLayers(nlayers) = $ROT/" \
+  -e "s/^integer(dp), parameter :: numk = .*/integer(dp), parameter :: numk = $NUMK/" \
+  -e "s/^integer(dp), parameter :: numkIn = .*/integer(dp), parameter :: numkIn = $NUMK/" \
+  -e "s/^real(dp), parameter :: nfilling = .*/real(dp), parameter :: nfilling = ${NFILL}_dp/" \
+  -e "s/^real(dp), parameter :: nfillingIn = .*/real(dp), parameter :: nfillingIn = ${NFILL}_dp/" \
+  -e "s/^integer(dp), parameter :: itmax = .*/integer(dp), parameter :: itmax = $ITMAX/" \
+  -e "s/^integer(dp), parameter :: nRead = .*/integer(dp), parameter :: nRead = $NREAD/" \
+  -e "s/^integer(dp), parameter :: nPrintOutputs = .*/integer(dp), parameter :: nPrintOutputs = 0/" \
+  -e "s/^integer(dp), parameter :: nWriteFock = .*/integer(dp), parameter :: nWriteFock = $WRITEFOCK/" \
+  -e "s/^real(dp), parameter:: xi = .*/real(dp), parameter:: xi = ${XINM}_dp\/0.246_dp/" \
+  -e "s/^real(dp), parameter:: xiIn = .*/real(dp), parameter:: xiIn = ${XINM}_dp\/0.246_dp/" \
+  "$SCR/Setup.f90"
+grep -q "ntheta = $NTHETA\$" "$SCR/Setup.f90"
+grep -q "numk = $NUMK\$" "$SCR/Setup.f90"
+sed -i \
+  -e '58s/nOccStates(numS), nPartOccStates(numS)/nOccStates(2), nPartOccStates(2)/' \
+  -e '635s/KineticEnergyIn(2)/KineticEnergyIn(numS)/' \
+  -e '636s/FockEnergyIn(2)/FockEnergyIn(numS)/' \
+  "$SCR/Main.f90"
+cp "$HERE/ref_harness.f90" "$SCR/ref_harness.f90"
+cd "$SCR"
+FFLAGS="-O2 -fopenmp -w"
+for f in LapackRoutines Setup TightBinding HartreeFock Geometry; do
+  $FC $FFLAGS -c $f.f90
+done
+LIBS="-L$MKL -Wl,-rpath,$MKL -lmkl_gf_lp64 -lmkl_sequential -lmkl_core -lpthread -lm -ldl"
+$FC $FFLAGS -o "$OUT/forge.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o Main.f90 $LIBS
+$FC $FFLAGS -o "$OUT/harness.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o ref_harness.f90 $LIBS
+echo "built $OUT (ntheta=$NTHETA numk=$NUMK nfilling=$NFILL itmax=$ITMAX nRead=$NREAD xi=${XINM}nm nlayers=$NLAYERS)"
