@@ -1,0 +1,896 @@
+// C ABI of libforge_hip: context, reference-interface entry points and the
+// device-resident SCF loop (the call sequence of Main.f90:236-679, numS = 1).
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace forge {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+static std::mutex g_blas_mu;
+static rocblas_handle g_blas[16] = {};
+rocblas_handle blas_handle(hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) {
+    set_error("blas_handle: no device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_blas_mu);
+  if (!g_blas[dev]) {
+    if (rocblas_create_handle(&g_blas[dev]) != rocblas_status_success) {
+      set_error("rocblas_create_handle failed");
+      return nullptr;
+    }
+  }
+  rocblas_set_stream(g_blas[dev], s);
+  return g_blas[dev];
+}
+
+static const double PI = 3.14159265358979323846264338327950288;
+
+// small kernels local to the engine -----------------------------------------
+__global__ void k_zero_upper(i64 n, z2* H) {
+  i64 q = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n * n) return;
+  i64 r = q % n, c = q / n;
+  if (r < c) H[q] = mk(0.0, 0.0);
+}
+
+// out[0] = a.b, out[1] = a.c, out[2] = b.c, out[3] = a.a, out[4] = b.b,
+// out[5] = a.w, out[6] = b.w  (single block, fixed order)
+__global__ void k_dots(i64 n, const double* a, const double* b, const double* ya,
+                       const double* yb, const double* w, double* out) {
+  __shared__ double sh[7][256];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (i64 i = threadIdx.x; i < n; i += 256) {
+    double av = a[i], bv = b ? b[i] : 0.0, yav = ya[i], ybv = yb ? yb[i] : 0.0, wv = w[i];
+    acc[0] += av * yav;  // da.L.da
+    acc[1] += av * ybv;  // da.L.db
+    acc[2] += bv * ybv;  // db.L.db
+    acc[3] += av * av;
+    acc[4] += av * bv;
+    acc[5] += bv * bv;
+    acc[6] += 0.0;
+    (void)wv;
+  }
+  for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 7; ++q) out[q] = sh[q][0];
+}
+
+// bias term of GetKineticEnergy: sum_i (d_i + 1/2) bias_i for d = a and b.
+__global__ void k_bias_dot(i64 n, const double* a, const double* b, const double* bias,
+                           double* out) {
+  __shared__ double sh[2][256];
+  double s0 = 0, s1 = 0;
+  for (i64 i = threadIdx.x; i < n; i += 256) {
+    s0 += (a[i] + 0.5) * bias[i];
+    if (b) s1 += (b[i] + 0.5) * bias[i];
+  }
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = sh[0][0]; out[1] = sh[1][0]; }
+}
+
+// pot = 2 alphaH ((1-s) ya + s yb) + U ((1-s) da + s db); d_out = (1-s) da + s db
+__global__ void k_potential(i64 n, const double* ya, const double* yb, const double* da,
+                            const double* db, double s, double alphaH, double U, double* pot,
+                            double* dout) {
+  i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double y = yb ? (1.0 - s) * ya[i] + s * yb[i] : ya[i];
+  double d = db ? (1.0 - s) * da[i] + s * db[i] : da[i];
+  pot[i] = 2.0 * alphaH * y + U * d;
+  if (dout) dout[i] = d;
+}
+
+// Psi(:, s) = Cwin(:, idx[s])
+__global__ void k_gather_cols(i64 n, i64 S, const int* idx, const z2* Cwin, z2* Psi) {
+  i64 q = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n * S) return;
+  i64 r = q % n, s = q / n;
+  Psi[q] = Cwin[r + n * (i64)idx[s]];
+}
+
+// window columns nL-1 .. nU-1 of C (batch) -> Cwin(:, b + nb*k)
+__global__ void k_copy_window(i64 n, i64 nb, i64 col0, int K, i64 k0, const z2* C, size_t ldC,
+                              size_t sC, z2* Cwin) {
+  i64 q = (i64)blockIdx.x * 256 + threadIdx.x;
+  int kb = blockIdx.y;
+  if (q >= n * nb) return;
+  i64 r = q % n, b = q / n;
+  Cwin[r + n * (b + nb * (k0 + kb))] = C[kb * sC + r + ldC * (col0 + b)];
+}
+
+}  // namespace forge
+
+using namespace forge;
+
+struct forge_ctx {
+  forge_params P;
+  PairParams pp;
+  hipStream_t s = nullptr;
+  i64 n = 0, nnc = 0, numk = 0, Nk = 0, nL = 0, nU = 0, nb = 0;
+  std::vector<double> Coords, tn, MomVal;
+  std::vector<i64> nMomComp, u1, u2;
+  DeviceGeom g;
+  double *X = nullptr, *V = nullptr, *T = nullptr, *L = nullptr, *bias = nullptr, *pot = nullptr;
+  z2 *rho = nullptr, *rho_new = nullptr;  // rho_new has nnc+1 matrices (last = R scratch)
+  // eigensolver batch
+  int Kb = 0;
+  z2* H = nullptr;
+  void* trd_ws = nullptr;
+  TridiagWork trd{};
+  void* dc_ws = nullptr;
+  double *Zq = nullptr, *evals = nullptr;
+  z2* Cv = nullptr;
+  void* bt_ws = nullptr;
+  z2* Cwin = nullptr;  // n x nb x Nk
+  double *ph_asm = nullptr, *ph_cell = nullptr;  // device [Nk][nnc][2]
+  std::vector<double> h_ph_asm, h_ph_cell;
+  z2* Psi = nullptr;
+  double* wph = nullptr;
+  int* gidx = nullptr;
+  double *part = nullptr, *e5 = nullptr, *dots = nullptr;
+  double *da = nullptr, *db = nullptr, *ya = nullptr, *yb = nullptr;
+  // SCF state
+  forge_scf_options opt{};
+  bool scf_ready = false;
+  double e[4] = {0, 0, 0, 0}, Etot = 0;
+  std::vector<double> h_da;
+  int it = 0;
+  double tm[16] = {};
+  hipEvent_t ev[16];
+};
+
+static double total_energy(const forge_params& P, const double* e) {
+  return P.alpha * e[1] * 2.0 + e[0] * 2.0 + P.U * e[3] + P.alphaH * e[2];
+}
+
+extern "C" const char* forge_last_error(void) { return g_err.c_str(); }
+
+extern "C" int forge_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+#define CK(x)                 \
+  do {                        \
+    int _r = (x);             \
+    if (_r != 0) return _r;   \
+  } while (0)
+
+static int dalloc(void** p, size_t bytes) {
+  FORGE_HIP(hipMalloc(p, bytes ? bytes : 256));
+  return 0;
+}
+
+extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, const double* tn,
+                                const double* MomentaValues, const int64_t* nMomentaComponents,
+                                const int64_t* nUnitCell_1, const int64_t* nUnitCell_2,
+                                forge_ctx** out) {
+  if (!p || !Coords || !tn || !MomentaValues || !nMomentaComponents || !nUnitCell_1 ||
+      !nUnitCell_2 || !out) {
+    set_error("forge_ctx_create: null argument");
+    return -2;
+  }
+  if (p->ndim < 2 || p->nnc < 1 || p->nnc > MAXCELL || p->numk < 1 || p->nUpper > p->ndim ||
+      p->nLower < 1 || p->nLower > p->nUpper) {
+    set_error("forge_ctx_create: invalid sizes in forge_params");
+    return -2;
+  }
+  if (p->fphase != 0) {
+    set_error("forge_ctx_create: fphase != 0 is not supported (Setup.f90:25 uses 0)");
+    return -2;
+  }
+  int ndev = forge_device_count();
+  if (ndev < 1) {
+    set_error("forge_ctx_create: no HIP device");
+    return -3;
+  }
+  forge_ctx* c = new forge_ctx();
+  c->P = *p;
+  c->pp = pair_params(*p);
+  c->n = p->ndim;
+  c->nnc = p->nnc;
+  c->numk = p->numk;
+  c->Nk = p->numk * p->numk;
+  c->nL = p->nLower;
+  c->nU = p->nUpper;
+  c->nb = p->nUpper - p->nLower + 1;
+  const i64 n = c->n, Nk = c->Nk, nnc = c->nnc;
+  c->Coords.assign(Coords, Coords + 3 * n);
+  c->tn.assign(tn, tn + 6);
+  c->MomVal.assign(MomentaValues, MomentaValues + 2 * Nk);
+  c->nMomComp.assign(nMomentaComponents, nMomentaComponents + 2 * Nk);
+  c->u1.assign(nUnitCell_1, nUnitCell_1 + nnc);
+  c->u2.assign(nUnitCell_2, nUnitCell_2 + nnc);
+  c->opt.occ_mode = 0;
+  c->opt.mix_mode = 0;
+  c->opt.beta = 0.3;
+  c->opt.etol = 0.0;
+  FORGE_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  for (int q = 0; q < 16; ++q) FORGE_HIP(hipEventCreate(&c->ev[q]));
+  const size_t nn = (size_t)n * n;
+  CK(dalloc((void**)&c->X, sizeof(double) * 3 * n));
+  FORGE_HIP(hipMemcpy(c->X, Coords, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  c->g.n = n;
+  c->g.nnc = (int)nnc;
+  c->g.X = c->X;
+  c->g.Y = c->X + n;
+  c->g.Z = c->X + 2 * n;
+  for (int q = 0; q < MAXCELL; ++q) {
+    c->g.cells.n1[q] = q < nnc ? (double)nUnitCell_1[q] : 0.0;
+    c->g.cells.n2[q] = q < nnc ? (double)nUnitCell_2[q] : 0.0;
+  }
+  c->g.cells.t1x = tn[0];
+  c->g.cells.t1y = tn[1];
+  c->g.cells.t2x = tn[2];
+  c->g.cells.t2y = tn[3];
+  CK(dalloc((void**)&c->V, sizeof(double) * nnc * nn));
+  CK(dalloc((void**)&c->T, sizeof(double) * nnc * nn));
+  CK(pair_tables(c->g, c->pp, c->V, c->T, c->s));
+  // layer bias, HamiltonianHartreeFock.f90:119-126
+  std::vector<double> hb(n, 0.0);
+  if (p->nlayers > 1) {
+    i64 per = n / p->nlayers;
+    for (i64 l = 0; l < p->nlayers; ++l)
+      for (i64 i = 0; i < per; ++i)
+        hb[i + l * per] = (double)((float)l - (float)(p->nlayers - 1) / 2.0f) * p->Delta /
+                          (double)(float)(p->nlayers - 1);
+  }
+  CK(dalloc((void**)&c->bias, sizeof(double) * n));
+  FORGE_HIP(hipMemcpy(c->bias, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  CK(dalloc((void**)&c->pot, sizeof(double) * n));
+  FORGE_HIP(hipMemset(c->pot, 0, sizeof(double) * n));
+  CK(dalloc((void**)&c->rho, sizeof(z2) * nnc * nn));
+  CK(dalloc((void**)&c->rho_new, sizeof(z2) * (nnc + 1) * nn));
+  // k phases: assembly uses k.T_c (HartreeFock.f90:90), cells use the exact
+  // grid phase -2 pi (i1 n1 + i2 (n1 + n2)) / numk (HartreeFock.f90:471)
+  c->h_ph_asm.assign(Nk * nnc * 2, 0.0);
+  c->h_ph_cell.assign(Nk * nnc * 2, 0.0);
+  for (i64 k = 0; k < Nk; ++k) {
+    i64 i1 = nMomentaComponents[k], i2 = nMomentaComponents[k + Nk];
+    double vx = MomentaValues[i1 + c->numk * i2], vy = MomentaValues[i1 + c->numk * i2 + Nk];
+    for (i64 cc = 0; cc < nnc; ++cc) {
+      double n1 = (double)nUnitCell_1[cc], n2 = (double)nUnitCell_2[cc];
+      double th = vx * (n1 * tn[0] + n2 * tn[2]) + vy * (n1 * tn[1] + n2 * tn[3]);
+      c->h_ph_asm[(k * nnc + cc) * 2] = cos(th);
+      c->h_ph_asm[(k * nnc + cc) * 2 + 1] = sin(th);
+      double a = -2.0 * PI *
+                 (double)(i1 * nUnitCell_1[cc] + i2 * nUnitCell_1[cc] + i2 * nUnitCell_2[cc]) /
+                 (double)c->numk;
+      c->h_ph_cell[(k * nnc + cc) * 2] = cos(a);
+      c->h_ph_cell[(k * nnc + cc) * 2 + 1] = sin(a);
+    }
+  }
+  CK(dalloc((void**)&c->ph_asm, sizeof(double) * Nk * nnc * 2));
+  CK(dalloc((void**)&c->ph_cell, sizeof(double) * Nk * nnc * 2));
+  FORGE_HIP(hipMemcpy(c->ph_asm, c->h_ph_asm.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
+  FORGE_HIP(hipMemcpy(c->ph_cell, c->h_ph_cell.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
+  // eigensolver batch size from free memory
+  size_t fr = 0, tot = 0;
+  FORGE_HIP(hipMemGetInfo(&fr, &tot));
+  const i64 nU = c->nU;
+  size_t per_k = sizeof(z2) * nn + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
+                 sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
+  size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + (size_t)64 << 20;
+  size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
+  i64 Kb = (i64)(budget / per_k);
+  if (Kb < 1) {
+    set_error("forge_ctx_create: not enough device memory for one k-point");
+    return -4;
+  }
+  c->Kb = (int)std::min<i64>(Kb, Nk);
+  const int Kb_ = c->Kb;
+  CK(dalloc((void**)&c->H, sizeof(z2) * nn * Kb_));
+  CK(dalloc(&c->trd_ws, tridiag_workspace_bytes(n, Kb_)));
+  CK(tridiag_bind(c->trd, c->trd_ws, n, Kb_));
+  c->trd.A = c->H;
+  c->trd.sA = nn;
+  CK(dalloc(&c->dc_ws, dc_workspace_bytes(n, Kb_, nU)));
+  CK(dalloc((void**)&c->Zq, sizeof(double) * n * nU * Kb_));
+  CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
+  CK(dalloc((void**)&c->Cv, sizeof(z2) * n * nU * Kb_));
+  CK(dalloc(&c->bt_ws, backtr_workspace_bytes(n, Kb_, nU)));
+  CK(dalloc((void**)&c->Cwin, sizeof(z2) * n * c->nb * Nk));
+  CK(dalloc((void**)&c->Psi, sizeof(z2) * n * c->nb * Nk));
+  CK(dalloc((void**)&c->wph, sizeof(double) * c->nb * Nk * nnc * 2));
+  CK(dalloc((void**)&c->gidx, sizeof(int) * c->nb * Nk));
+  CK(dalloc((void**)&c->part, sizeof(double) * 5 * 2048));
+  CK(dalloc((void**)&c->e5, sizeof(double) * 16));
+  CK(dalloc((void**)&c->dots, sizeof(double) * 16));
+  CK(dalloc((void**)&c->da, sizeof(double) * n));
+  CK(dalloc((void**)&c->db, sizeof(double) * n));
+  CK(dalloc((void**)&c->ya, sizeof(double) * n));
+  CK(dalloc((void**)&c->yb, sizeof(double) * n));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  *out = c;
+  return 0;
+}
+
+extern "C" int forge_ctx_destroy(forge_ctx* c) {
+  if (!c) return 0;
+  hipStreamSynchronize(c->s);
+  void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
+                  c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
+                  c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  for (i64 q = 0; q < c->trd.nev; ++q) hipEventDestroy(c->trd.evs[q]);
+  delete[] c->trd.evs;
+  for (int q = 0; q < 16; ++q) hipEventDestroy(c->ev[q]);
+  hipStreamDestroy(c->s);
+  delete c;
+  return 0;
+}
+
+static int ensure_longrange(forge_ctx* c) {
+  if (c->L) return 0;
+  CK(dalloc((void**)&c->L, sizeof(double) * c->n * c->n));
+  CK(longrange(c->g, c->pp, (int)c->P.numC, c->L, c->s));
+  return 0;
+}
+
+extern "C" int forge_longrange(forge_ctx* c, double* LongRange) {
+  if (!c || !LongRange) { set_error("forge_longrange: null argument"); return -2; }
+  CK(ensure_longrange(c));
+  FORGE_HIP(hipMemcpyAsync(LongRange, c->L, sizeof(double) * c->n * c->n, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+static int hamiltonian_one(forge_ctx* c, forge_z* zH, const double* Potential, const forge_z* zFock,
+                           const double* vk) {
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  std::vector<double> ph(nnc * 2);
+  for (i64 cc = 0; cc < nnc; ++cc) {
+    double n1 = (double)c->u1[cc], n2 = (double)c->u2[cc];
+    double th = vk[0] * (n1 * c->tn[0] + n2 * c->tn[2]) + vk[1] * (n1 * c->tn[1] + n2 * c->tn[3]);
+    ph[cc * 2] = cos(th);
+    ph[cc * 2 + 1] = sin(th);
+  }
+  double* dph = (double*)c->part;  // scratch
+  FORGE_HIP(hipMemcpyAsync(dph, ph.data(), sizeof(double) * nnc * 2, hipMemcpyHostToDevice, c->s));
+  if (Potential)
+    FORGE_HIP(hipMemcpyAsync(c->da, Potential, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  else
+    FORGE_HIP(hipMemsetAsync(c->da, 0, sizeof(double) * n, c->s));
+  if (zFock)
+    FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  CK(assemble(n, (int)nnc, 1, zFock ? c->rho_new : nullptr, c->V, c->T, c->da, c->bias,
+              c->P.alpha, dph, c->H, n, nn, c->s));
+  hipLaunchKernelGGL(k_zero_upper, dim3(cdiv(n * n, 256)), dim3(256), 0, c->s, n, c->H);
+  FORGE_LAUNCH();
+  FORGE_HIP(hipMemcpyAsync(zH, c->H, sizeof(z2) * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+extern "C" int forge_tb_hamiltonian(forge_ctx* c, forge_z* zH, const double* vk) {
+  if (!c || !zH || !vk) { set_error("forge_tb_hamiltonian: null argument"); return -2; }
+  return hamiltonian_one(c, zH, nullptr, nullptr, vk);
+}
+
+extern "C" int forge_hamiltonian_hf(forge_ctx* c, forge_z* zH, const double* Potential,
+                                    const forge_z* zFock, const double* vk) {
+  if (!c || !zH || !Potential || !zFock || !vk) {
+    set_error("forge_hamiltonian_hf: null argument");
+    return -2;
+  }
+  return hamiltonian_one(c, zH, Potential, zFock, vk);
+}
+
+// Eigenpairs 1..mout of the Kb' matrices in c->H (lower): evals (Kb x n),
+// Cv (Kb x n x nU, ld n).  Returns D&C failures (> 0) or < 0.
+static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
+  const i64 n = c->n;
+  c->trd.K = K;
+  double hemv_ms = 0;
+  long hemv_n = 0;
+  if (timed) FORGE_HIP(hipEventRecord(c->ev[2], c->s));
+  CK(tridiag(c->trd, c->s, timed ? &hemv_ms : nullptr, timed ? &hemv_n : nullptr));
+  if (timed) FORGE_HIP(hipEventRecord(c->ev[3], c->s));
+  int fails = dc_solve(c->trd.d, c->trd.e, n, K, mout, c->evals, c->Zq, c->dc_ws, c->s);
+  if (fails < 0) return fails;
+  if (timed) FORGE_HIP(hipEventRecord(c->ev[4], c->s));
+  CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws, c->s));
+  if (timed) {
+    FORGE_HIP(hipEventRecord(c->ev[5], c->s));
+    FORGE_HIP(hipEventSynchronize(c->ev[5]));
+    float a, b, d;
+    hipEventElapsedTime(&a, c->ev[2], c->ev[3]);
+    hipEventElapsedTime(&b, c->ev[3], c->ev[4]);
+    hipEventElapsedTime(&d, c->ev[4], c->ev[5]);
+    c->tm[1] += a;
+    c->tm[2] += b;
+    c->tm[3] += d;
+    c->tm[9] += hemv_n;
+    c->tm[10] += hemv_ms;
+  }
+  return fails;
+}
+
+extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_t il, int64_t iu,
+                              int32_t* info) {
+  if (!A || !eigenvalues || n < 1 || il < 1 || iu < il || iu > n) {
+    set_error("forge_zdiagred: invalid argument");
+    return -2;
+  }
+  if (forge_device_count() < 1) { set_error("forge_zdiagred: no HIP device"); return -3; }
+  // a stand-alone one-matrix solve
+  hipStream_t s;
+  FORGE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const size_t nn = (size_t)n * n;
+  z2* H = nullptr; void* tw = nullptr; void* dw = nullptr; double* Zq = nullptr; double* ev = nullptr;
+  z2* Cv = nullptr; void* bw = nullptr;
+  CK(dalloc((void**)&H, sizeof(z2) * nn));
+  CK(dalloc(&tw, tridiag_workspace_bytes(n, 1)));
+  CK(dalloc(&dw, dc_workspace_bytes(n, 1, iu)));
+  CK(dalloc((void**)&Zq, sizeof(double) * n * iu));
+  CK(dalloc((void**)&ev, sizeof(double) * n));
+  CK(dalloc((void**)&Cv, sizeof(z2) * n * iu));
+  CK(dalloc(&bw, backtr_workspace_bytes(n, 1, iu)));
+  TridiagWork t{};
+  CK(tridiag_bind(t, tw, n, 1));
+  t.A = H;
+  t.sA = nn;
+  FORGE_HIP(hipMemcpyAsync(H, A, sizeof(z2) * nn, hipMemcpyHostToDevice, s));
+  CK(tridiag(t, s, nullptr, nullptr));
+  int fails = dc_solve(t.d, t.e, n, 1, iu, ev, Zq, dw, s);
+  if (fails < 0) return fails;
+  CK(backtransform(t, Zq, n, (size_t)n * iu, iu, Cv, n, (size_t)n * iu, bw, s));
+  const i64 m = iu - il + 1;
+  FORGE_HIP(hipMemcpyAsync(A, Cv + n * (il - 1), sizeof(z2) * n * m, hipMemcpyDeviceToHost, s));
+  std::vector<double> hv(n);
+  FORGE_HIP(hipMemcpyAsync(hv.data(), ev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  FORGE_HIP(hipStreamSynchronize(s));
+  for (i64 q = 0; q < m; ++q) eigenvalues[q] = hv[il - 1 + q];
+  if (info) *info = fails;
+  hipFree(H); hipFree(tw); hipFree(dw); hipFree(Zq); hipFree(ev); hipFree(Cv); hipFree(bw);
+  hipStreamDestroy(s);
+  return fails;
+}
+
+// Solve on the device: rho_dev (nnc cells) and pot_dev in, bulk density
+// (unnormalised, all cells) into c->rho_new, window bands to hband (nb x Nk,
+// host) and window eigenvectors to c->Cwin.
+static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
+                        std::vector<double>& hband, bool timed) {
+  const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nU = c->nU, nb = c->nb;
+  const size_t nn = (size_t)n * n;
+  hband.assign(nb * Nk, 0.0);
+  std::vector<double> hev(n * c->Kb);
+  int fails = 0;
+  for (i64 k0 = 0; k0 < Nk; k0 += c->Kb) {
+    const int K = (int)std::min<i64>(c->Kb, Nk - k0);
+    if (timed) FORGE_HIP(hipEventRecord(c->ev[0], c->s));
+    CK(assemble(n, (int)nnc, K, rho_dev, c->V, c->T, pot_dev, c->bias, c->P.alpha,
+                c->ph_asm + k0 * nnc * 2, c->H, n, nn, c->s));
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[1], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[1]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      c->tm[0] += a;
+      c->tm[7] += 1;
+      c->tm[8] += a;
+    }
+    int f = eig_batch(c, K, nU, timed);
+    if (f < 0) return f;
+    fails += f;
+    if (timed) FORGE_HIP(hipEventRecord(c->ev[6], c->s));
+    // bulk density: bands 1..nLower-1, phases of k0..k0+K-1, summed over k
+    // (the first k of the first batch overwrites)
+    {
+      // bulk_density overwrites for its first k; for later batches accumulate
+      CK(bulk_density(n, K, c->Cv, n, (size_t)n * nU, c->nL - 1, (int)nnc,
+                      c->ph_cell + k0 * nnc * 2, c->rho_new, k0 > 0, c->s));
+    }
+    hipLaunchKernelGGL(k_copy_window, dim3(cdiv(n * nb, 256), K), dim3(256), 0, c->s, n, nb,
+                       c->nL - 1, K, k0, c->Cv, (size_t)n, (size_t)n * nU, c->Cwin);
+    FORGE_LAUNCH();
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[7], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[7]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[6], c->ev[7]);
+      c->tm[4] += a;
+    }
+    FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    for (int kb = 0; kb < K; ++kb)
+      for (i64 b = 0; b < nb; ++b) hband[b + nb * (k0 + kb)] = hev[kb * n + (c->nL - 1) + b];
+  }
+  return fails;
+}
+
+extern "C" int forge_solve(forge_ctx* c, forge_z* zFockBulk, const forge_z* zFock,
+                           const double* Potential, double* Bands, forge_z* zEigenvectors) {
+  if (!c || !zFockBulk || !zFock || !Potential || !Bands || !zEigenvectors) {
+    set_error("forge_solve: null argument");
+    return -2;
+  }
+  const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nb = c->nb;
+  const size_t nn = (size_t)n * n;
+  FORGE_HIP(hipMemcpyAsync(c->rho, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  FORGE_HIP(hipMemcpyAsync(c->db, Potential, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  std::vector<double> hb;
+  int f = solve_device(c, c->rho, c->db, hb, false);
+  if (f < 0) return f;
+  FORGE_HIP(hipMemcpyAsync(zFockBulk, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(zEigenvectors, c->Cwin, sizeof(z2) * n * nb * Nk, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  std::memcpy(Bands, hb.data(), sizeof(double) * nb * Nk);
+  c->scf_ready = false;  // device rho buffers were reused
+  return f;
+}
+
+// ---------------------------------------------------------------- host-side sort / Fermi
+// dsort_lists, TightBinding.f90:984-1027 (heap sort; the value column carries
+// the flat index (b-1)*Nk + k).
+static void heap_sort2(std::vector<double>& key, std::vector<double>& val) {
+  const i64 N = (i64)key.size();
+  if (N < 2) return;
+  i64 l = N / 2 + 1, ir = N, i, j;
+  double ck, cv;
+  for (;;) {
+    if (l > 1) {
+      --l; ck = key[l - 1]; cv = val[l - 1];
+    } else {
+      ck = key[ir - 1]; cv = val[ir - 1];
+      key[ir - 1] = key[0]; val[ir - 1] = val[0];
+      if (--ir == 1) { key[0] = ck; val[0] = cv; return; }
+    }
+    i = l; j = 2 * l;
+    while (j <= ir) {
+      if (j < ir && key[j - 1] < key[j]) ++j;
+      if (ck < key[j - 1]) { key[i - 1] = key[j - 1]; val[i - 1] = val[j - 1]; i = j; j = 2 * j; }
+      else j = ir + 1;
+    }
+    key[i - 1] = ck; val[i - 1] = cv;
+  }
+}
+
+struct Occupation {
+  i64 nF = 0, nOcc = 0, nPart = 0;
+  double EF = 0, Deg = 0;
+  std::vector<int> idx;       // Cwin column of each weighted state
+  std::vector<double> wph;    // [S][nnc][2]
+};
+
+// SortEnergies_1Spin (HartreeFock.f90:885-925) + the Fermi block
+// (Main.f90:400-412) + GetFock weights (HartreeFock.f90:506-555).
+static void occupations(const forge_ctx* c, const std::vector<double>& band, Occupation& o) {
+  const i64 Nk = c->Nk, nb = c->nb, N = nb * Nk, nnc = c->nnc;
+  std::vector<double> key(N), val(N);
+  for (i64 ic = 1; ic <= N; ++ic) {
+    i64 indev = (ic - 1) / Nk + 1;
+    key[ic - 1] = band[(indev - 1) + nb * (ic - (indev - 1) * Nk - 1)];
+    val[ic - 1] = (double)ic;
+  }
+  heap_sort2(key, val);
+  const i64 NP = (c->n / 2 - c->nL) + 1;
+  double x = (double)(Nk * NP) + (double)Nk / 2.0 * c->P.nfilling;
+  o.nF = (i64)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5));
+  o.EF = (key[o.nF - 1] + key[o.nF]) / 2.0;
+  o.nOcc = o.nPart = 0;
+  for (i64 q = 0; q < N; ++q) {
+    if ((key[q] - o.EF) < -c->P.energy_tol) ++o.nOcc;
+    else if (std::fabs(key[q] - o.EF) < c->P.energy_tol) { ++o.nOcc; ++o.nPart; }
+  }
+  o.Deg = 1.0 / (double)o.nPart * (double)(o.nF + o.nPart - o.nOcc);
+  std::vector<double> w(N, 0.0), wi(N, 0.0);
+  if (c->opt.occ_mode == 2) {
+    double kT = c->opt.kT > 0 ? c->opt.kT : 1e-3;
+    double lo = key[0] - 50 * kT - 1.0, hi = key[N - 1] + 50 * kT + 1.0, mu = 0;
+    for (int it = 0; it < 200; ++it) {
+      mu = 0.5 * (lo + hi);
+      double s = 0;
+      for (i64 q = 0; q < N; ++q) s += 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
+      if (s > (double)o.nF) hi = mu; else lo = mu;
+    }
+    o.EF = mu;
+    for (i64 q = 0; q < N; ++q) w[q] = 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
+  } else {
+    for (i64 q = 0; q < o.nOcc && q < N; ++q) {
+      if (q < o.nOcc - o.nPart) w[q] = 1.0;
+      else if (c->opt.occ_mode == 0) { w[q] = (double)(float)o.Deg; wi[q] = 8.0; }
+      else w[q] = o.Deg;
+    }
+  }
+  o.idx.clear();
+  o.wph.clear();
+  for (i64 q = 0; q < N; ++q) {
+    if (w[q] == 0.0 && wi[q] == 0.0) continue;
+    i64 src = (i64)std::llround(val[q]);
+    i64 indev = (src - 1) / Nk + 1, k = src - (indev - 1) * Nk;  // band indev, k-point k (1-based)
+    o.idx.push_back((int)((indev - 1) + nb * (k - 1)));
+    i64 i1 = c->nMomComp[k - 1], i2 = c->nMomComp[k - 1 + Nk];
+    for (i64 cc = 0; cc < nnc; ++cc) {
+      double a = -2.0 * PI * (double)(i1 * c->u1[cc] + i2 * c->u1[cc] + i2 * c->u2[cc]) / (double)c->numk;
+      std::complex<double> z = std::complex<double>(std::cos(a), std::sin(a)) * std::complex<double>(w[q], wi[q]);
+      o.wph.push_back(z.real());
+      o.wph.push_back(z.imag());
+    }
+  }
+}
+
+extern "C" int forge_getfock(forge_ctx* c, forge_z* zFock, const forge_z* zSorted,
+                             const int64_t* nSortedMomenta, double DegFactor, int64_t OccStates,
+                             int64_t PartOccStates) {
+  if (!c || !zFock || !zSorted || !nSortedMomenta) { set_error("forge_getfock: null argument"); return -2; }
+  const i64 n = c->n, nnc = c->nnc, N = c->nb * c->Nk;
+  if (OccStates < 0 || OccStates > N || PartOccStates < 0 || PartOccStates > OccStates) {
+    set_error("forge_getfock: invalid state counts");
+    return -2;
+  }
+  const size_t nn = (size_t)n * n;
+  std::vector<double> wph;
+  for (i64 s = 0; s < OccStates; ++s) {
+    std::complex<double> w = 1.0;
+    if (s >= OccStates - PartOccStates)
+      w = c->opt.occ_mode == 0 ? std::complex<double>((double)(float)DegFactor, 8.0)
+                               : std::complex<double>(DegFactor, 0.0);
+    i64 i1 = nSortedMomenta[s], i2 = nSortedMomenta[s + N];
+    for (i64 cc = 0; cc < nnc; ++cc) {
+      double a = -2.0 * PI * (double)(i1 * c->u1[cc] + i2 * c->u1[cc] + i2 * c->u2[cc]) / (double)c->numk;
+      std::complex<double> z = std::complex<double>(std::cos(a), std::sin(a)) * w;
+      wph.push_back(z.real());
+      wph.push_back(z.imag());
+    }
+  }
+  FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  if (OccStates > 0) {
+    FORGE_HIP(hipMemcpyAsync(c->Psi, zSorted, sizeof(z2) * n * OccStates, hipMemcpyHostToDevice, c->s));
+    FORGE_HIP(hipMemcpyAsync(c->wph, wph.data(), sizeof(double) * wph.size(), hipMemcpyHostToDevice, c->s));
+    CK(window_density(n, (int)nnc, OccStates, c->Psi, c->wph, c->rho_new, 1.0, c->s));
+  }
+  FORGE_HIP(hipMemcpyAsync(zFock, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->scf_ready = false;
+  return 0;
+}
+
+// Energy pieces of a = rho (dens da) and optionally b = rho_new (dens db):
+// pieces[0..4] = K_a, K_b, F_aa, F_ab, F_bb; [5..7] = H_aa, H_ab, H_bb
+// (d.L.d), [8..10] = U_aa, U_ab, U_bb; [11..12] bias terms of a, b.
+static int energy_pieces(forge_ctx* c, const z2* a, const z2* b, double* pieces) {
+  const i64 n = c->n;
+  const size_t nn = (size_t)n * n;
+  CK(ensure_longrange(c));
+  const int nblk = 1024;
+  CK(energy_pass(nn, (int)c->nnc, a, b, c->V, c->T, c->part, nblk, c->e5, c->s));
+  CK(hartree_matvec(n, c->L, c->da, b ? c->db : nullptr, c->ya, b ? c->yb : nullptr, c->s));
+  hipLaunchKernelGGL(k_dots, dim3(1), dim3(256), 0, c->s, n, c->da, b ? c->db : nullptr, c->ya,
+                     b ? c->yb : nullptr, c->da, c->dots);
+  FORGE_LAUNCH();
+  hipLaunchKernelGGL(k_bias_dot, dim3(1), dim3(256), 0, c->s, n, c->da, b ? c->db : nullptr, c->bias,
+                     c->dots + 8);
+  FORGE_LAUNCH();
+  double h5[5], hd[10];
+  FORGE_HIP(hipMemcpyAsync(h5, c->e5, sizeof(h5), hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(hd, c->dots, sizeof(hd), hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  for (int q = 0; q < 5; ++q) pieces[q] = h5[q];
+  pieces[5] = hd[0];
+  pieces[6] = hd[1];
+  pieces[7] = hd[2];
+  pieces[8] = hd[3];
+  pieces[9] = hd[4];
+  pieces[10] = hd[5];
+  pieces[11] = hd[8];
+  pieces[12] = hd[9];
+  return 0;
+}
+
+// energies (Ekin, EF, EH, EU) of (1-l) a + l b from the pieces
+static void energies_at(const double* pc, double l, double* e) {
+  double u = 1.0 - l;
+  e[0] = u * (pc[0] + pc[11]) + l * (pc[1] + pc[12]);
+  e[1] = u * u * pc[2] + 2.0 * u * l * pc[3] + l * l * pc[4];
+  e[2] = 2.0 * (u * u * pc[5] + 2.0 * u * l * pc[6] + l * l * pc[7]);
+  e[3] = u * u * pc[8] + 2.0 * u * l * pc[9] + l * l * pc[10];
+}
+
+extern "C" int forge_energies(forge_ctx* c, const forge_z* zFock, const double* dens, double* out4) {
+  if (!c || !zFock || !dens || !out4) { set_error("forge_energies: null argument"); return -2; }
+  const size_t nn = (size_t)c->n * c->n;
+  FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * c->nnc * nn, hipMemcpyHostToDevice, c->s));
+  FORGE_HIP(hipMemcpyAsync(c->da, dens, sizeof(double) * c->n, hipMemcpyHostToDevice, c->s));
+  double pc[13];
+  CK(energy_pieces(c, c->rho_new, nullptr, pc));
+  energies_at(pc, 0.0, out4);
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_scf_set_options(forge_ctx* c, const forge_scf_options* opt) {
+  if (!c || !opt) { set_error("forge_scf_set_options: null argument"); return -2; }
+  if (opt->occ_mode < 0 || opt->occ_mode > 2 || opt->mix_mode < 0 || opt->mix_mode > 1 ||
+      (opt->mix_mode == 1 && !(opt->beta > 0.0 && opt->beta <= 1.0)) ||
+      (opt->occ_mode == 2 && !(opt->kT > 0.0))) {
+    set_error("forge_scf_set_options: invalid option values");
+    return -2;
+  }
+  if (opt->precision != 0) {
+    set_error("forge_scf_set_options: only precision 0 (FP64) is available");
+    return -2;
+  }
+  c->opt = *opt;
+  return 0;
+}
+
+// window density into c->rho_new (which holds the bulk): rho_new = (bulk + win) / Nk
+static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation& occ) {
+  occupations(c, band, occ);
+  const i64 n = c->n, S = (i64)occ.idx.size();
+  if (S > 0) {
+    FORGE_HIP(hipMemcpyAsync(c->gidx, occ.idx.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->s));
+    FORGE_HIP(hipMemcpyAsync(c->wph, occ.wph.data(), sizeof(double) * occ.wph.size(), hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(k_gather_cols, dim3(cdiv(n * S, 256)), dim3(256), 0, c->s, n, S, c->gidx,
+                       c->Cwin, c->Psi);
+    FORGE_LAUNCH();
+  }
+  CK(window_density(n, (int)c->nnc, S, c->Psi, c->wph, c->rho_new, 1.0 / (double)c->Nk, c->s));
+  return 0;
+}
+
+extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
+  if (!c) { set_error("forge_scf_init: null context"); return -2; }
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  std::fill(c->tm, c->tm + 16, 0.0);
+  int fails = 0;
+  if (rho_start) {
+    FORGE_HIP(hipMemcpyAsync(c->rho, rho_start, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  } else {
+    // first diagonalisation: rho = 0, Potential = 0 (Main.f90:238-282)
+    FORGE_HIP(hipMemsetAsync(c->pot, 0, sizeof(double) * n, c->s));
+    std::vector<double> band;
+    fails = solve_device(c, nullptr, c->pot, band, false);
+    if (fails < 0) return fails;
+    Occupation occ;
+    CK(window_step(c, band, occ));
+    FORGE_HIP(hipMemcpyAsync(c->rho, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToDevice, c->s));
+  }
+  CK(site_density(n, c->rho, c->da, c->s));
+  double pc[13];
+  CK(energy_pieces(c, c->rho, nullptr, pc));
+  energies_at(pc, 0.0, c->e);
+  c->Etot = total_energy(c->P, c->e);
+  hipLaunchKernelGGL(k_potential, dim3(cdiv(n, 256)), dim3(256), 0, c->s, n, c->ya, nullptr, c->da,
+                     nullptr, 0.0, c->P.alphaH, c->P.U, c->pot, nullptr);
+  FORGE_LAUNCH();
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->it = 0;
+  c->scf_ready = true;
+  return fails;
+}
+
+extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64_t* iters_done) {
+  if (!c || max_iter < 0) { set_error("forge_scf_run: invalid argument"); return -2; }
+  if (!c->scf_ready) { set_error("forge_scf_run: call forge_scf_init first"); return -5; }
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  std::fill(c->tm, c->tm + 16, 0.0);
+  const bool timed = c->opt.timing != 0;
+  int fails = 0;
+  i64 done = 0;
+  std::vector<double> hda(n), hdb(n);
+  FORGE_HIP(hipEventRecord(c->ev[8], c->s));
+  for (i64 itn = 0; itn < max_iter; ++itn) {
+    std::vector<double> band;
+    int f = solve_device(c, c->rho, c->pot, band, timed);
+    if (f < 0) return f;
+    fails += f;
+    if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
+    Occupation occ;
+    CK(window_step(c, band, occ));
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[11], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[11]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[10], c->ev[11]);
+      c->tm[4] += a;
+      FORGE_HIP(hipEventRecord(c->ev[12], c->s));
+    }
+    CK(site_density(n, c->rho_new, c->db, c->s));
+    double pc[13];
+    CK(energy_pieces(c, c->rho, c->rho_new, pc));
+    double ein[4] = {c->e[0], c->e[1], c->e[2], c->e[3]};
+    const double EtotIn = c->Etot;
+    double step;
+    double e1[4], eh[4];
+    energies_at(pc, 1.0, e1);
+    energies_at(pc, 0.5, eh);
+    if (c->opt.mix_mode == 0) {
+      // OptimalStep, HartreeFock.f90:1162-1208
+      double c0 = EtotIn, E1 = total_energy(c->P, e1), Eh = total_energy(c->P, eh);
+      double c1 = 4.0 * Eh - E1 - 3.0 * c0, c2 = E1 - c0 - c1;
+      if (c2 > 0 && std::fabs(-c1 / 2.0 / c2 - 0.5) < 0.5) step = -c1 / 2.0 / c2;
+      else if (E1 < c0) step = 1.0;
+      else step = c->P.step_alt;
+    } else {
+      step = c->opt.beta;
+    }
+    energies_at(pc, step, c->e);
+    c->Etot = total_energy(c->P, c->e);
+    // DensityConvergence = max |D_mixed - D_in| (Main.f90:564-571)
+    FORGE_HIP(hipMemcpyAsync(hda.data(), c->da, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    FORGE_HIP(hipMemcpyAsync(hdb.data(), c->db, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    CK(mix(nnc * nn, c->rho, c->rho_new, step, c->s));
+    hipLaunchKernelGGL(k_potential, dim3(cdiv(n, 256)), dim3(256), 0, c->s, n, c->ya, c->yb, c->da,
+                       c->db, step, c->P.alphaH, c->P.U, c->pot, c->da);
+    FORGE_LAUNCH();
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    double dconv = 0;
+    for (i64 q = 0; q < n; ++q) {
+      double dmix = (1.0 - step) * hda[q] + step * hdb[q];
+      dconv = std::max(dconv, std::fabs(dmix - hda[q]));
+    }
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[13], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[13]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[12], c->ev[13]);
+      c->tm[5] += a;
+    }
+    ++c->it;
+    ++done;
+    if (traj) {
+      double* r = traj + 12 * itn;
+      r[0] = (double)c->it; r[1] = EtotIn; r[2] = 2 * ein[0]; r[3] = 2 * ein[1]; r[4] = ein[2];
+      r[5] = ein[3]; r[6] = dconv; r[7] = step; r[8] = EtotIn - c->Etot; r[9] = occ.EF;
+      r[10] = (double)occ.nOcc; r[11] = (double)occ.nPart;
+    }
+    if (c->opt.etol > 0 && std::fabs(EtotIn - c->Etot) < c->opt.etol) break;
+  }
+  FORGE_HIP(hipEventRecord(c->ev[9], c->s));
+  FORGE_HIP(hipEventSynchronize(c->ev[9]));
+  float tt;
+  hipEventElapsedTime(&tt, c->ev[8], c->ev[9]);
+  c->tm[6] = tt;
+  if (iters_done) *iters_done = done;
+  return fails;
+}
+
+extern "C" int forge_scf_get_density(forge_ctx* c, forge_z* rho) {
+  if (!c || !rho) { set_error("forge_scf_get_density: null argument"); return -2; }
+  FORGE_HIP(hipMemcpyAsync(rho, c->rho, sizeof(z2) * c->nnc * c->n * c->n, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+extern "C" int forge_scf_timings(forge_ctx* c, double* out16) {
+  if (!c || !out16) { set_error("forge_scf_timings: null argument"); return -2; }
+  for (int q = 0; q < 16; ++q) out16[q] = c->tm[q];
+  return 0;
+}

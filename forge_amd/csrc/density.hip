@@ -1,0 +1,141 @@
+// Density-matrix products of Solve / GetFock (HartreeFock.f90:456-475, 487-560).
+//
+// Bulk (bands 1..nLower-1 of every k): the reference accumulates nLower-1
+// rank-1 zgemm updates per k, T(i,j) = sum_b conj(psi_b(i)) psi_b(j), and adds
+// T e^{-2 pi i (i1 n1 + i2 (n1 + n2)) / numk} to every stored cell.  Here
+// R_k = C_occ C_occ^H is one rank-nocc Hermitian product per k (rocBLAS zherk,
+// lower), T = conj(R_k), and k_bulk_accumulate adds the phased copies to all
+// cells in one pass (both triangles from the lower one).
+//
+// Window (sorted states s of the 20-band window): the reference adds
+// w_s e^{-i k_s.T_c} conj(psi_s) psi_s^T per state and cell with w_s = 1 for
+// occupied states and zdeg for the degenerate shell.  k_window_density forms
+// all cells for all states in one LDS-tiled pass and writes
+// rho_new = (bulk + window) / Nk (Main.f90:510-511).
+#include <rocblas/rocblas.h>
+
+#include "kernels.hpp"
+
+namespace forge {
+
+// rho_c(i,j) += ph_c * conj(R(i,j)), R Hermitian with the lower triangle valid.
+// ph: nnc complex phases for this k.
+__global__ void __launch_bounds__(256) k_bulk_accumulate(i64 n, int nnc, const z2* __restrict__ R,
+                                                         const double* __restrict__ ph,
+                                                         z2* __restrict__ rho, int first) {
+  const size_t nn = (size_t)n * n;
+  __shared__ z2 tr[32][33];
+  const i64 I = blockIdx.y, J = blockIdx.x;  // 32x32 tile (rows I, cols J)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // stage R(J-tile rows, I-tile cols) for the upper-triangle reflection
+  for (int q = 0; q < 4; ++q) {
+    int cc = ty * 4 + q;
+    i64 r = J * 32 + tx, c = I * 32 + cc;
+    tr[tx][cc] = (r < n && c < n) ? R[r + n * c] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int q = 0; q < 4; ++q) {
+    int cc = ty * 4 + q;
+    i64 i = I * 32 + tx, j = J * 32 + cc;
+    if (i >= n || j >= n) continue;
+    z2 Rij = (i >= j) ? R[i + n * j] : conjz(tr[cc][tx]);  // R(i,j) = conj(R(j,i)) above
+    z2 T = conjz(Rij);
+    size_t o = i + n * j;
+    for (int c = 0; c < nnc; ++c) {
+      z2 p = mk(ph[2 * c], ph[2 * c + 1]);
+      z2 add = p * T;
+      rho[c * nn + o] = first ? add : rho[c * nn + o] + add;
+    }
+  }
+}
+
+rocblas_handle blas_handle(hipStream_t s);
+
+// rho (nnc + 1 matrices: the cells, then the R scratch) receives
+// sum_k ph_c(k) conj(R_k); accumulate = false overwrites at the first k.
+int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
+                 const double* phase_cells, z2* rho, bool accumulate, hipStream_t s) {
+  const size_t nn = (size_t)n * n;
+  z2* R = rho + nnc * nn;
+  rocblas_handle h = blas_handle(s);
+  if (!h) return -1;
+  const double one = 1.0, zero = 0.0;
+  dim3 grid(cdiv(n, 32), cdiv(n, 32));
+  for (int k = 0; k < K; ++k) {
+    if (nocc > 0) {
+      rocblas_status st = rocblas_zherk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)n,
+                                        (rocblas_int)nocc, &one,
+                                        (const rocblas_double_complex*)(C + k * sC), (rocblas_int)ldC,
+                                        &zero, (rocblas_double_complex*)R, (rocblas_int)n);
+      if (st != rocblas_status_success) {
+        set_error(std::string("rocblas_zherk: ") + rocblas_status_to_string(st));
+        return -1;
+      }
+    } else {
+      FORGE_HIP(hipMemsetAsync(R, 0, sizeof(z2) * nn, s));
+    }
+    hipLaunchKernelGGL(k_bulk_accumulate, grid, dim3(256), 0, s, n, nnc, R,
+                       phase_cells + (size_t)k * nnc * 2, rho, (k == 0 && !accumulate) ? 1 : 0);
+    FORGE_LAUNCH();
+  }
+  return 0;
+}
+
+// rho_out_c(i,j) = scale * (bulk_c(i,j) + sum_s wph[s][c] conj(Psi(i,s)) Psi(j,s)).
+// Psi: n x S (ld n); wph: [S][nnc][2] complex weights (occupation x cell phase).
+// 32x32 output tiles, states staged 16 at a time.  bulk and out may alias.
+__global__ void __launch_bounds__(256) k_window_density(i64 n, int nnc, i64 S,
+                                                        const z2* __restrict__ Psi,
+                                                        const double* __restrict__ wph,
+                                                        const z2* bulk, z2* out, double scale) {
+  const size_t nn = (size_t)n * n;
+  const i64 I = blockIdx.y, J = blockIdx.x;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  __shared__ z2 Pi[16][33], Pj[16][33];
+  __shared__ z2 W[16][MAXCELL];
+  z2 acc[4][MAXCELL];
+  for (int q = 0; q < 4; ++q)
+    for (int c = 0; c < MAXCELL; ++c) acc[q][c] = mk(0.0, 0.0);
+  for (i64 s0 = 0; s0 < S; s0 += 16) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < 16 * 32; q += 256) {
+      int ss = q >> 5, rr = q & 31;
+      i64 sidx = s0 + ss;
+      i64 ri = I * 32 + rr, rj = J * 32 + rr;
+      Pi[ss][rr] = (sidx < S && ri < n) ? conjz(Psi[ri + n * sidx]) : mk(0.0, 0.0);
+      Pj[ss][rr] = (sidx < S && rj < n) ? Psi[rj + n * sidx] : mk(0.0, 0.0);
+    }
+    if (threadIdx.x < 16 * nnc) {
+      int ss = threadIdx.x / nnc, c = threadIdx.x % nnc;
+      i64 sidx = s0 + ss;
+      W[ss][c] = sidx < S ? mk(wph[(sidx * nnc + c) * 2], wph[(sidx * nnc + c) * 2 + 1])
+                          : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int ss = 0; ss < 16; ++ss) {
+      for (int q = 0; q < 4; ++q) {
+        z2 pr = Pi[ss][tx] * Pj[ss][ty * 4 + q];
+        for (int c = 0; c < nnc; ++c) acc[q][c] = acc[q][c] + W[ss][c] * pr;
+      }
+    }
+  }
+  for (int q = 0; q < 4; ++q) {
+    i64 i = I * 32 + tx, j = J * 32 + ty * 4 + q;
+    if (i >= n || j >= n) continue;
+    size_t o = i + n * j;
+    for (int c = 0; c < nnc; ++c) {
+      z2 b = bulk ? bulk[c * nn + o] : mk(0.0, 0.0);
+      out[c * nn + o] = scale * (b + acc[q][c]);
+    }
+  }
+}
+
+int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
+                   double scale, hipStream_t s) {
+  dim3 grid(cdiv(n, 32), cdiv(n, 32));
+  hipLaunchKernelGGL(k_window_density, grid, dim3(256), 0, s, n, nnc, S, Psi, wph, rho, rho, scale);
+  FORGE_LAUNCH();
+  return 0;
+}
+
+}  // namespace forge

@@ -1,0 +1,78 @@
+// Internal kernel entry points (host wrappers) of libforge_hip.
+#pragma once
+#include "common.hpp"
+
+namespace forge {
+
+constexpr int MAXCELL = 8;
+
+struct CellShifts {
+  double n1[MAXCELL], n2[MAXCELL];
+  double t1x, t1y, t2x, t2y;
+};
+
+struct DeviceGeom {
+  i64 n;
+  int nnc;
+  const double *X, *Y, *Z;
+  CellShifts cells;
+};
+
+// operators.hip
+int pair_tables(const DeviceGeom& g, const PairParams& pp, double* V, double* T, hipStream_t s);
+int longrange(const DeviceGeom& g, const PairParams& pp, int numC, double* L, hipStream_t s);
+int assemble(i64 n, int nnc, int K, const z2* rho, const double* V, const double* T,
+             const double* pot, const double* bias, double alpha, const double* phase, z2* H,
+             size_t ldH, size_t strideH, hipStream_t s);
+int energy_pass(size_t nn, int nnc, const z2* a, const z2* b, const double* V, const double* T,
+                double* part, int nblocks, double* out5, hipStream_t s);
+int hartree_matvec(i64 n, const double* L, const double* da, const double* db, double* ya,
+                   double* yb, hipStream_t s);
+int mix(size_t tot, z2* a, const z2* b, double s, hipStream_t st);
+int site_density(i64 n, const z2* rho, double* d, hipStream_t s);
+
+// tridiag.hip: batched Hermitian -> real symmetric tridiagonal (lower storage).
+struct TridiagWork {
+  i64 n;
+  int K;
+  z2* A;          // K matrices, column-major, lda = n, stride sA
+  size_t sA;
+  double* d;      // K x n
+  double* e;      // K x n
+  z2* tau;        // K x n
+  z2* W;          // K x n x NB
+  z2* P;          // hemv partials: K x (nt*nt*TS)   (y_I from tile (I,J))
+  z2* P2;         // hemv partials: K x (nt*maxseg*TS) (y_J from segment s)
+  z2* tpart;      // K x nt x 2NB
+  z2* t12;        // K x 2NB
+  double* npart;  // K x nblk (column norm partials, then alpha partials)
+  z2* scal;       // K x 4: tau, beta(+0i), scale, alpha
+  unsigned* cnt;  // K x 4 completion counters (zero on entry, left zero)
+  hipEvent_t* evs;  // 2 per column when hemv launches are timed (created lazily)
+  i64 nev;
+};
+constexpr int TRD_NB = 32;  // panel width
+constexpr int TRD_TS = 32;  // hemv tile edge
+constexpr int TRD_SEG = 8;  // tiles per hemv segment
+size_t tridiag_workspace_bytes(i64 n, int K);
+int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
+int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches);
+
+// dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
+struct DCWork;
+size_t dc_workspace_bytes(i64 n, int K, i64 mout);
+int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* evals,
+             double* Q, void* work, hipStream_t s);
+
+// backtr.hip: C(n x m, complex) = Q_house * Z(n x m, real), Q from tridiag.
+size_t backtr_workspace_bytes(i64 n, int K, i64 m);
+int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
+                  size_t ldC, size_t sC, void* work, hipStream_t s);
+
+// density.hip
+int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
+                 const double* phase_cells, z2* rho, bool accumulate, hipStream_t s);
+int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
+                   double scale, hipStream_t s);
+
+}  // namespace forge

@@ -1,0 +1,193 @@
+"""Python host of the MI355X Hartree-Fock engine.
+
+`Engine` mirrors the reference's hot-path subroutines (HartreeFock.f90,
+TightBinding.f90, LapackRoutines.f90) with the same argument meaning, array
+layout (Fortran order, complex128) and unnormalised outputs, on top of the
+C ABI in include/forge_hf.h.  `scf_*` drive the device-resident loop of
+Main.f90:236-679.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from .geometry import Geometry, build_geometry, ndim_of
+
+
+@dataclass
+class Setup:
+    """Runtime form of Setup.f90's parameters."""
+    ntheta: int = 5
+    nlayers: int = 2
+    numk: int = 6
+    nfilling: float = 0.0
+    U: float = 4.0
+    epsilon: float = 10.0
+    Delta: float = 0.0
+    xi_nm: float = 40.0        # xi * 0.246: Setup.f90:22 expresses xi in units of a
+    numI: int = 1
+    numC: int = 3
+    nscreen: int = 2
+    energy_tol: float = 1e-4   # EnergyTolerance, Setup.f90:51
+    step_alt: float = 1.0      # StepAlternative, Setup.f90:52
+
+    @property
+    def ndim(self):
+        return ndim_of(self.ntheta, self.nlayers)
+
+    def params(self, nnc: int) -> L.ForgeParams:
+        n = self.ndim
+        numb = min(20, n)
+        ncb = numb // 2
+        p = L.ForgeParams()
+        p.ndim, p.nlayers, p.numk, p.numI, p.numC = n, self.nlayers, self.numk, self.numI, self.numC
+        p.nscreen, p.fphase, p.nnc = self.nscreen, 0, nnc
+        p.nLower = (n - 2 * numb + 2 * ncb) // 2 + 1     # Setup.f90:67
+        p.nUpper = (n + 2 * ncb) // 2                    # Setup.f90:68
+        p.numb = numb
+        p.nfilling, p.U = self.nfilling, self.U
+        p.alpha = p.alphaH = 5.853 / self.epsilon * 1.0  # Setup.f90:58-59
+        p.Delta, p.xi, p.tz = self.Delta, self.xi_nm / 0.246, 1.35772
+        p.a0, p.r0, p.d0 = 1.0 / np.sqrt(3.0), 0.184, 1.35772
+        p.energy_tol, p.step_alt = self.energy_tol, self.step_alt
+        return p
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+class Engine:
+    def __init__(self, setup: Setup, geom: Geometry | None = None):
+        self.setup = setup
+        self.geom = geom or build_geometry(setup.ntheta, setup.nlayers, setup.numk, setup.numI)
+        g = self.geom
+        self.p = setup.params(g.nnc)
+        self.n = g.ndim
+        self.nnc = g.nnc
+        self.Nk = setup.numk ** 2
+        self.nb = self.p.nUpper - self.p.nLower + 1
+        self._keep = [g.Coords, g.tn, g.MomentaValues, g.nMomentaComponents, g.nUnitCell_1,
+                      g.nUnitCell_2]
+        self.lib = L.load()
+        ctx = C.c_void_p()
+        L.check(self.lib.forge_ctx_create(C.byref(self.p), _p(g.Coords), _p(g.tn),
+                                          _p(g.MomentaValues), _p(g.nMomentaComponents),
+                                          _p(g.nUnitCell_1), _p(g.nUnitCell_2), C.byref(ctx)),
+                "forge_ctx_create")
+        self.ctx = ctx
+
+    def close(self):
+        if self.ctx:
+            self.lib.forge_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference interfaces ------------------------------------------------
+    def longrange(self):
+        """LongRangeInteraction (HartreeFock.f90:14-53)."""
+        out = np.zeros((self.n, self.n), order="F")
+        L.check(self.lib.forge_longrange(self.ctx, _p(out)), "forge_longrange")
+        return out
+
+    def tb_hamiltonian(self, vk):
+        """TightBindingHamiltonian (TightBinding.f90:13-55): lower triangle."""
+        zH = np.zeros((self.n, self.n), np.complex128, order="F")
+        v = _f(vk, np.float64)
+        L.check(self.lib.forge_tb_hamiltonian(self.ctx, _p(zH), _p(v)), "forge_tb_hamiltonian")
+        return zH
+
+    def hamiltonian_hf(self, vk, Potential, zFock):
+        """HamiltonianHartreeFock (HartreeFock.f90:57-127): lower triangle."""
+        zH = np.zeros((self.n, self.n), np.complex128, order="F")
+        v, pot, rho = _f(vk, np.float64), _f(Potential, np.float64), _f(zFock, np.complex128)
+        L.check(self.lib.forge_hamiltonian_hf(self.ctx, _p(zH), _p(pot), _p(rho), _p(v)),
+                "forge_hamiltonian_hf")
+        return zH
+
+    def solve(self, zFock, Potential):
+        """Solve (HartreeFock.f90:402-483): (zFockBulk unnormalised, Bands, zEigenvectors)."""
+        bulk = np.zeros((self.n, self.n, self.nnc), np.complex128, order="F")
+        bands = np.zeros((self.nb, self.Nk), order="F")
+        vecs = np.zeros((self.n, self.nb, self.Nk), np.complex128, order="F")
+        rho, pot = _f(zFock, np.complex128), _f(Potential, np.float64)
+        rc = L.check(self.lib.forge_solve(self.ctx, _p(bulk), _p(rho), _p(pot), _p(bands), _p(vecs)),
+                     "forge_solve")
+        if rc > 0:
+            print("zheev failed", rc)  # the reference's reaction, LapackRoutines.f90:430
+        return bulk, bands, vecs
+
+    def getfock(self, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, OccStates,
+                PartOccStates):
+        """GetFock (HartreeFock.f90:487-560): returns zFock + window density."""
+        out = np.array(zFock, dtype=np.complex128, order="F", copy=True)
+        vecs = _f(zSortedEigenvectors, np.complex128)
+        mom = _f(nSortedMomenta, np.int64)
+        L.check(self.lib.forge_getfock(self.ctx, _p(out), _p(vecs), _p(mom), float(DegFactor),
+                                       int(OccStates), int(PartOccStates)), "forge_getfock")
+        return out
+
+    def energies(self, zFock, dens):
+        """(Ekin, EF, EH, EU) of GetKineticEnergy/GetFockEnergy/GetHartreeHubbardEnergy."""
+        out = np.zeros(4)
+        rho, d = _f(zFock, np.complex128), _f(dens, np.float64)
+        L.check(self.lib.forge_energies(self.ctx, _p(rho), _p(d), _p(out)), "forge_energies")
+        return out
+
+    def total_energy(self, e):
+        p = self.p
+        return p.alpha * e[1] * 2.0 + e[0] * 2.0 + p.U * e[3] + p.alphaH * e[2]
+
+    # -- device-resident SCF -----------------------------------------------
+    def set_options(self, occ_mode=0, mix_mode=0, kT=0.0, beta=0.3, etol=0.0, precision=0,
+                    timing=0):
+        o = L.ScfOptions(occ_mode, mix_mode, kT, beta, etol, precision, timing)
+        L.check(self.lib.forge_scf_set_options(self.ctx, C.byref(o)), "forge_scf_set_options")
+
+    def scf_init(self, rho_start=None):
+        rho = None if rho_start is None else _f(rho_start, np.complex128)
+        return L.check(self.lib.forge_scf_init(self.ctx, _p(rho)), "forge_scf_init")
+
+    def scf_run(self, max_iter):
+        traj = np.zeros((max(max_iter, 1), 12))
+        done = C.c_int64(0)
+        rc = L.check(self.lib.forge_scf_run(self.ctx, int(max_iter), _p(traj), C.byref(done)),
+                     "forge_scf_run")
+        if rc > 0:
+            print("zheev failed", rc)
+        return traj[:done.value]
+
+    def density(self):
+        rho = np.zeros((self.n, self.n, self.nnc), np.complex128, order="F")
+        L.check(self.lib.forge_scf_get_density(self.ctx, _p(rho)), "forge_scf_get_density")
+        return rho
+
+    def timings(self):
+        out = np.zeros(16)
+        L.check(self.lib.forge_scf_timings(self.ctx, _p(out)), "forge_scf_timings")
+        return out
+
+
+def zdiagred(A, il, iu):
+    """zdiagRed (LapackRoutines.f90:381-433) on the GPU: eigenpairs il..iu of the
+    Hermitian matrix whose lower triangle A holds -> (eigenvalues, vectors, info)."""
+    lib = L.load()
+    n = A.shape[0]
+    a = np.array(A, dtype=np.complex128, order="F", copy=True)
+    w = np.zeros(n)
+    info = C.c_int32(0)
+    L.check(lib.forge_zdiagred(_p(a), n, _p(w), int(il), int(iu), C.byref(info)), "forge_zdiagred")
+    m = iu - il + 1
+    return w[:m], a[:, :m], info.value

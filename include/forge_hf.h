@@ -1,0 +1,139 @@
+/*
+ * forge_hf.h -- C ABI of the MI355X Hartree-Fock engine (libforge_hip.so).
+ *
+ * The entry points replace the hot-path subroutines of the reference FORGE
+ * code; each declaration names the reference interface it stands in for.
+ * Conventions (the reference's, so a Fortran host binds them with
+ * ISO_C_BINDING and passes its arrays unchanged):
+ *   - arrays are column-major (Fortran order) host arrays;
+ *   - int64_t    = integer(dp)   (Setup.f90:6, dp = 8)
+ *   - double     = real(dp)
+ *   - forge_z    = complex(dp) = double _Complex (two doubles, re then im)
+ *   - return 0 on success, > 0 an eigensolver failure count (LAPACK "info"
+ *     meaning), < 0 an argument or HIP runtime error (forge_last_error()
+ *     describes it).
+ * Module-level constants of Setup.f90 are passed once in forge_params.
+ * Calls on one context must come from one host thread at a time.
+ */
+#ifndef FORGE_HF_H
+#define FORGE_HF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } forge_z;
+
+/* Runtime form of the Setup.f90 parameters (Setup.f90:6-74). */
+typedef struct {
+  int64_t ndim, nlayers, numk, numI, numC, nscreen, fphase, nnc;
+  int64_t nLower, nUpper, numb;
+  double nfilling, U, alpha, alphaH, Delta, xi, tz, a0, r0, d0;
+  double energy_tol, step_alt;
+} forge_params;
+
+/* SCF options beyond the reference (defaults reproduce the reference:
+ * occ_mode 0, mix_mode 0, etol 0). */
+typedef struct {
+  int32_t occ_mode;   /* 0: T=0 rule with the reference's zdeg = (float(DegFactor), 8) weight
+                         (HartreeFock.f90:529); 1: T=0 rule with real DegFactor;
+                         2: Fermi-Dirac occupations at temperature kT */
+  int32_t mix_mode;   /* 0: optimal damping (OptimalStep, HartreeFock.f90:1148-1227); 1: linear */
+  double kT;          /* eV, occ_mode 2 */
+  double beta;        /* linear mixing weight, mix_mode 1 */
+  double etol;        /* > 0: forge_scf_run stops once |E_in - E_new| < etol (eV per cell) */
+  int32_t precision;  /* 0: FP64 everywhere; 1: FP64 reduction + tridiagonal solve, FP32-split
+                         back-transform and density products */
+  int32_t timing;     /* nonzero: per-stage HIP-event timings (forge_scf_timings) */
+} forge_scf_options;
+
+typedef struct forge_ctx forge_ctx;
+
+/* Last error message of the calling thread (static storage). */
+const char* forge_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int forge_device_count(void);
+
+/* Uploads the geometry (Coords n x 3, tn 2 x 3 = [t1 t2 t3], the k grid of
+ * SampleBZ, Geometry.f90:96-127, and the stored cells of OrderNeighborCells,
+ * Geometry.f90:40-91) and builds the k-independent pair tables fv/ft of
+ * every stored cell on the device.  The context owns all device memory. */
+int forge_ctx_create(const forge_params* p, const double* Coords, const double* tn,
+                     const double* MomentaValues, const int64_t* nMomentaComponents,
+                     const int64_t* nUnitCell_1, const int64_t* nUnitCell_2, forge_ctx** out);
+int forge_ctx_destroy(forge_ctx* ctx);
+
+/* LongRangeInteraction(LongRange, Coords, ndim, t1, t2), HartreeFock.f90:14-53. */
+int forge_longrange(forge_ctx* ctx, double* LongRange);
+
+/* TightBindingHamiltonian(zH, Delta, Coords, ..., vk, tn), TightBinding.f90:13-55.
+ * Lower triangle of zH (ndim x ndim) is set, the strict upper triangle is 0. */
+int forge_tb_hamiltonian(forge_ctx* ctx, forge_z* zH, const double* vk);
+
+/* HamiltonianHartreeFock(zH, Coords, Potential, alpha, Delta, zFock, ..., vk, tn),
+ * HartreeFock.f90:57-127.  zFock is ndim x ndim x nnc.  Lower triangle only. */
+int forge_hamiltonian_hf(forge_ctx* ctx, forge_z* zH, const double* Potential,
+                         const forge_z* zFock, const double* vk);
+
+/* zdiagRed(A, eigenvalues, 'V', il, iu), LapackRoutines.f90:381-433: the
+ * eigenpairs il..iu (1-based, ascending) of the Hermitian matrix whose lower
+ * triangle A (n x n) holds.  On return A(:, 1:iu-il+1) are the eigenvectors
+ * and eigenvalues(1:iu-il+1) the eigenvalues.  *info as zheevr's. */
+int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_t il, int64_t iu,
+                   int32_t* info);
+
+/* Solve(zFockBulk, zFock, Potential, alpha, Bands, zEigenvectors, ...),
+ * HartreeFock.f90:402-483: every k of the grid is assembled, diagonalised and
+ * its bulk density accumulated; zFockBulk (ndim x ndim x nnc) is the
+ * unnormalised sum over k, Bands (numb x Nk) and zEigenvectors
+ * (ndim x numb x Nk) hold the window nLower..nUpper. */
+int forge_solve(forge_ctx* ctx, forge_z* zFockBulk, const forge_z* zFock,
+                const double* Potential, double* Bands, forge_z* zEigenvectors);
+
+/* GetFock(zFock, zSortedEigenvectors, ndim, numb, numk, DegFactor, OccStates,
+ * PartOccStates, ..., nSortedMomenta), HartreeFock.f90:487-560.  Adds the
+ * window density to zFock (not cleared), with the reference's weights
+ * (occ_mode of the last forge_scf_set_options; 0 by default). */
+int forge_getfock(forge_ctx* ctx, forge_z* zFock, const forge_z* zSortedEigenvectors,
+                  const int64_t* nSortedMomenta, double DegFactor, int64_t OccStates,
+                  int64_t PartOccStates);
+
+/* GetKineticEnergy / GetFockEnergy / GetHartreeHubbardEnergy
+ * (HartreeFock.f90:1012-1145) in one pass: out = {Ekin, EF, EH, EU} for the
+ * density matrix zFock and site densities dens (Density - DensitySub). */
+int forge_energies(forge_ctx* ctx, const forge_z* zFock, const double* dens, double* out4);
+
+/* ---- device-resident SCF (the loop of Main.f90:364-679 on the device) ---- */
+
+int forge_scf_set_options(forge_ctx* ctx, const forge_scf_options* opt);
+
+/* Initial state: rho_start (ndim x ndim x nnc) as read from a checkpoint
+ * (Main.f90:187-235), or NULL for the first diagonalisation
+ * (Main.f90:238-288).  Computes densities, energies and the potential
+ * (Main.f90:290-346). */
+int forge_scf_init(forge_ctx* ctx, const forge_z* rho_start);
+
+/* Runs up to max_iter SCF iterations (Main.f90:365-679).  traj receives one
+ * row of 12 doubles per iteration: it, E_in, 2Ekin_in, 2EF_in, EH_in, EU_in,
+ * max|dDensity|, Step, E_in - E_new, E_Fermi, nOcc, nPart.  *iters_done is
+ * set.  Stops early once |E_in - E_new| < opt.etol when etol > 0. */
+int forge_scf_run(forge_ctx* ctx, int64_t max_iter, double* traj, int64_t* iters_done);
+
+/* Copies the current density matrix (ndim x ndim x nnc) to the host. */
+int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
+
+/* Per-stage device time of the last forge_scf_run (milliseconds, HIP events):
+ * [0] assembly, [1] tridiagonal reduction, [2] tridiagonal eigensolver,
+ * [3] back-transformation, [4] density products, [5] energies + mixing,
+ * [6] total, [7] launches of the assembly kernel, [8] summed time of the
+ * assembly kernel (ms), [9] launches of the hemv kernel, [10] summed hemv
+ * kernel time (ms). */
+int forge_scf_timings(forge_ctx* ctx, double* out16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+"""CPU checks of the product: the C-ABI library loads and exports every
+symbol include/forge_hf.h declares, and the host-side geometry and Setup
+parameters reproduce the reference (no device calls)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from golden_io import case_params, load_case
+from forge_amd import lib as L
+from forge_amd.geometry import build_geometry, ndim_of, number_neighbor_cells, order_neighbor_cells
+from forge_amd.hf import Setup
+
+
+@pytest.fixture(scope="module")
+def built_lib():
+    if not os.path.exists(L.LIB_PATH):
+        from forge_amd.build import build
+        build()
+    return ctypes.CDLL(L.LIB_PATH)
+
+
+def test_library_exports_every_declared_symbol(built_lib):
+    names = L.header_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(built_lib, n)]
+    assert not missing, missing
+    assert set(names) == set(L._SIGS)
+
+
+def test_library_is_gfx950_and_reports_devices_without_computing(built_lib):
+    lib = L.load()
+    assert lib.forge_device_count() >= 0
+    # error text is reachable without a device
+    assert isinstance(L.last_error(), str)
+
+
+def test_ctx_create_rejects_bad_params_before_any_device_work():
+    lib = L.load()
+    p = Setup(ntheta=1, numk=3).params(4)
+    p.fphase = 1
+    ctx = ctypes.c_void_p()
+    z = np.zeros(64)
+    i = np.zeros(64, np.int64)
+    rc = lib.forge_ctx_create(ctypes.byref(p), z.ctypes.data, z.ctypes.data, z.ctypes.data,
+                              i.ctypes.data, i.ctypes.data, i.ctypes.data, ctypes.byref(ctx))
+    assert rc < 0 and "fphase" in L.last_error()
+
+
+@pytest.mark.parametrize("case", ["t1k3", "t2k3", "t3k6", "tri1k3"])
+def test_product_geometry_is_bitwise_reference(case):
+    f = load_case(case)
+    c = case_params(f)
+    g = build_geometry(c["ntheta"], c["nlayers"], c["numk"])
+    assert np.array_equal(g.Coords, f["Coords"])
+    assert np.array_equal(g.tn, f["tn"])
+    assert np.array_equal(g.MomentaValues, f["MomentaValues"])
+    assert np.array_equal(g.nMomentaComponents, f["nMomentaComponents"])
+    assert np.array_equal(g.nUnitCell_1, f["nUnitCell_1"])
+    assert np.array_equal(g.nUnitCell_2, f["nUnitCell_2"])
+
+
+def test_large_cells_have_the_configured_sizes():
+    assert ndim_of(5, 2) == 364 and ndim_of(20, 2) == 5044 and ndim_of(31, 2) == 11908
+    assert ndim_of(31, 3) == 17862
+    assert build_geometry(31, 2, 3).ndim == 11908
+    assert [ndim_of(m, 2) for m in range(8, 15)] == [868, 1084, 1324, 1588, 1876, 2188, 2524]
+
+
+def test_neighbor_cells_match_reference_counts():
+    assert number_neighbor_cells(1) == 4
+    u1, u2 = order_neighbor_cells(1, 4)
+    assert list(u1) == [0, -1, -1, 0] and list(u2) == [0, 0, 1, -1]
+    assert number_neighbor_cells(2) == 10
+
+
+def test_setup_band_window_matches_reference_params():
+    f = load_case("t1k3")
+    prm = f["params"]  # ndim, numk, nfilling, xi, nLower, nUpper, nnc, nlayers
+    p = Setup(ntheta=1, numk=3, nfilling=1.0).params(4)
+    assert (p.ndim, p.nLower, p.nUpper) == (int(prm[0]), int(prm[4]), int(prm[5]))
+    assert p.xi == pytest.approx(prm[3], rel=1e-15)
