@@ -300,7 +300,7 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
   const i64 nU = c->nU;
   size_t per_k = sizeof(z2) * nn + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
                  sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
-  size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + (size_t)64 << 20;
+  size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + ((size_t)64 << 20);
   size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
   i64 Kb = (i64)(budget / per_k);
   if (Kb < 1) {
