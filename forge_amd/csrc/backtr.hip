@@ -30,7 +30,7 @@ __global__ void k_bt_promote(size_t nm, const double* __restrict__ Z, size_t ldZ
 }
 
 // V(r', q) for rows r = i0+1+r' of reflector j = i0+q: 1 at r = j+1, A(r, j) below.
-__global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, size_t sA,
+__global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, size_t sA, i64 lda,
                             z2* __restrict__ V, size_t sV) {
   int k = blockIdx.y;
   i64 mt = n - i0 - 1;
@@ -40,7 +40,7 @@ __global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, siz
     i64 r = i0 + 1 + rr, j = i0 + c;
     z2 v;
     if (r == j + 1) v = mk(1.0, 0.0);
-    else if (r > j + 1) v = A[k * sA + r + n * j];
+    else if (r > j + 1) v = A[k * sA + r + lda * j];
     else v = mk(0.0, 0.0);
     V[k * sV + rr + (size_t)n * c] = v;
   }
@@ -106,7 +106,7 @@ int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, 
     const int nb = (int)((n - 1 - i0) < BT ? (n - 1 - i0) : BT);
     const i64 mt = n - i0 - 1;
     hipLaunchKernelGGL(k_bt_vclean, dim3(cdiv(mt * nb, 256) < 4096 ? cdiv(mt * nb, 256) : 4096, K),
-                       dim3(256), 0, s, n, i0, nb, t.A, t.sA, V, sV);
+                       dim3(256), 0, s, n, i0, nb, t.A, t.sA, t.lda, V, sV);
     FORGE_LAUNCH();
     // G = V^H V
     RB(rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none,

@@ -40,12 +40,16 @@ rocblas_handle blas_handle(hipStream_t s) {
 static const double PI = 3.14159265358979323846264338327950288;
 
 // small kernels local to the engine -----------------------------------------
-__global__ void k_zero_upper(i64 n, z2* H) {
+__global__ void k_zero_upper(i64 n, i64 ld, z2* H) {
   i64 q = (i64)blockIdx.x * 256 + threadIdx.x;
   if (q >= n * n) return;
   i64 r = q % n, c = q / n;
-  if (r < c) H[q] = mk(0.0, 0.0);
+  if (r < c) H[r + ld * c] = mk(0.0, 0.0);
 }
+
+// leading dimension of the eigensolver's matrices: a multiple of 16 elements
+// keeps every 16-row column chunk one aligned 256-byte segment
+static inline i64 pad_ld(i64 n) { return (n + 15) & ~(i64)15; }
 
 // out[0] = a.b, out[1] = a.c, out[2] = b.c, out[3] = a.a, out[4] = b.b,
 // out[5] = a.w, out[6] = b.w  (single block, fixed order)
@@ -143,6 +147,7 @@ struct forge_ctx {
   z2 *rho = nullptr, *rho_new = nullptr;  // rho_new has nnc+1 matrices (last = R scratch)
   // eigensolver batch
   int Kb = 0;
+  i64 ldH = 0;
   z2* H = nullptr;
   void* trd_ws = nullptr;
   TridiagWork trd{};
@@ -298,7 +303,7 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
   size_t fr = 0, tot = 0;
   FORGE_HIP(hipMemGetInfo(&fr, &tot));
   const i64 nU = c->nU;
-  size_t per_k = sizeof(z2) * nn + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
+  size_t per_k = sizeof(z2) * pad_ld(n) * n + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
                  sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
   size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + ((size_t)64 << 20);
   size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
@@ -309,11 +314,13 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
   }
   c->Kb = (int)std::min<i64>(Kb, Nk);
   const int Kb_ = c->Kb;
-  CK(dalloc((void**)&c->H, sizeof(z2) * nn * Kb_));
+  c->ldH = pad_ld(n);
+  CK(dalloc((void**)&c->H, sizeof(z2) * c->ldH * n * Kb_));
   CK(dalloc(&c->trd_ws, tridiag_workspace_bytes(n, Kb_)));
   CK(tridiag_bind(c->trd, c->trd_ws, n, Kb_));
   c->trd.A = c->H;
-  c->trd.sA = nn;
+  c->trd.sA = (size_t)c->ldH * n;
+  c->trd.lda = c->ldH;
   CK(dalloc(&c->dc_ws, dc_workspace_bytes(n, Kb_, nU)));
   CK(dalloc((void**)&c->Zq, sizeof(double) * n * nU * Kb_));
   CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
@@ -386,10 +393,11 @@ static int hamiltonian_one(forge_ctx* c, forge_z* zH, const double* Potential, c
   if (zFock)
     FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
   CK(assemble(n, (int)nnc, 1, zFock ? c->rho_new : nullptr, c->V, c->T, c->da, c->bias,
-              c->P.alpha, dph, c->H, n, nn, c->s));
-  hipLaunchKernelGGL(k_zero_upper, dim3(cdiv(n * n, 256)), dim3(256), 0, c->s, n, c->H);
+              c->P.alpha, dph, c->H, c->ldH, (size_t)c->ldH * n, c->s));
+  hipLaunchKernelGGL(k_zero_upper, dim3(cdiv(n * n, 256)), dim3(256), 0, c->s, n, c->ldH, c->H);
   FORGE_LAUNCH();
-  FORGE_HIP(hipMemcpyAsync(zH, c->H, sizeof(z2) * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpy2DAsync(zH, sizeof(z2) * n, c->H, sizeof(z2) * c->ldH, sizeof(z2) * n, n,
+                             hipMemcpyDeviceToHost, c->s));
   FORGE_HIP(hipStreamSynchronize(c->s));
   return 0;
 }
@@ -451,7 +459,8 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   const size_t nn = (size_t)n * n;
   z2* H = nullptr; void* tw = nullptr; void* dw = nullptr; double* Zq = nullptr; double* ev = nullptr;
   z2* Cv = nullptr; void* bw = nullptr;
-  CK(dalloc((void**)&H, sizeof(z2) * nn));
+  const i64 ld = pad_ld(n);
+  CK(dalloc((void**)&H, sizeof(z2) * ld * n));
   CK(dalloc(&tw, tridiag_workspace_bytes(n, 1)));
   CK(dalloc(&dw, dc_workspace_bytes(n, 1, iu)));
   CK(dalloc((void**)&Zq, sizeof(double) * n * iu));
@@ -461,8 +470,10 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   TridiagWork t{};
   CK(tridiag_bind(t, tw, n, 1));
   t.A = H;
-  t.sA = nn;
-  FORGE_HIP(hipMemcpyAsync(H, A, sizeof(z2) * nn, hipMemcpyHostToDevice, s));
+  t.sA = (size_t)ld * n;
+  t.lda = ld;
+  FORGE_HIP(hipMemcpy2DAsync(H, sizeof(z2) * ld, A, sizeof(z2) * n, sizeof(z2) * n, n,
+                             hipMemcpyHostToDevice, s));
   CK(tridiag(t, s, nullptr, nullptr));
   int fails = dc_solve(t.d, t.e, n, 1, iu, ev, Zq, dw, s);
   if (fails < 0) return fails;
@@ -493,7 +504,7 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
     const int K = (int)std::min<i64>(c->Kb, Nk - k0);
     if (timed) FORGE_HIP(hipEventRecord(c->ev[0], c->s));
     CK(assemble(n, (int)nnc, K, rho_dev, c->V, c->T, pot_dev, c->bias, c->P.alpha,
-                c->ph_asm + k0 * nnc * 2, c->H, n, nn, c->s));
+                c->ph_asm + k0 * nnc * 2, c->H, c->ldH, (size_t)c->ldH * n, c->s));
     if (timed) {
       FORGE_HIP(hipEventRecord(c->ev[1], c->s));
       FORGE_HIP(hipEventSynchronize(c->ev[1]));

@@ -84,7 +84,8 @@ int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int
 // rho_out_c(i,j) = scale * (bulk_c(i,j) + sum_s wph[s][c] conj(Psi(i,s)) Psi(j,s)).
 // Psi: n x S (ld n); wph: [S][nnc][2] complex weights (occupation x cell phase).
 // 32x32 output tiles, states staged 16 at a time.  bulk and out may alias.
-__global__ void __launch_bounds__(256) k_window_density(i64 n, int nnc, i64 S,
+template <int NNC>
+__global__ void __launch_bounds__(256) k_window_density(i64 n, i64 S,
                                                         const z2* __restrict__ Psi,
                                                         const double* __restrict__ wph,
                                                         const z2* bulk, z2* out, double scale) {
@@ -92,10 +93,13 @@ __global__ void __launch_bounds__(256) k_window_density(i64 n, int nnc, i64 S,
   const i64 I = blockIdx.y, J = blockIdx.x;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   __shared__ z2 Pi[16][33], Pj[16][33];
-  __shared__ z2 W[16][MAXCELL];
-  z2 acc[4][MAXCELL];
+  constexpr int nnc = NNC;
+  __shared__ z2 W[16][NNC];
+  z2 acc[4][NNC];
+#pragma unroll
   for (int q = 0; q < 4; ++q)
-    for (int c = 0; c < MAXCELL; ++c) acc[q][c] = mk(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < NNC; ++c) acc[q][c] = mk(0.0, 0.0);
   for (i64 s0 = 0; s0 < S; s0 += 16) {
     __syncthreads();
     for (int q = threadIdx.x; q < 16 * 32; q += 256) {
@@ -112,18 +116,23 @@ __global__ void __launch_bounds__(256) k_window_density(i64 n, int nnc, i64 S,
                           : mk(0.0, 0.0);
     }
     __syncthreads();
+#pragma unroll 4
     for (int ss = 0; ss < 16; ++ss) {
+#pragma unroll
       for (int q = 0; q < 4; ++q) {
         z2 pr = Pi[ss][tx] * Pj[ss][ty * 4 + q];
-        for (int c = 0; c < nnc; ++c) acc[q][c] = acc[q][c] + W[ss][c] * pr;
+#pragma unroll
+        for (int c = 0; c < NNC; ++c) acc[q][c] = acc[q][c] + W[ss][c] * pr;
       }
     }
   }
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
     i64 i = I * 32 + tx, j = J * 32 + ty * 4 + q;
     if (i >= n || j >= n) continue;
     size_t o = i + n * j;
-    for (int c = 0; c < nnc; ++c) {
+#pragma unroll
+    for (int c = 0; c < NNC; ++c) {
       z2 b = bulk ? bulk[c * nn + o] : mk(0.0, 0.0);
       out[c * nn + o] = scale * (b + acc[q][c]);
     }
@@ -133,7 +142,8 @@ __global__ void __launch_bounds__(256) k_window_density(i64 n, int nnc, i64 S,
 int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
                    double scale, hipStream_t s) {
   dim3 grid(cdiv(n, 32), cdiv(n, 32));
-  hipLaunchKernelGGL(k_window_density, grid, dim3(256), 0, s, n, nnc, S, Psi, wph, rho, rho, scale);
+  if (nnc != 4) { set_error("window_density: 4 stored cells (numI = 1) expected"); return -2; }
+  hipLaunchKernelGGL(k_window_density<4>, grid, dim3(256), 0, s, n, S, Psi, wph, rho, rho, scale);
   FORGE_LAUNCH();
   return 0;
 }

@@ -35,25 +35,25 @@ int site_density(i64 n, const z2* rho, double* d, hipStream_t s);
 struct TridiagWork {
   i64 n;
   int K;
-  z2* A;          // K matrices, column-major, lda = n, stride sA
+  z2* A;          // K matrices, column-major, leading dimension lda, stride sA
   size_t sA;
+  i64 lda;        // >= n; a multiple of 16 keeps 16-row column chunks 256-byte aligned
   double* d;      // K x n
   double* e;      // K x n
   z2* tau;        // K x n
   z2* W;          // K x n x NB
-  z2* P;          // hemv partials: K x (nt*nt*TS)   (y_I from tile (I,J))
-  z2* P2;         // hemv partials: K x (nt*maxseg*TS) (y_J from segment s)
-  z2* tpart;      // K x nt x 2NB
-  z2* t12;        // K x 2NB
-  double* npart;  // K x nblk (column norm partials, then alpha partials)
-  z2* scal;       // K x 4: tau, beta(+0i), scale, alpha
-  unsigned* cnt;  // K x 4 completion counters (zero on entry, left zero)
+  z2* P;          // hemv row partials: K x nt x n   (A v over column strip J, row r)
+  z2* P2;         // hemv column partials: K x nt x nseg x TS (A^H v of strip J, segment s)
+  z2* tpart;      // K x nseg x 2NB: W^H v, V^H v partials per row segment
+  double* npart;  // K x nblk: column-norm partials of k_trd_col
+  z2* dpart;      // K x nblk: w^H v partials of k_trd_fin
+  z2* scal;       // K x 4: tau, 1/(alpha - beta), beta of the current column
   hipEvent_t* evs;  // 2 per column when hemv launches are timed (created lazily)
   i64 nev;
 };
 constexpr int TRD_NB = 32;  // panel width
-constexpr int TRD_TS = 32;  // hemv tile edge
-constexpr int TRD_SEG = 8;  // tiles per hemv segment
+constexpr int TRD_TS = 32;     // hemv column-strip width
+constexpr int TRD_SEGR = 2048; // hemv rows per block
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches);

@@ -1,41 +1,49 @@
-// Batched reduction of K Hermitian matrices (lower triangle, column-major) to
-// real symmetric tridiagonal form A = Q T Q^H, Q = H_0 H_1 ... H_{n-2},
-// H_j = I - tau_j v_j v_j^H.  This is the blocked lower Householder reduction
-// that LAPACK zhetrd/zlatrd define (the reference reaches it through zheevr,
-// LapackRoutines.f90:414-426); the arithmetic per column is the same:
+// Batched reduction of K Hermitian matrices (lower triangle, column-major,
+// leading dimension lda) to real symmetric tridiagonal form A = Q T Q^H,
+// Q = H_0 H_1 ... H_{n-2}, H_j = I - tau_j v_j v_j^H.  This is the blocked
+// lower Householder reduction of LAPACK zhetrd/zlatrd (the reference reaches
+// it through zheevr, LapackRoutines.f90:414-426) with the same per-column
+// arithmetic:
 //
 //   column update   a(j:n) -= A(j:n, i0:j) conj(W(j, :)) + W(j:n, :) conj(A(j, i0:j))
 //   reflector       beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha)/beta,
 //                   v = (1, x / (alpha - beta)), e_j = beta
 //   w               = tau (A22 v - V (W^H v) - W (V^H v)),   w += -tau/2 (w^H v) v
-//   panel end       A22 -= V W^H + W V^H   (lower triangle)
+//   panel end       A22 -= V W^H + W V^H   (lower triangle, k_trd_her2k)
 //
-// Each column is three launches over all K matrices:
-//   k_trd_col    column update; the last block to finish forms the reflector.
-//   k_trd_hemv   A22 v over lower-triangle tiles (A22 read once, LDS-staged 32x32
-//                tiles, each tile feeding both A v and A^H v), plus W^H v, V^H v.
-//   k_trd_fin    reduces the hemv partials into w; the last block applies the
-//                -tau/2 (w^H v) v correction.
-// Completion counters implement the last-block reductions; each last block
-// resets its counter, so the counters stay zero between launches.
+// Each column is three launches over all K matrices, with no atomics and no
+// device-scope fences (the L2 caches of the XCDs are not coherent with each
+// other, so every such fence would write back L2): a producer kernel leaves
+// per-block partial sums and every block of the consumer kernel reduces the
+// few partials it needs itself, in a fixed order.
+//   k_trd_col    correction of the previous w (-tau/2 (w^H v) v, own rows), the
+//                column update, norm partials of the new column
+//   k_trd_hemv   reflector scalars from the norm partials; A22 v with the lower
+//                triangle of A22 read once (32-column strips); W^H v, V^H v
+//                partials in dedicated blocks
+//   k_trd_fin    w = tau (A22 v - V W^H v - W V^H v), v written back, w^H v
+//                partials
+// The correction of the panel's last w is applied on the fly by k_trd_her2k.
 #include "kernels.hpp"
 
 namespace forge {
 
 constexpr int NB = TRD_NB;
 constexpr int TS = TRD_TS;
-constexpr int SEG = TRD_SEG;
+constexpr int SEGR = TRD_SEGR;
 
+// column strips of width TS over the trailing order m
 __host__ __device__ static inline i64 nt_of(i64 m) { return (m + TS - 1) / TS; }
-__host__ __device__ static inline i64 seg_of(i64 nt) { return (nt + SEG - 1) / SEG; }
+// row segments of strip J (rows 32J .. m-1)
+__host__ __device__ static inline i64 nseg_of(i64 m, i64 J) { return (m - TS * J + SEGR - 1) / SEGR; }
 __host__ __device__ static inline i64 nblk_of(i64 m) { return (m + 255) / 256; }
 
 struct TrdLayout {
-  size_t A_d, A_e, A_tau, A_W, A_P, A_P2, A_tp, A_t12, A_np, A_scal, A_cnt, total;
+  size_t A_d, A_e, A_tau, A_W, A_P, A_P2, A_tp, A_np, A_dp, A_sc, total;
 };
 
 static TrdLayout layout(i64 n, int K) {
-  i64 nt = nt_of(n), ms = seg_of(nt), nb = nblk_of(n) + 1;
+  i64 nt = nt_of(n), ms = nseg_of(n, 0), nb = nblk_of(n) + 1;
   TrdLayout L;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -47,13 +55,12 @@ static TrdLayout layout(i64 n, int K) {
   L.A_e = take(sizeof(double) * K * n);
   L.A_tau = take(sizeof(z2) * K * n);
   L.A_W = take(sizeof(z2) * K * n * NB);
-  L.A_P = take(sizeof(z2) * (size_t)K * nt * nt * TS);
+  L.A_P = take(sizeof(z2) * (size_t)K * nt * n);
   L.A_P2 = take(sizeof(z2) * (size_t)K * nt * ms * TS);
-  L.A_tp = take(sizeof(z2) * (size_t)K * nt * 2 * NB);
-  L.A_t12 = take(sizeof(z2) * (size_t)K * 2 * NB);
-  L.A_np = take(sizeof(double) * 2 * (size_t)K * nb);
-  L.A_scal = take(sizeof(z2) * K * 4);
-  L.A_cnt = take(sizeof(unsigned) * K * 4);
+  L.A_tp = take(sizeof(z2) * (size_t)K * ms * 2 * NB);
+  L.A_np = take(sizeof(double) * (size_t)K * nb);
+  L.A_dp = take(sizeof(z2) * (size_t)K * nb);
+  L.A_sc = take(sizeof(z2) * (size_t)K * 4);
   L.total = off;
   return L;
 }
@@ -65,6 +72,7 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   char* b = (char*)base;
   w.n = n;
   w.K = K;
+  w.lda = n;
   w.d = (double*)(b + L.A_d);
   w.e = (double*)(b + L.A_e);
   w.tau = (z2*)(b + L.A_tau);
@@ -72,13 +80,11 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   w.P = (z2*)(b + L.A_P);
   w.P2 = (z2*)(b + L.A_P2);
   w.tpart = (z2*)(b + L.A_tp);
-  w.t12 = (z2*)(b + L.A_t12);
   w.npart = (double*)(b + L.A_np);
-  w.scal = (z2*)(b + L.A_scal);
-  w.cnt = (unsigned*)(b + L.A_cnt);
+  w.dpart = (z2*)(b + L.A_dp);
+  w.scal = (z2*)(b + L.A_sc);
   w.evs = nullptr;
   w.nev = 0;
-  FORGE_HIP(hipMemset(w.cnt, 0, sizeof(unsigned) * K * 4));
   return 0;
 }
 
@@ -95,327 +101,380 @@ __device__ inline double block_sum256(double v, double* sh) {
   return r;
 }
 
-__device__ inline z2 vload(const z2* p) {
-  const volatile double* q = (const volatile double*)p;
-  return z2{q[0], q[1]};
-}
-__device__ inline double vloadd(const double* p) { return *(const volatile double*)p; }
+struct Reflector {
+  z2 tau, scale;
+  double beta;
+};
 
-// Returns true in every thread of the last block (of gridDim.x blocks on this
-// y/z slice) to arrive; that block then sees all other blocks' writes.
-__device__ inline bool last_block(unsigned* counter, unsigned total) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
+// zlarfg of the column j: alpha = A(j+1, j), |x|^2 = xn2 (rows j+2..).
+__device__ inline Reflector reflector(z2 alpha, double xn2) {
+  Reflector R;
+  if (xn2 == 0.0 && alpha.y == 0.0) {
+    R.tau = mk(0.0, 0.0);
+    R.beta = alpha.x;
+    R.scale = mk(0.0, 0.0);
+  } else {
+    double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+    R.beta = alpha.x >= 0.0 ? -nr : nr;
+    R.tau = mk((R.beta - alpha.x) / R.beta, -alpha.y / R.beta);
+    z2 den = mk(alpha.x - R.beta, alpha.y);  // 1 / (alpha - beta)
+    double dd = abs2(den);
+    R.scale = mk(den.x / dd, -den.y / dd);
+  }
+  return R;
+}
+
+// Reflector of column j from the norm partials of k_trd_col (fixed order).
+// All threads of the block return the same value.
+__device__ inline Reflector reflector_of(const z2* A, i64 lda, i64 n, i64 j, const double* npart) {
+  __shared__ Reflector Rs;
   if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == total - 1);
+    const unsigned nb = (unsigned)nblk_of(n - j);
+    double xn2 = 0.0;
+    for (unsigned b = 0; b < nb; ++b) xn2 += npart[b];
+    Rs = reflector(A[(j + 1) + lda * j], xn2);
   }
   __syncthreads();
-  if (is_last) __threadfence();
-  return is_last != 0;
+  Reflector R = Rs;
+  __syncthreads();
+  return R;
+}
+
+// -tau_j/2 (w_j^H v_j) of the finished column j from the partials of
+// k_trd_fin (its grid had nblk_of(n - j - 1) blocks).
+__device__ inline z2 alpha_of(i64 n, i64 j, z2 tau, const z2* dpart) {
+  __shared__ z2 as;
+  if (threadIdx.x == 0) {
+    const unsigned nb = (unsigned)nblk_of(n - j - 1);
+    z2 s = mk(0.0, 0.0);
+    for (unsigned b = 0; b < nb; ++b) s = s + dpart[b];
+    as = (-0.5) * (tau * s);
+  }
+  __syncthreads();
+  z2 a = as;
+  __syncthreads();
+  return a;
 }
 
 // ---------------------------------------------------------------- K1: column update
-__global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z2* A_,
-                                                 size_t sA, z2* W_, double* d_, double* e_,
-                                                 z2* tau_, double* npart_, z2* scal_,
-                                                 unsigned* cnt_, i64 nbmax) {
+__global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z2* A_, size_t sA,
+                                                 i64 lda, z2* W_, double* d_, const z2* tau_,
+                                                 double* npart_, const z2* dpart_, i64 nbmax) {
   const int k = blockIdx.y;
   z2* A = A_ + k * sA;
-  const z2* W = W_ + (size_t)k * n * NB;
-  double* npart = npart_ + (size_t)k * 2 * nbmax;
+  z2* W = W_ + (size_t)k * n * NB;
   __shared__ z2 arow[NB], wrow[NB];
   __shared__ double sh[256];
+  // pending correction of column jj-1 (reflector j-1): w += alpha v
+  z2 alpha = mk(0.0, 0.0);
+  if (jj > 0) alpha = alpha_of(n, j - 1, tau_[(size_t)k * n + j - 1], dpart_ + (size_t)k * nbmax);
   if (threadIdx.x < jj) {
-    arow[threadIdx.x] = A[j + n * (i0 + threadIdx.x)];
-    wrow[threadIdx.x] = W[j + n * threadIdx.x];
+    const int q = threadIdx.x;
+    arow[q] = A[j + lda * (i0 + q)];
+    z2 wq = W[j + n * q];
+    if (q == jj - 1) wq = wq + alpha;  // v_{j-1} has its unit element in row j
+    wrow[q] = wq;
   }
   __syncthreads();
-  i64 r = j + (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 r = j + (i64)blockIdx.x * 256 + threadIdx.x;
   double nrm = 0.0;
   if (r < n) {
-    z2 a = A[r + n * j];
+    z2 a = A[r + lda * j];
     if (r == j) a.y = 0.0;
-    for (int q = 0; q < jj; ++q) {
-      a = a - mulc(A[r + n * (i0 + q)], wrow[q]);
-      a = a - mulc(W[r + n * q], arow[q]);
+    if (jj > 0) {
+      z2 wc = W[r + n * (jj - 1)];
+      wc = wc + alpha * A[r + lda * (j - 1)];
+      if (r > j) W[r + n * (jj - 1)] = wc;  // row j is not read again
+      for (int q = 0; q < jj - 1; ++q) {
+        a = a - mulc(A[r + lda * (i0 + q)], wrow[q]);
+        a = a - mulc(W[r + n * q], arow[q]);
+      }
+      a = a - mulc(A[r + lda * (j - 1)], wrow[jj - 1]);
+      a = a - mulc(wc, arow[jj - 1]);
     }
     if (r == j) {
       a.y = 0.0;
       d_[(size_t)k * n + j] = a.x;
     }
-    A[r + n * j] = a;
+    A[r + lda * j] = a;
     if (r >= j + 2) nrm = abs2(a);
   }
   nrm = block_sum256(nrm, sh);
-  if (threadIdx.x == 0) npart[blockIdx.x] = nrm;
-  if (!last_block(&cnt_[k * 4 + 0], gridDim.x)) return;
-  if (threadIdx.x == 0) {
-    double xn2 = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) xn2 += vloadd(&npart[b]);
-    z2 alpha = vload(&A[(j + 1) + n * j]);
-    z2 tau, scale;
-    double beta;
-    if (xn2 == 0.0 && alpha.y == 0.0) {
-      tau = mk(0.0, 0.0);
-      beta = alpha.x;
-      scale = mk(0.0, 0.0);
-    } else {
-      double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-      beta = alpha.x >= 0.0 ? -nr : nr;
-      tau = mk((beta - alpha.x) / beta, -alpha.y / beta);
-      z2 den = mk(alpha.x - beta, alpha.y);  // 1 / (alpha - beta)
-      double dd = abs2(den);
-      scale = mk(den.x / dd, -den.y / dd);
-    }
-    z2* sc = scal_ + k * 4;
-    sc[0] = tau;
-    sc[1] = mk(beta, 0.0);
-    sc[2] = scale;
-    e_[(size_t)k * n + j] = beta;
-    tau_[(size_t)k * n + j] = tau;
-    cnt_[k * 4 + 0] = 0;
-  }
-}
-
-__device__ inline z2 vcomp(const z2* A, i64 n, i64 j, i64 base, i64 r, z2 scale) {
-  if (r >= n) return mk(0.0, 0.0);
-  if (r == base) return mk(1.0, 0.0);
-  return A[r + n * j] * scale;
+  if (threadIdx.x == 0) npart_[(size_t)k * nbmax + blockIdx.x] = nrm;
 }
 
 // ---------------------------------------------------------------- K2: hemv
-// grid (nt, maxseg, K); block (J, s) covers tiles (I, J), I in [J + s*SEG, ...).
+// grid (nt + 1, nseg(m, 0), K).  Strip blocks (J < nt) stream column strip J
+// (32 columns, c0 = base + 32 J) over rows [32 J + s SEGR, min(m, 32 J + (s+1) SEGR))
+// of the trailing matrix.  Lane l of a wave covers row (l mod 16) and the 8
+// columns 8 (l / 16) .. + 7 of a 16-row step, so one load instruction reads
+// four 256-byte column segments, a thread holds only 8 column partials, and
+// the 4 waves take interleaved 16-row steps:
+//   P[J][r]      = sum_c A(r, c) v_c                (c <= r inside the diagonal tile)
+//   P2[J][s][c]  = sum_r conj(A(r, c)) v_r          (r >  c inside the diagonal tile)
+// Blocks J == nt form the W^H v and V^H v partials of row segment s.
 __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, const z2* A_,
-                                                  size_t sA, const z2* W_, const z2* scal_,
-                                                  z2* P_, z2* P2_, z2* tpart_, z2* t12_,
-                                                  unsigned* cnt_, i64 ntmax, i64 msmax) {
+                                                  size_t sA, i64 lda, const z2* W_,
+                                                  const double* npart_, z2* P_, z2* P2_, z2* tpart_,
+                                                  z2* scal_, i64 ntmax, i64 msmax, i64 nbmax) {
   const int k = blockIdx.z;
   const i64 J = blockIdx.x, s = blockIdx.y;
-  const i64 base = j + 1, m = n - base, nt = nt_of(m), ms = seg_of(nt);
-  const i64 Ib = J + s * SEG;
-  if (J >= nt || Ib >= nt) return;
-  const i64 Ie = Ib + SEG < nt ? Ib + SEG : nt;
+  const i64 base = j + 1, m = n - base, nt = nt_of(m);
   const z2* A = A_ + k * sA;
-  const z2 scale = scal_[k * 4 + 2];
-  __shared__ z2 tile[TS][TS + 1];
-  __shared__ z2 vJ[TS], vI[TS];
-  __shared__ z2 red[8][TS];
-  const int tx = threadIdx.x & 31, g = threadIdx.x >> 5;
-  const i64 c0 = base + J * TS;
-  if (threadIdx.x < TS) vJ[threadIdx.x] = vcomp(A, n, j, base, c0 + threadIdx.x, scale);
-  z2 yJ = mk(0.0, 0.0);
-  z2* P = P_ + (size_t)k * ntmax * ntmax * TS;
-  for (i64 I = Ib; I < Ie; ++I) {
-    const i64 r0 = base + I * TS;
-    __syncthreads();
-    if (threadIdx.x < TS) vI[threadIdx.x] = vcomp(A, n, j, base, r0 + threadIdx.x, scale);
-    // load tile rows r0.., cols c0.. (lower part only on the diagonal tile)
-    for (int q = 0; q < 4; ++q) {
-      int cc = g * 4 + q;
-      i64 r = r0 + tx, c = c0 + cc;
-      z2 a = mk(0.0, 0.0);
-      if (r < n && c < n && (I != J || tx >= cc)) a = A[r + n * c];
-      tile[tx][cc] = a;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (J == nt) {
+    // ---- W^H v and V^H v over rows [s SEGR, (s+1) SEGR) of the trailing part
+    const i64 rs = s * SEGR;
+    if (rs >= m) return;
+    const i64 re = rs + SEGR < m ? rs + SEGR : m;
+    const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
+    if (s == 0 && threadIdx.x == 0) {  // published for k_trd_fin, which rewrites A(j+1, j)
+      scal_[k * 4 + 0] = R.tau;
+      scal_[k * 4 + 1] = R.scale;
+      scal_[k * 4 + 2] = mk(R.beta, 0.0);
     }
-    __syncthreads();
-    if (I == J) {  // Hermitian completion of the diagonal tile
-      for (int q = 0; q < 4; ++q) {
-        int cc = g * 4 + q;
-        if (tx < cc) tile[tx][cc] = conjz(tile[cc][tx]);
-      }
-      __syncthreads();
-    }
-    // y_I(r) partial over this thread's 4 columns
-    z2 acc = mk(0.0, 0.0);
-    for (int q = 0; q < 4; ++q) acc = acc + tile[tx][g * 4 + q] * vJ[g * 4 + q];
-    red[g][tx] = acc;
-    // y_J(c) partial: A^H v_I over this thread's 4 rows (off-diagonal tiles)
-    if (I != J) {
-      for (int q = 0; q < 4; ++q) yJ = yJ + cmul(tile[g * 4 + q][tx], vI[g * 4 + q]);
-    }
-    __syncthreads();
-    if (threadIdx.x < TS) {
-      z2 sacc = red[0][threadIdx.x];
-      for (int q = 1; q < 8; ++q) sacc = sacc + red[q][threadIdx.x];
-      P[((size_t)I * ntmax + J) * TS + threadIdx.x] = sacc;
-    }
-  }
-  __syncthreads();
-  red[g][tx] = yJ;
-  __syncthreads();
-  if (threadIdx.x < TS) {
-    z2 sacc = red[0][threadIdx.x];
-    for (int q = 1; q < 8; ++q) sacc = sacc + red[q][threadIdx.x];
-    P2_[(((size_t)k * ntmax + J) * msmax + s) * TS + threadIdx.x] = sacc;
-  }
-  if (s != 0) return;
-  // W^H v and V^H v over the rows of tile J (q < jj)
-  __syncthreads();
-  const z2* W = W_ + (size_t)k * n * NB;
-  z2* tpart = tpart_ + (size_t)k * ntmax * 2 * NB;
-  {
-    int q = threadIdx.x >> 3, part = threadIdx.x & 7;  // 32 columns x 8 row groups
-    z2 t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0);
-    if (q < jj) {
-      for (int rr = part * 4; rr < part * 4 + 4; ++rr) {
-        i64 r = c0 + rr;
-        if (r < n) {
-          t1 = t1 + cmul(W[r + n * q], vJ[rr]);
-          t2 = t2 + cmul(A[r + n * (i0 + q)], vJ[rr]);
+    const z2* W = W_ + (size_t)k * n * NB;
+    const int qb = wv * 8;  // 8 panel columns per wave, lanes over rows
+    z2 t1[8], t2[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t1[q] = t2[q] = mk(0.0, 0.0);
+    if (qb < jj) {
+      for (i64 rr = rs + lane; rr < re; rr += 64) {
+        const i64 r = base + rr;
+        const z2 vr = (rr == 0) ? mk(1.0, 0.0) : A[r + lda * j] * R.scale;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (qb + q < jj) {
+            t1[q] = t1[q] + cmul(W[r + n * (qb + q)], vr);
+            t2[q] = t2[q] + cmul(A[r + lda * (i0 + qb + q)], vr);
+          }
         }
       }
     }
-    __shared__ z2 tr1[32][8], tr2[32][8];
-    tr1[q][part] = t1;
-    tr2[q][part] = t2;
-    __syncthreads();
-    if (threadIdx.x < NB) {
-      z2 a = mk(0.0, 0.0), b = mk(0.0, 0.0);
-      for (int p = 0; p < 8; ++p) {
-        a = a + tr1[threadIdx.x][p];
-        b = b + tr2[threadIdx.x][p];
+    z2* tp = tpart_ + ((size_t)k * msmax + s) * 2 * NB;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+        d += __shfl_xor(d, off, 64);
       }
-      tpart[J * 2 * NB + threadIdx.x] = a;
-      tpart[J * 2 * NB + NB + threadIdx.x] = b;
+      if (lane == 0) {
+        tp[qb + q] = mk(a, b);
+        tp[NB + qb + q] = mk(c, d);
+      }
     }
+    return;
   }
-  // last of the nt segment-0 blocks reduces tpart -> t12
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = (atomicAdd(&cnt_[k * 4 + 1], 1u) == (unsigned)nt - 1);
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  if (threadIdx.x < 2 * NB) {
-    z2 a = mk(0.0, 0.0);
-    for (i64 JJ = 0; JJ < nt; ++JJ) a = a + vload(&tpart[JJ * 2 * NB + threadIdx.x]);
-    t12_[k * 2 * NB + threadIdx.x] = a;
+  const i64 rs = TS * J + s * SEGR;  // first row (relative to base) of this block
+  if (J > nt || rs >= m) return;
+  const i64 re = rs + SEGR < m ? rs + SEGR : m;
+  const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
+  const z2 scale = R.scale;
+  const i64 c0 = base + J * TS;
+  const int ncol = (int)((n - c0) < TS ? (n - c0) : TS);
+  __shared__ z2 vJ[TS];
+  __shared__ z2 red[4][TS];
+  if (threadIdx.x < TS) {
+    const i64 c = c0 + threadIdx.x;
+    vJ[threadIdx.x] = c >= n ? mk(0.0, 0.0) : (c == base ? mk(1.0, 0.0) : A[c + lda * j] * scale);
   }
-  if (threadIdx.x == 0) cnt_[k * 4 + 1] = 0;
+  __syncthreads();
+  const int rsub = lane & 15, cg = lane >> 4;  // row in the step, column group
+  const int cb = cg * 8;                       // first column of this lane's group
+  z2 vc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) vc[q] = vJ[cb + q];
+  z2 acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = mk(0.0, 0.0);
+  z2* P = P_ + ((size_t)k * ntmax + J) * n;
+  const z2* colp = A + (size_t)lda * (c0 + cb);
+  // row steps start at an absolute multiple of 16 rows, so with lda a multiple
+  // of 16 every 16-row column chunk is one aligned 256-byte segment
+  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+  for (i64 rr0 = ra0 + 16 * wv; rr0 < re; rr0 += 64) {
+    const i64 rr = rr0 + rsub;  // row relative to base
+    const bool valid = rr >= rs && rr < re;
+    const i64 r = base + rr;
+    const z2 vr = !valid ? mk(0.0, 0.0) : (rr == 0 ? mk(1.0, 0.0) : A[r + lda * j] * scale);
+    const i64 dr = rr - TS * J;  // < 32 inside the diagonal tile
+    z2 a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      a[q] = (valid && cb + q < ncol) ? colp[r + (size_t)lda * q] : mk(0.0, 0.0);
+    z2 y = mk(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const z2 zero = mk(0.0, 0.0);
+      const int c = cb + q;
+      y = y + ((c <= dr) ? a[q] : zero) * vc[q];
+      acc[q] = acc[q] + cmul((c < dr) ? a[q] : zero, vr);
+    }
+    // row sum over the 4 column groups (lanes l, l^16, l^32, l^48)
+    y.x += __shfl_xor(y.x, 16, 64);
+    y.y += __shfl_xor(y.y, 16, 64);
+    y.x += __shfl_xor(y.x, 32, 64);
+    y.y += __shfl_xor(y.y, 32, 64);
+    if (cg == 0 && valid) P[rr] = y;
+  }
+  // column partials: sum over the 16 row lanes, then over the 4 waves
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    double x = acc[q].x, yv = acc[q].y;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      x += __shfl_xor(x, off, 64);
+      yv += __shfl_xor(yv, off, 64);
+    }
+    if (rsub == 0) red[wv][cb + q] = mk(x, yv);
+  }
+  __syncthreads();
+  if (threadIdx.x < TS) {
+    z2 t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    P2_[(((size_t)k * ntmax + J) * msmax + s) * TS + threadIdx.x] = t;
+  }
 }
 
 // ---------------------------------------------------------------- K3: finalize w
 __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z2* A_, size_t sA,
-                                                 z2* W_, const z2* scal_, const z2* P_,
-                                                 const z2* P2_, const z2* t12_, double* npart_,
-                                                 unsigned* cnt_, i64 ntmax, i64 msmax,
+                                                 i64 lda, z2* W_, double* e_, z2* tau_,
+                                                 const z2* scal_, const z2* P_, const z2* P2_,
+                                                 const z2* tpart_, z2* dpart_, i64 ntmax, i64 msmax,
                                                  i64 nbmax) {
   const int k = blockIdx.y;
-  const i64 base = j + 1, m = n - base, nt = nt_of(m);
+  const i64 base = j + 1, m = n - base, nt = nt_of(m), ns0 = nseg_of(m, 0);
+  (void)nt;
   z2* A = A_ + k * sA;
   z2* W = W_ + (size_t)k * n * NB;
-  const z2 tau = scal_[k * 4 + 0], scale = scal_[k * 4 + 2];
+  const z2 tau = scal_[k * 4 + 0], scale = scal_[k * 4 + 1];
+  const double beta = scal_[k * 4 + 2].x;
   __shared__ z2 t1[NB], t2[NB];
   __shared__ double sh[256];
-  if (threadIdx.x < jj) {
-    t1[threadIdx.x] = t12_[k * 2 * NB + threadIdx.x];
-    t2[threadIdx.x] = t12_[k * 2 * NB + NB + threadIdx.x];
+  if (threadIdx.x < 2 * NB) {
+    const z2* tp = tpart_ + (size_t)k * msmax * 2 * NB;
+    z2 a = mk(0.0, 0.0);
+    for (i64 s = 0; s < ns0; ++s) a = a + tp[s * 2 * NB + threadIdx.x];
+    if (threadIdx.x < NB) t1[threadIdx.x] = a;
+    else t2[threadIdx.x - NB] = a;
   }
   __syncthreads();
   const i64 r = base + (i64)blockIdx.x * 256 + threadIdx.x;
   z2 dot = mk(0.0, 0.0);
   if (r < n) {
-    const i64 R = (r - base) / TS, rr = (r - base) % TS;
-    const z2* P = P_ + (size_t)k * ntmax * ntmax * TS;
+    const i64 rel = r - base, R = rel / TS, rr = rel % TS;
+    const z2* P = P_ + (size_t)k * ntmax * n;
     z2 y = mk(0.0, 0.0);
-    for (i64 J = 0; J <= R; ++J) y = y + P[((size_t)R * ntmax + J) * TS + rr];
-    i64 nseg = seg_of(nt - R);
+    for (i64 J = 0; J <= R; ++J) y = y + P[(size_t)J * n + rel];
+    const i64 nseg = nseg_of(m, R);
     const z2* P2 = P2_ + ((size_t)k * ntmax + R) * msmax * TS;
     for (i64 s = 0; s < nseg; ++s) y = y + P2[s * TS + rr];
-    z2 v = (r == base) ? mk(1.0, 0.0) : A[r + n * j] * scale;
+    const z2 v = (r == base) ? mk(1.0, 0.0) : A[r + lda * j] * scale;
     for (int q = 0; q < jj; ++q) {
-      y = y - A[r + n * (i0 + q)] * t1[q];
+      y = y - A[r + lda * (i0 + q)] * t1[q];
       y = y - W[r + n * q] * t2[q];
     }
-    z2 w = tau * y;
+    const z2 w = tau * y;
     W[r + n * jj] = w;
-    A[r + n * j] = v;
+    A[r + lda * j] = v;
     dot = cmul(w, v);
   }
-  double* np = npart_ + (size_t)k * 2 * nbmax;
-  double dr = block_sum256(dot.x, sh);
-  double di = block_sum256(dot.y, sh);
+  const double dr = block_sum256(dot.x, sh);
+  const double di = block_sum256(dot.y, sh);
   if (threadIdx.x == 0) {
-    np[2 * blockIdx.x] = dr;
-    np[2 * blockIdx.x + 1] = di;
-  }
-  if (!last_block(&cnt_[k * 4 + 2], gridDim.x)) return;
-  __shared__ z2 alpha_s;
-  if (threadIdx.x == 0) {
-    z2 s = mk(0.0, 0.0);
-    for (unsigned b = 0; b < gridDim.x; ++b) s = s + mk(vloadd(&np[2 * b]), vloadd(&np[2 * b + 1]));
-    alpha_s = (-0.5) * (tau * s);
-    cnt_[k * 4 + 2] = 0;
-  }
-  __syncthreads();
-  const z2 alpha = alpha_s;
-  for (i64 rr = base + threadIdx.x; rr < n; rr += 256) {
-    z2 w = vload(&W[rr + n * jj]);
-    z2 v = vload(&A[rr + n * j]);
-    W[rr + n * jj] = w + alpha * v;
+    dpart_[(size_t)k * nbmax + blockIdx.x] = mk(dr, di);
+    if (blockIdx.x == 0) {
+      e_[(size_t)k * n + j] = beta;
+      tau_[(size_t)k * n + j] = tau;
+    }
   }
 }
 
 // ---------------------------------------------------------------- her2k
-// A(p:n, p:n) -= V W^H + W V^H, lower tiles of 32x32, V = A(p:n, i0:i0+nbp).
+// A(p:n, p:n) -= V W^H + W V^H on the lower triangle, V = A(p:n, i0:i0+nbp),
+// W = W(p:n, 0:nbp), whose last column still lacks its -tau/2 (w^H v) v term
+// (added here while staging).  64x64 output tiles (lower tiles only), 256
+// threads with 4x4 complex outputs each, panel columns staged 8 at a time.
 __global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp, z2* A_,
-                                                   size_t sA, const z2* W_) {
+                                                   size_t sA, i64 lda, const z2* W_,
+                                                   const z2* tau_, const z2* dpart_, i64 nbmax) {
   const int k = blockIdx.y;
   z2* A = A_ + k * sA;
   const z2* W = W_ + (size_t)k * n * NB;
+  const i64 jl = i0 + nbp - 1;  // last reflector of the panel
+  const z2 alpha = alpha_of(n, jl, tau_[(size_t)k * n + jl], dpart_ + (size_t)k * nbmax);
   i64 t = blockIdx.x;
   i64 I = (i64)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
   while (I * (I + 1) / 2 > t) --I;
-  i64 J = t - I * (I + 1) / 2;
-  const i64 r0 = p + I * 32, c0 = p + J * 32;
-  __shared__ z2 Vr[32][9], Wr[32][9], Vc[32][9], Wc[32][9];
-  const int tx = threadIdx.x & 31, g = threadIdx.x >> 5;
-  z2 acc[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  const i64 J = t - I * (I + 1) / 2;
+  const i64 r0 = p + I * 64, c0 = p + J * 64;
+  __shared__ z2 Vr[8][64], Wr[8][64], Vc[8][64], Wc[8][64];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  z2 acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = mk(0.0, 0.0);
+  const z2 zero = mk(0.0, 0.0);
   for (int q0 = 0; q0 < nbp; q0 += 8) {
     __syncthreads();
-    {
-      int q = q0 + g;  // 8 groups load 8 panel columns
-      i64 r = r0 + tx, c = c0 + tx;
-      z2 z = mk(0.0, 0.0);
-      Vr[tx][g] = (q < nbp && r < n) ? A[r + n * (i0 + q)] : z;
-      Wr[tx][g] = (q < nbp && r < n) ? W[r + n * q] : z;
-      Vc[tx][g] = (q < nbp && c < n) ? A[c + n * (i0 + q)] : z;
-      Wc[tx][g] = (q < nbp && c < n) ? W[c + n * q] : z;
+    for (int e = threadIdx.x; e < 8 * 64; e += 256) {
+      const int qq = e >> 6, rr = e & 63, q = q0 + qq;
+      const i64 r = r0 + rr, c = c0 + rr;
+      const bool qv = q < nbp, last = q == nbp - 1;
+      const z2 vr = (qv && r < n) ? A[r + lda * (i0 + q)] : zero;
+      const z2 vc = (qv && c < n) ? A[c + lda * (i0 + q)] : zero;
+      z2 wr = (qv && r < n) ? W[r + n * q] : zero;
+      z2 wc = (qv && c < n) ? W[c + n * q] : zero;
+      if (last) {
+        wr = wr + alpha * vr;
+        wc = wc + alpha * vc;
+      }
+      Vr[qq][rr] = vr;
+      Wr[qq][rr] = wr;
+      Vc[qq][rr] = vc;
+      Wc[qq][rr] = wc;
     }
     __syncthreads();
-    for (int e4 = 0; e4 < 4; ++e4) {
-      int cc = g * 4 + e4;
-      z2 a = acc[e4];
-      for (int q = 0; q < 8; ++q) {
-        a = a + mulc(Vr[tx][q], Wc[cc][q]);
-        a = a + mulc(Wr[tx][q], Vc[cc][q]);
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) {
+      z2 va[4], wa[4], vb[4], wb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        va[u] = Vr[qq][tx + 16 * u];
+        wa[u] = Wr[qq][tx + 16 * u];
+        vb[u] = Vc[qq][ty + 16 * u];
+        wb[u] = Wc[qq][ty + 16 * u];
       }
-      acc[e4] = a;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = acc[u][v] + mulc(va[u], wb[v]) + mulc(wa[u], vb[v]);
     }
   }
-  for (int e4 = 0; e4 < 4; ++e4) {
-    i64 r = r0 + tx, c = c0 + g * 4 + e4;
-    if (r < n && c < n && r >= c) {
-      z2 a = A[r + n * c];
-      A[r + n * c] = a - acc[e4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const i64 r = r0 + tx + 16 * u, c = c0 + ty + 16 * v;
+      if (r < n && c < n && r >= c) A[r + lda * c] = A[r + lda * c] - acc[u][v];
     }
-  }
 }
 
-__global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, double* d_) {
+__global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* d_) {
   int k = blockIdx.x;
-  if (threadIdx.x == 0) d_[(size_t)k * n + n - 1] = A_[k * sA + (n - 1) + n * (n - 1)].x;
+  if (threadIdx.x == 0) d_[(size_t)k * n + n - 1] = A_[k * sA + (n - 1) + lda * (n - 1)].x;
 }
 
 // ---------------------------------------------------------------- driver
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches) {
   const i64 n = w.n;
   const int K = w.K;
-  const i64 ntmax = nt_of(n), msmax = seg_of(ntmax), nbmax = nblk_of(n) + 1;
+  const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = nblk_of(n) + 1;
   const bool timed = hemv_ms != nullptr;
   if (timed && w.nev < 2 * n) {
     hipEvent_t* ev = new hipEvent_t[2 * n];
@@ -431,28 +490,29 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches)
     for (int jj = 0; jj < nbp; ++jj) {
       const i64 j = i0 + jj;
       hipLaunchKernelGGL(k_trd_col, dim3(cdiv(n - j, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
-                         w.sA, w.W, w.d, w.e, w.tau, w.npart, w.scal, w.cnt, nbmax);
+                         w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
-      const i64 m = n - j - 1, nt = nt_of(m), ms = seg_of(nt);
+      const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
       if (timed) FORGE_HIP(hipEventRecord(w.evs[2 * nrec], s));
-      hipLaunchKernelGGL(k_trd_hemv, dim3((unsigned)nt, (unsigned)ms, K), dim3(256), 0, s, n, j,
-                         i0, jj, w.A, w.sA, w.W, w.scal, w.P, w.P2, w.tpart, w.t12, w.cnt, ntmax,
-                         msmax);
+      hipLaunchKernelGGL(k_trd_hemv, dim3((unsigned)nt + 1, (unsigned)ms, K), dim3(256), 0, s, n, j,
+                         i0, jj, w.A, w.sA, w.lda, w.W, w.npart, w.P, w.P2, w.tpart, w.scal, ntmax,
+                         msmax, nbmax);
       FORGE_LAUNCH();
       if (timed) {
         FORGE_HIP(hipEventRecord(w.evs[2 * nrec + 1], s));
         ++nrec;
       }
       hipLaunchKernelGGL(k_trd_fin, dim3(cdiv(m, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
-                         w.sA, w.W, w.scal, w.P, w.P2, w.t12, w.npart, w.cnt, ntmax, msmax, nbmax);
+                         w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
+                         msmax, nbmax);
       FORGE_LAUNCH();
     }
-    const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 31) / 32;
+    const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
     hipLaunchKernelGGL(k_trd_her2k, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s, n,
-                       p, i0, nbp, w.A, w.sA, w.W);
+                       p, i0, nbp, w.A, w.sA, w.lda, w.W, w.tau, w.dpart, nbmax);
     FORGE_LAUNCH();
   }
-  hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.d);
+  hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);
   FORGE_LAUNCH();
   if (timed && nrec > 0) {
     FORGE_HIP(hipEventSynchronize(w.evs[2 * nrec - 1]));
