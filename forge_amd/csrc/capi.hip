@@ -421,10 +421,11 @@ extern "C" int forge_hamiltonian_hf(forge_ctx* c, forge_z* zH, const double* Pot
 static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   const i64 n = c->n;
   c->trd.K = K;
-  double hemv_ms = 0;
+  double hemv_ms = 0, hemv_bytes = 0;
   long hemv_n = 0;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[2], c->s));
-  CK(tridiag(c->trd, c->s, timed ? &hemv_ms : nullptr, timed ? &hemv_n : nullptr));
+  CK(tridiag(c->trd, c->s, timed ? &hemv_ms : nullptr, timed ? &hemv_n : nullptr,
+             timed ? &hemv_bytes : nullptr));
   if (timed) FORGE_HIP(hipEventRecord(c->ev[3], c->s));
   int fails = dc_solve(c->trd.d, c->trd.e, n, K, mout, c->evals, c->Zq, c->dc_ws, c->s);
   if (fails < 0) return fails;
@@ -442,6 +443,7 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
     c->tm[3] += d;
     c->tm[9] += hemv_n;
     c->tm[10] += hemv_ms;
+    c->tm[11] += hemv_bytes;
   }
   return fails;
 }
@@ -474,7 +476,7 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   t.lda = ld;
   FORGE_HIP(hipMemcpy2DAsync(H, sizeof(z2) * ld, A, sizeof(z2) * n, sizeof(z2) * n, n,
                              hipMemcpyHostToDevice, s));
-  CK(tridiag(t, s, nullptr, nullptr));
+  CK(tridiag(t, s, nullptr, nullptr, nullptr));
   int fails = dc_solve(t.d, t.e, n, 1, iu, ev, Zq, dw, s);
   if (fails < 0) return fails;
   CK(backtransform(t, Zq, n, (size_t)n * iu, iu, Cv, n, (size_t)n * iu, bw, s));

@@ -48,7 +48,7 @@ struct TridiagWork {
   double* npart;  // K x nblk: column-norm partials of k_trd_col
   z2* dpart;      // K x nblk: w^H v partials of k_trd_fin
   z2* scal;       // K x 4: tau, 1/(alpha - beta), beta of the current column
-  hipEvent_t* evs;  // 2 per column when hemv launches are timed (created lazily)
+  hipEvent_t* evs;  // event pairs of the sampled hemv launches (created lazily)
   i64 nev;
 };
 constexpr int TRD_NB = 32;  // panel width
@@ -56,7 +56,8 @@ constexpr int TRD_TS = 32;     // hemv column-strip width
 constexpr int TRD_SEGR = 2048; // hemv rows per block
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
-int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches);
+int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
+            double* hemv_bytes);
 
 // dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
 struct DCWork;

@@ -24,6 +24,8 @@
 //   k_trd_fin    w = tau (A22 v - V W^H v - W V^H v), v written back, w^H v
 //                partials
 // The correction of the panel's last w is applied on the fly by k_trd_her2k.
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 
 namespace forge {
@@ -471,20 +473,29 @@ __global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* 
 }
 
 // ---------------------------------------------------------------- driver
-int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches) {
+// Timing (hemv_ms != nullptr): the hemv launches of every TIME_STRIDE-th
+// column carry a pair of HIP events (hipExtLaunchKernel); their summed time, count and
+// algorithmic bytes (8 m^2 per matrix: the lower triangle of the trailing
+// matrix read once) are accumulated into the outputs.
+constexpr int TIME_STRIDE = 16;
+
+int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
+            double* hemv_bytes) {
   const i64 n = w.n;
   const int K = w.K;
   const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = nblk_of(n) + 1;
   const bool timed = hemv_ms != nullptr;
-  if (timed && w.nev < 2 * n) {
-    hipEvent_t* ev = new hipEvent_t[2 * n];
-    for (i64 q = 0; q < 2 * n; ++q) FORGE_HIP(hipEventCreate(&ev[q]));
+  const i64 nev_need = 2 * (n / TIME_STRIDE + 1);
+  if (timed && w.nev < nev_need) {
+    hipEvent_t* ev = new hipEvent_t[nev_need];
+    for (i64 q = 0; q < nev_need; ++q) FORGE_HIP(hipEventCreate(&ev[q]));
     for (i64 q = 0; q < w.nev; ++q) FORGE_HIP(hipEventDestroy(w.evs[q]));
     delete[] w.evs;
     w.evs = ev;
-    w.nev = 2 * n;
+    w.nev = nev_need;
   }
   i64 nrec = 0;
+  double bytes = 0.0;
   for (i64 i0 = 0; i0 < n - 1; i0 += NB) {
     const int nbp = (int)((n - 1 - i0) < NB ? (n - 1 - i0) : NB);
     for (int jj = 0; jj < nbp; ++jj) {
@@ -493,15 +504,19 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches)
                          w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
       const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
-      if (timed) FORGE_HIP(hipEventRecord(w.evs[2 * nrec], s));
-      hipLaunchKernelGGL(k_trd_hemv, dim3((unsigned)nt + 1, (unsigned)ms, K), dim3(256), 0, s, n, j,
-                         i0, jj, w.A, w.sA, w.lda, w.W, w.npart, w.P, w.P2, w.tpart, w.scal, ntmax,
-                         msmax, nbmax);
-      FORGE_LAUNCH();
-      if (timed) {
-        FORGE_HIP(hipEventRecord(w.evs[2 * nrec + 1], s));
+      const dim3 hgrid((unsigned)nt + 1, (unsigned)ms, K);
+      if (timed && (j % TIME_STRIDE == 0)) {
+        // the events take the dispatch's own begin and end timestamps
+        hipExtLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, w.evs[2 * nrec],
+                              w.evs[2 * nrec + 1], 0, n, j, i0, jj, w.A, w.sA, w.lda, w.W, w.npart,
+                              w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
         ++nrec;
+        bytes += 8.0 * (double)m * (double)m * K;
+      } else {
+        hipLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, n, j, i0, jj, w.A, w.sA, w.lda, w.W,
+                           w.npart, w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
       }
+      FORGE_LAUNCH();
       hipLaunchKernelGGL(k_trd_fin, dim3(cdiv(m, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
                          msmax, nbmax);
@@ -522,6 +537,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches)
       *hemv_ms += ms_;
     }
     *hemv_launches += nrec;
+    *hemv_bytes += bytes;
   }
   return 0;
 }

@@ -129,8 +129,9 @@ int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
  * [0] assembly, [1] tridiagonal reduction, [2] tridiagonal eigensolver,
  * [3] back-transformation, [4] density products, [5] energies + mixing,
  * [6] total, [7] launches of the assembly kernel, [8] summed time of the
- * assembly kernel (ms), [9] launches of the hemv kernel, [10] summed hemv
- * kernel time (ms). */
+ * assembly kernel (ms), [9] timed launches of the Householder hemv kernel
+ * (every 16th column), [10] their summed time (ms), [11] their algorithmic
+ * bytes (8 m^2 per matrix, m the trailing order). */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
 
 #ifdef __cplusplus
