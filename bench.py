@@ -156,8 +156,8 @@ def main():
         value, method = per_iter * iters_conv, "projected"
         ms_per_step = per_iter * 1e3
         converged = None
-    # hemv launches of every 16th column are bracketed by HIP events; their
-    # algorithmic bytes (8 m^2 per matrix) are summed by the library
+    # hemv launches of every 17th column carry HIP events on the library's
+    # stream; their algorithmic bytes (8 m^2 per matrix) are summed by the library
     hemv_ms, hemv_launches, hemv_bytes = stage[10], int(stage[9]), stage[11]
     avg_launch_ms = hemv_ms / max(hemv_launches, 1)
     bytes_per_launch = hemv_bytes / max(hemv_launches, 1)

@@ -51,32 +51,31 @@ __global__ void k_zero_upper(i64 n, i64 ld, z2* H) {
 // keeps every 16-row column chunk one aligned 256-byte segment
 static inline i64 pad_ld(i64 n) { return (n + 15) & ~(i64)15; }
 
-// out[0] = a.b, out[1] = a.c, out[2] = b.c, out[3] = a.a, out[4] = b.b,
-// out[5] = a.w, out[6] = b.w  (single block, fixed order)
+// With ya = L a and yb = L b: out[0] = a.ya, out[1] = a.yb, out[2] = b.yb,
+// out[3] = a.a, out[4] = a.b, out[5] = b.b (single block, fixed order, so
+// the sums are reproducible); b and yb may be null (terms read 0).
 __global__ void k_dots(i64 n, const double* a, const double* b, const double* ya,
-                       const double* yb, const double* w, double* out) {
-  __shared__ double sh[7][256];
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+                       const double* yb, double* out) {
+  __shared__ double sh[6][256];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
   for (i64 i = threadIdx.x; i < n; i += 256) {
-    double av = a[i], bv = b ? b[i] : 0.0, yav = ya[i], ybv = yb ? yb[i] : 0.0, wv = w[i];
-    acc[0] += av * yav;  // da.L.da
-    acc[1] += av * ybv;  // da.L.db
-    acc[2] += bv * ybv;  // db.L.db
+    double av = a[i], bv = b ? b[i] : 0.0, yav = ya[i], ybv = yb ? yb[i] : 0.0;
+    acc[0] += av * yav;
+    acc[1] += av * ybv;
+    acc[2] += bv * ybv;
     acc[3] += av * av;
     acc[4] += av * bv;
     acc[5] += bv * bv;
-    acc[6] += 0.0;
-    (void)wv;
   }
-  for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s)
-      for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < 7; ++q) out[q] = sh[q][0];
+    for (int q = 0; q < 6; ++q) out[q] = sh[q][0];
 }
 
 // bias term of GetKineticEnergy: sum_i (d_i + 1/2) bias_i for d = a and b.
@@ -703,7 +702,7 @@ static int energy_pieces(forge_ctx* c, const z2* a, const z2* b, double* pieces)
   CK(energy_pass(nn, (int)c->nnc, a, b, c->V, c->T, c->part, nblk, c->e5, c->s));
   CK(hartree_matvec(n, c->L, c->da, b ? c->db : nullptr, c->ya, b ? c->yb : nullptr, c->s));
   hipLaunchKernelGGL(k_dots, dim3(1), dim3(256), 0, c->s, n, c->da, b ? c->db : nullptr, c->ya,
-                     b ? c->yb : nullptr, c->da, c->dots);
+                     b ? c->yb : nullptr, c->dots);
   FORGE_LAUNCH();
   hipLaunchKernelGGL(k_bias_dot, dim3(1), dim3(256), 0, c->s, n, c->da, b ? c->db : nullptr, c->bias,
                      c->dots + 8);
@@ -741,6 +740,26 @@ extern "C" int forge_energies(forge_ctx* c, const forge_z* zFock, const double* 
   double pc[13];
   CK(energy_pieces(c, c->rho_new, nullptr, pc));
   energies_at(pc, 0.0, out4);
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_hartree_hubbard(forge_ctx* c, const double* dens1, const double* dens2,
+                                     double* out2) {
+  if (!c || !dens1 || !dens2 || !out2) { set_error("forge_hartree_hubbard: null argument"); return -2; }
+  const i64 n = c->n;
+  CK(ensure_longrange(c));
+  FORGE_HIP(hipMemcpyAsync(c->da, dens1, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  FORGE_HIP(hipMemcpyAsync(c->db, dens2, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  CK(hartree_matvec(n, c->L, c->da, c->db, c->ya, c->yb, c->s));
+  hipLaunchKernelGGL(k_dots, dim3(1), dim3(256), 0, c->s, n, c->da, c->db, c->ya, c->yb, c->dots);
+  FORGE_LAUNCH();
+  double hd[6];
+  FORGE_HIP(hipMemcpyAsync(hd, c->dots, sizeof(hd), hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  // (d1 + d2).L.(d1 + d2) / 2 with L symmetric; d1.d2
+  out2[0] = 0.5 * (hd[0] + 2.0 * hd[1] + hd[2]);
+  out2[1] = hd[4];
   c->scf_ready = false;
   return 0;
 }

@@ -474,10 +474,24 @@ __global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* 
 
 // ---------------------------------------------------------------- driver
 // Timing (hemv_ms != nullptr): the hemv launches of every TIME_STRIDE-th
-// column carry a pair of HIP events (hipExtLaunchKernel); their summed time, count and
+// column carry a pair of HIP events (hipExtLaunchKernel), so the events take
+// the dispatch's own begin and end timestamps; their summed time, count and
 // algorithmic bytes (8 m^2 per matrix: the lower triangle of the trailing
-// matrix read once) are accumulated into the outputs.
-constexpr int TIME_STRIDE = 16;
+// matrix read once) are accumulated into the outputs.  The pairs form a ring
+// of TIME_RING entries; a pair is read back before it is reused, which waits
+// only for a launch TIME_RING * TIME_STRIDE columns back.  The stride is
+// coprime with the panel width, so the sample visits every panel position
+// (the hemv's panel term grows with the position in the panel).
+constexpr int TIME_STRIDE = 17;
+constexpr int TIME_RING = 8;
+
+static int read_pair(TridiagWork& w, i64 q, double* ms) {
+  float ms_ = 0.f;
+  FORGE_HIP(hipEventSynchronize(w.evs[2 * q + 1]));
+  FORGE_HIP(hipEventElapsedTime(&ms_, w.evs[2 * q], w.evs[2 * q + 1]));
+  *ms += ms_;
+  return 0;
+}
 
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes) {
@@ -485,17 +499,13 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   const int K = w.K;
   const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = nblk_of(n) + 1;
   const bool timed = hemv_ms != nullptr;
-  const i64 nev_need = 2 * (n / TIME_STRIDE + 1);
-  if (timed && w.nev < nev_need) {
-    hipEvent_t* ev = new hipEvent_t[nev_need];
-    for (i64 q = 0; q < nev_need; ++q) FORGE_HIP(hipEventCreate(&ev[q]));
-    for (i64 q = 0; q < w.nev; ++q) FORGE_HIP(hipEventDestroy(w.evs[q]));
-    delete[] w.evs;
-    w.evs = ev;
-    w.nev = nev_need;
+  if (timed && w.nev == 0) {
+    w.evs = new hipEvent_t[2 * TIME_RING];
+    for (int q = 0; q < 2 * TIME_RING; ++q) FORGE_HIP(hipEventCreate(&w.evs[q]));
+    w.nev = 2 * TIME_RING;
   }
   i64 nrec = 0;
-  double bytes = 0.0;
+  double bytes = 0.0, tms = 0.0;
   for (i64 i0 = 0; i0 < n - 1; i0 += NB) {
     const int nbp = (int)((n - 1 - i0) < NB ? (n - 1 - i0) : NB);
     for (int jj = 0; jj < nbp; ++jj) {
@@ -506,10 +516,11 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
       const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
       const dim3 hgrid((unsigned)nt + 1, (unsigned)ms, K);
       if (timed && (j % TIME_STRIDE == 0)) {
-        // the events take the dispatch's own begin and end timestamps
-        hipExtLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, w.evs[2 * nrec],
-                              w.evs[2 * nrec + 1], 0, n, j, i0, jj, w.A, w.sA, w.lda, w.W, w.npart,
-                              w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
+        const i64 q = nrec % TIME_RING;
+        if (nrec >= TIME_RING && read_pair(w, q, &tms)) return -1;
+        hipExtLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, w.evs[2 * q], w.evs[2 * q + 1],
+                              0, n, j, i0, jj, w.A, w.sA, w.lda, w.W, w.npart, w.P, w.P2, w.tpart,
+                              w.scal, ntmax, msmax, nbmax);
         ++nrec;
         bytes += 8.0 * (double)m * (double)m * K;
       } else {
@@ -529,13 +540,10 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   }
   hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);
   FORGE_LAUNCH();
-  if (timed && nrec > 0) {
-    FORGE_HIP(hipEventSynchronize(w.evs[2 * nrec - 1]));
-    for (i64 q = 0; q < nrec; ++q) {
-      float ms_ = 0.f;
-      FORGE_HIP(hipEventElapsedTime(&ms_, w.evs[2 * q], w.evs[2 * q + 1]));
-      *hemv_ms += ms_;
-    }
+  if (timed) {
+    for (i64 r = (nrec > TIME_RING ? nrec - TIME_RING : 0); r < nrec; ++r)
+      if (read_pair(w, r % TIME_RING, &tms)) return -1;
+    *hemv_ms += tms;
     *hemv_launches += nrec;
     *hemv_bytes += bytes;
   }

@@ -1,0 +1,191 @@
+! forge_c.f90 -- ISO_C_BINDING interfaces of libforge_hip.so (include/forge_hf.h).
+!
+! This is the layer a Fortran host uses to call the MI355X engine.  Arrays are
+! passed as the host's own column-major arrays (assumed size), int64 is
+! integer(dp), complex(c_double_complex) is complex(dp).  Every function
+! returns 0 on success, > 0 for an eigensolver failure count, < 0 for an
+! argument or HIP error (text in forge_error_message()).
+module forge_c
+  use iso_c_binding
+  implicit none
+  private
+  public :: forge_params_t, forge_scf_options_t, forge_error_message
+  public :: forge_device_count, forge_ctx_create, forge_ctx_destroy, forge_longrange
+  public :: forge_tb_hamiltonian, forge_hamiltonian_hf, forge_zdiagred, forge_solve
+  public :: forge_getfock, forge_energies, forge_scf_set_options, forge_scf_init
+  public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
+
+  ! struct forge_params (Setup.f90:6-74 at run time)
+  type, bind(C) :: forge_params_t
+    integer(c_int64_t) :: ndim, nlayers, numk, numI, numC, nscreen, fphase, nnc
+    integer(c_int64_t) :: nLower, nUpper, numb
+    real(c_double) :: nfilling, U, alpha, alphaH, Delta, xi, tz, a0, r0, d0
+    real(c_double) :: energy_tol, step_alt
+  end type forge_params_t
+
+  ! struct forge_scf_options
+  type, bind(C) :: forge_scf_options_t
+    integer(c_int32_t) :: occ_mode = 0, mix_mode = 0
+    real(c_double) :: kT = 0.0_c_double, beta = 0.3_c_double, etol = 0.0_c_double
+    integer(c_int32_t) :: precision = 0, timing = 0
+  end type forge_scf_options_t
+
+  interface
+    function forge_last_error() bind(C, name="forge_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function forge_last_error
+
+    function forge_device_count() bind(C, name="forge_device_count") result(r)
+      import :: c_int
+      integer(c_int) :: r
+    end function forge_device_count
+
+    function forge_ctx_create(p, Coords, tn, MomentaValues, nMomentaComponents, nUnitCell_1, &
+                              nUnitCell_2, ctx) bind(C, name="forge_ctx_create") result(r)
+      import :: c_int, c_double, c_int64_t, c_ptr, forge_params_t
+      type(forge_params_t), intent(in) :: p
+      real(c_double), intent(in) :: Coords(*), tn(*), MomentaValues(*)
+      integer(c_int64_t), intent(in) :: nMomentaComponents(*), nUnitCell_1(*), nUnitCell_2(*)
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: r
+    end function forge_ctx_create
+
+    function forge_ctx_destroy(ctx) bind(C, name="forge_ctx_destroy") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: r
+    end function forge_ctx_destroy
+
+    function forge_longrange(ctx, LongRange) bind(C, name="forge_longrange") result(r)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: LongRange(*)
+      integer(c_int) :: r
+    end function forge_longrange
+
+    function forge_tb_hamiltonian(ctx, zH, vk) bind(C, name="forge_tb_hamiltonian") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: zH(*)
+      real(c_double), intent(in) :: vk(2)
+      integer(c_int) :: r
+    end function forge_tb_hamiltonian
+
+    function forge_hamiltonian_hf(ctx, zH, Potential, zFock, vk) &
+        bind(C, name="forge_hamiltonian_hf") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: zH(*)
+      real(c_double), intent(in) :: Potential(*), vk(2)
+      complex(c_double_complex), intent(in) :: zFock(*)
+      integer(c_int) :: r
+    end function forge_hamiltonian_hf
+
+    function forge_zdiagred(A, n, eigenvalues, il, iu, info) bind(C, name="forge_zdiagred") result(r)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex
+      complex(c_double_complex), intent(inout) :: A(*)
+      integer(c_int64_t), value :: n, il, iu
+      real(c_double), intent(out) :: eigenvalues(*)
+      integer(c_int32_t), intent(out) :: info
+      integer(c_int) :: r
+    end function forge_zdiagred
+
+    function forge_solve(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors) &
+        bind(C, name="forge_solve") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: zFockBulk(*), zEigenvectors(*)
+      complex(c_double_complex), intent(in) :: zFock(*)
+      real(c_double), intent(in) :: Potential(*)
+      real(c_double), intent(out) :: Bands(*)
+      integer(c_int) :: r
+    end function forge_solve
+
+    function forge_getfock(ctx, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, OccStates, &
+                           PartOccStates) bind(C, name="forge_getfock") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: zFock(*)
+      complex(c_double_complex), intent(in) :: zSortedEigenvectors(*)
+      integer(c_int64_t), intent(in) :: nSortedMomenta(*)
+      real(c_double), value :: DegFactor
+      integer(c_int64_t), value :: OccStates, PartOccStates
+      integer(c_int) :: r
+    end function forge_getfock
+
+    function forge_energies(ctx, zFock, dens, out4) bind(C, name="forge_energies") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: zFock(*)
+      real(c_double), intent(in) :: dens(*)
+      real(c_double), intent(out) :: out4(4)
+      integer(c_int) :: r
+    end function forge_energies
+
+    function forge_scf_set_options(ctx, opt) bind(C, name="forge_scf_set_options") result(r)
+      import :: c_int, c_ptr, forge_scf_options_t
+      type(c_ptr), value :: ctx
+      type(forge_scf_options_t), intent(in) :: opt
+      integer(c_int) :: r
+    end function forge_scf_set_options
+
+    ! rho_start: c_loc of the ndim x ndim x nnc start density, or c_null_ptr
+    ! for the first diagonalisation
+    function forge_scf_init(ctx, rho_start) bind(C, name="forge_scf_init") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rho_start
+      integer(c_int) :: r
+    end function forge_scf_init
+
+    function forge_scf_run(ctx, max_iter, traj, iters_done) bind(C, name="forge_scf_run") result(r)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: max_iter
+      real(c_double), intent(out) :: traj(*)
+      integer(c_int64_t), intent(out) :: iters_done
+      integer(c_int) :: r
+    end function forge_scf_run
+
+    function forge_scf_get_density(ctx, rho) bind(C, name="forge_scf_get_density") result(r)
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: rho(*)
+      integer(c_int) :: r
+    end function forge_scf_get_density
+
+    function forge_scf_timings(ctx, out16) bind(C, name="forge_scf_timings") result(r)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out16(16)
+      integer(c_int) :: r
+    end function forge_scf_timings
+  end interface
+
+contains
+
+  ! forge_last_error() as a Fortran string
+  function forge_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    interface
+      function c_strlen(str) bind(C, name="strlen") result(len)
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: str
+        integer(c_size_t) :: len
+      end function c_strlen
+    end interface
+    p = forge_last_error()
+    if (.not. c_associated(p)) then
+      msg = ""
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function forge_error_message
+
+end module forge_c

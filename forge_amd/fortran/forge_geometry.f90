@@ -1,0 +1,187 @@
+! forge_geometry.f90 -- geometry of the commensurate twisted multilayer cell.
+!
+! Same results as the reference's Geometry module, routine by routine:
+! NumberNeighborCells (Geometry.f90:14-35), OrderNeighborCells (:40-91),
+! SampleBZ (:96-127) and the site list of WignerSeitzCell (:132-273) plus the
+! frame rotation of Main.f90:123-154.  The site order fixes the matrix index
+! order of every array exchanged with the engine, so sites are emitted in the
+! reference's order: per layer the A sublattice (n1 outer, n2 inner), its
+! zone-boundary site, the B sublattice, its boundary site.
+module forge_geometry
+  use forge_setup
+  implicit none
+  private
+  public :: NumberNeighborCells, OrderNeighborCells, SampleBZ, build_cell
+
+contains
+
+  ! Number of stored neighbour cells: half of the shell cells plus the home cell.
+  function NumberNeighborCells(numI_) result(nnc)
+    integer(dp), intent(in) :: numI_
+    integer(dp) :: nnc, n1, n2, c
+    c = 0
+    do n1 = -numI_, numI_
+      do n2 = -numI_, numI_
+        if (n1 + n2 >= -numI_ .and. n1 + n2 <= numI_ .and. (n1 /= 0 .or. n2 /= 0)) c = c + 1
+      end do
+    end do
+    nnc = c / 2 + 1
+  end function NumberNeighborCells
+
+  ! Stored cells: (0,0), then (-1,0), (-1,1), (0,-1), then the remaining
+  ! shell cells in scan order for numI > 1.
+  subroutine OrderNeighborCells(nUnitCell_1, nUnitCell_2, numI_, nnc)
+    integer(dp), intent(in) :: numI_, nnc
+    integer(dp), intent(out) :: nUnitCell_1(nnc), nUnitCell_2(nnc)
+    integer(dp) :: n1, n2, ic
+    logical :: first3
+    nUnitCell_1 = 0
+    nUnitCell_2 = 0
+    ic = 1
+    if (numI_ > 0) then
+      nUnitCell_1(2:4) = [-1_dp, -1_dp, 0_dp]
+      nUnitCell_2(2:4) = [0_dp, 1_dp, -1_dp]
+      ic = 4
+    end if
+    if (numI_ > 1) then
+      do n1 = -numI_, numI_
+        do n2 = -numI_, numI_
+          if (n1 + n2 >= -numI_ .and. n1 + n2 <= numI_ .and. ic < nnc .and. &
+              (n1 /= 0 .or. n2 /= 0)) then
+            first3 = (n1 == -1 .and. n2 == 0) .or. (n1 == -1 .and. n2 == 1) .or. &
+                     (n1 == 0 .and. n2 == -1)
+            if (.not. first3) then
+              ic = ic + 1
+              nUnitCell_1(ic) = n1
+              nUnitCell_2(ic) = n2
+            end if
+          end if
+        end do
+      end do
+    end if
+  end subroutine OrderNeighborCells
+
+  ! Gamma-centred k grid k = (i1 g1 + i2 g2) / numk, i1, i2 = 0..numk-1;
+  ! nMomentaComponents holds the 0-based components, i2 fastest.
+  subroutine SampleBZ(nMomentaComponents, MomentaValues, numk_, g1, g2)
+    integer(dp), intent(in) :: numk_
+    real(dp), intent(in) :: g1(2), g2(2)
+    integer(dp), intent(out) :: nMomentaComponents(numk_ * numk_, 2)
+    real(dp), intent(out) :: MomentaValues(numk_, numk_, 2)
+    integer(dp) :: i1, i2, ic
+    ic = 0
+    do i1 = 0, numk_ - 1
+      do i2 = 0, numk_ - 1
+        ic = ic + 1
+        nMomentaComponents(ic, 1) = i1
+        nMomentaComponents(ic, 2) = i2
+      end do
+    end do
+    do i2 = 0, numk_ - 1
+      do i1 = 0, numk_ - 1
+        MomentaValues(i1 + 1, i2 + 1, 1) = (i1 * g1(1) + i2 * g2(1)) / real(numk_, dp)
+        MomentaValues(i1 + 1, i2 + 1, 2) = (i1 * g1(2) + i2 * g2(2)) / real(numk_, dp)
+      end do
+    end do
+  end subroutine SampleBZ
+
+  ! Site test of the Wigner-Seitz cell in lattice coordinates (a, b).
+  logical function inside_cell(a, b, nt)
+    real(dp), intent(in) :: a, b
+    integer(dp), intent(in) :: nt
+    real(dp) :: rmax, cA, cB, eps6
+    eps6 = real(1.0e-6, dp)   ! a default-real literal, as in the reference
+    rmax = 3.0_dp * nt * nt + 3.0_dp * nt + 1.0_dp
+    cA = 3.0_dp * nt + 1
+    cB = 3.0_dp * nt + 2
+    inside_cell = (abs(a * cA + b * cB) + eps6 < rmax) .and. (abs(a - b * cA) + eps6 < rmax) &
+                  .and. (abs(a * cB + b) + eps6 < rmax)
+  end function inside_cell
+
+  ! Appends one sublattice of one layer to xy(:, 1:nsite).
+  subroutine add_sublattice(xy, nsite, nt, rot, cs, sn, s1, s2)
+    real(dp), intent(inout) :: xy(:, :)
+    integer(dp), intent(inout) :: nsite
+    integer(dp), intent(in) :: nt, rot
+    real(dp), intent(in) :: cs, sn, s1, s2   ! sublattice offsets (1/3, -2/3) or (2/3, -1/3)
+    integer(dp) :: nrad, i1, i2
+    real(dp) :: n1, n2, an1, an2, sq3
+    nrad = 3 * nt
+    sq3 = sqrt(3.0_dp)
+    do i1 = -nrad, nrad
+      do i2 = -nrad, nrad
+        n1 = real(i1, dp)
+        n2 = real(i2, dp)
+        if (rot == -1) then
+          an1 = n1 + s1
+          an2 = n2 + s2
+        else
+          an1 = (n1 + s1) * (cs - sn / sq3) - 2 * (n2 + s2) * sn / sq3
+          an2 = (n2 + s2) * (cs + sn / sq3) + 2 * (n1 + s1) * sn / sq3
+        end if
+        if (inside_cell(an1, an2, nt)) then
+          nsite = nsite + 1
+          xy(1, nsite) = an1 * a1(1) + an2 * a2(1)
+          xy(2, nsite) = an1 * a1(2) + an2 * a2(2)
+        end if
+      end do
+    end do
+  end subroutine add_sublattice
+
+  ! Coordinates of the moire cell (ndim x 3, rotated frame), lattice vectors
+  ! t1, t2, t3 = t2 - t1 and reciprocal vectors g1, g12 of the rotated frame.
+  subroutine build_cell(Coords, t1, t2, t3, g1, g12)
+    real(dp), intent(out) :: Coords(ndim, 3), t1(2), t2(2), t3(2), g1(2), g12(2)
+    real(dp), allocatable :: xy(:, :)
+    real(dp) :: aM, cs, sn, h, R(2, 2), u1(2), u2(2), z, x, y
+    integer(dp) :: l, nsite, start, i
+    aM = 3.0_dp * ntheta * ntheta + 3.0_dp * ntheta + 1.0_dp
+    g1 = (4.0_dp * pi / 3.0_dp) / aM * ((3 * ntheta + 1) * a1 + a2)
+    g12 = (4.0_dp * pi / 3.0_dp) / aM * ((3 * ntheta + 2) * a2 - a1)
+    cs = 1.0_dp - 1.0_dp / (2.0_dp * aM)
+    sn = sqrt(1.0_dp - cs * cs)
+    h = 0.5_dp * acos(cs)
+    R(1, :) = [cos(h), sin(h)]
+    R(2, :) = [-sin(h), cos(h)]
+    g1 = [R(1, 1) * g1(1) + R(1, 2) * g1(2), R(2, 1) * g1(1) + R(2, 2) * g1(2)]
+    g12 = [R(1, 1) * g12(1) + R(1, 2) * g12(2), R(2, 1) * g12(1) + R(2, 2) * g12(2)]
+    u1 = ntheta * a1 + (ntheta + 1) * a2
+    u2 = (-ntheta - 1) * a1 + (2 * ntheta + 1) * a2
+    allocate(xy(2, ndim))
+    nsite = 0
+    do l = 1, nlayers
+      start = nsite + 1
+      if (RotateLayers(l) == -1) then
+        call add_sublattice(xy, nsite, ntheta, -1_dp, cs, sn, 1.0_dp / 3.0_dp, -2.0_dp / 3.0_dp)
+        nsite = nsite + 1
+        xy(:, nsite) = [(-u2(1) - u1(1)) / 3.0_dp, (-u2(2) - u1(2)) / 3.0_dp]
+        call add_sublattice(xy, nsite, ntheta, -1_dp, cs, sn, 2.0_dp / 3.0_dp, -1.0_dp / 3.0_dp)
+        nsite = nsite + 1
+        xy(:, nsite) = [(u2(1) + u1(1)) / 3.0_dp, (u2(2) + u1(2)) / 3.0_dp]
+      else
+        call add_sublattice(xy, nsite, ntheta, 1_dp, cs, sn, 1.0_dp / 3.0_dp, -2.0_dp / 3.0_dp)
+        nsite = nsite + 1
+        xy(:, nsite) = [(u1(1) + u2(1)) / 3.0_dp, (u1(2) + u2(2)) / 3.0_dp]
+        call add_sublattice(xy, nsite, ntheta, 1_dp, cs, sn, 2.0_dp / 3.0_dp, -1.0_dp / 3.0_dp)
+        nsite = nsite + 1
+        xy(:, nsite) = [(-u1(1) - u2(1)) / 3.0_dp, (-u1(2) - u2(2)) / 3.0_dp]
+      end if
+      z = real(real(l - 1) - real(nlayers - 1) / 2.0, dp) * tz
+      Coords(start:nsite, 3) = z
+    end do
+    if (nsite /= ndim) then
+      write(*, *) 'build_cell: the Wigner-Seitz cell holds', nsite, 'sites, expected', ndim
+      error stop 1
+    end if
+    do i = 1, ndim
+      x = R(1, 1) * xy(1, i) + R(1, 2) * xy(2, i)
+      y = R(2, 1) * xy(1, i) + R(2, 2) * xy(2, i)
+      Coords(i, 1) = x
+      Coords(i, 2) = y
+    end do
+    t1 = [R(1, 1) * u1(1) + R(1, 2) * u1(2), R(2, 1) * u1(1) + R(2, 2) * u1(2)]
+    t2 = [R(1, 1) * u2(1) + R(1, 2) * u2(2), R(2, 1) * u2(1) + R(2, 2) * u2(2)]
+    t3 = t2 - t1
+  end subroutine build_cell
+
+end module forge_geometry

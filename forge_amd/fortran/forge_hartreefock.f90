@@ -1,0 +1,320 @@
+! forge_hartreefock.f90 -- drop-in shims for the reference's hot-path routines.
+!
+! Each subroutine keeps the name and the dummy-argument list of the reference
+! routine it replaces, so a host written against the reference calls it
+! unchanged; the body calls the MI355X engine through forge_c.  Arguments
+! the engine already holds (the geometry, uploaded once by forge_ctx_create)
+! are accepted and checked for size only.  numS = 1.
+!
+!   LongRangeInteraction     HartreeFock.f90:14-53
+!   HamiltonianHartreeFock   HartreeFock.f90:57-127
+!   Solve                    HartreeFock.f90:402-483
+!   GetFock                  HartreeFock.f90:487-560
+!   SortEnergies_1Spin       HartreeFock.f90:885-925   (host: 20 Nk keys)
+!   GetKineticEnergy         HartreeFock.f90:1012-1059
+!   GetHartreeHubbardEnergy  HartreeFock.f90:1066-1092
+!   GetFockEnergy            HartreeFock.f90:1098-1145
+!   OptimalStep              HartreeFock.f90:1148-1227
+!   TightBindingHamiltonian  TightBinding.f90:13-55
+!   zdiagRed                 LapackRoutines.f90:381-433
+module forge_hartreefock
+  use iso_c_binding
+  use forge_c
+  use forge_setup
+  implicit none
+  private
+  public :: forge_attach, forge_check, LongRangeInteraction, HamiltonianHartreeFock, Solve, GetFock
+  public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
+  public :: OptimalStep, TightBindingHamiltonian, zdiagRed
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer(dp), save :: ctx_ndim = -1, ctx_nnc = -1
+
+  interface
+    function forge_hartree_hubbard(c, dens1, dens2, out2) bind(C, name="forge_hartree_hubbard") &
+        result(r)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: dens1(*), dens2(*)
+      real(c_double), intent(out) :: out2(2)
+      integer(c_int) :: r
+    end function forge_hartree_hubbard
+  end interface
+
+contains
+
+  ! The engine context the shims call (created by the host with forge_ctx_create).
+  subroutine forge_attach(c, ndim_, nnc_)
+    type(c_ptr), intent(in) :: c
+    integer(dp), intent(in) :: ndim_, nnc_
+    ctx = c
+    ctx_ndim = ndim_
+    ctx_nnc = nnc_
+  end subroutine forge_attach
+
+  ! Stops with the engine's message on a negative return code.
+  subroutine forge_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc < 0) then
+      write(*, '(A,A,A,I0,A,A)') 'forge: ', where, ' failed (', rc, '): ', forge_error_message()
+      error stop 2
+    end if
+  end subroutine forge_check
+
+  subroutine need(ndim_, nnc_, where)
+    integer(dp), intent(in) :: ndim_, nnc_
+    character(*), intent(in) :: where
+    if (.not. c_associated(ctx)) then
+      write(*, *) 'forge: ', where, ' called before forge_attach'
+      error stop 2
+    end if
+    if (ndim_ /= ctx_ndim .or. (nnc_ >= 0 .and. nnc_ /= ctx_nnc)) then
+      write(*, *) 'forge: ', where, ': sizes', ndim_, nnc_, 'differ from the context', ctx_ndim, ctx_nnc
+      error stop 2
+    end if
+  end subroutine need
+
+  subroutine LongRangeInteraction(LongRange, Coords, ndim_, t1, t2)
+    integer(dp), intent(in) :: ndim_
+    real(dp), intent(in) :: t1(2), t2(2), Coords(ndim_, 3)
+    real(dp), intent(inout) :: LongRange(ndim_, ndim_)
+    call need(ndim_, -1_dp, 'LongRangeInteraction')
+    call forge_check(forge_longrange(ctx, LongRange), 'LongRangeInteraction')
+  end subroutine LongRangeInteraction
+
+  subroutine HamiltonianHartreeFock(zH, Coords, Potential, alpha_, Delta_, zFock, nUnitCell_1, &
+                                    nUnitCell_2, ndim_, numNeighborCells, vk, tn)
+    integer(dp), intent(in) :: ndim_, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), vk(2), Coords(ndim_, 3)
+    real(dp), intent(in) :: alpha_, Delta_, Potential(ndim_)
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
+    complex(dp), intent(out) :: zH(ndim_, ndim_)
+    call need(ndim_, numNeighborCells, 'HamiltonianHartreeFock')
+    call forge_check(forge_hamiltonian_hf(ctx, zH, Potential, zFock, vk), 'HamiltonianHartreeFock')
+  end subroutine HamiltonianHartreeFock
+
+  subroutine TightBindingHamiltonian(zH, Delta_, Coords, nUnitCell_1, nUnitCell_2, ndim_, &
+                                     numNeighborCells, vk, tn)
+    integer(dp), intent(in) :: ndim_, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), vk(2), Coords(ndim_, 3), Delta_
+    complex(dp), intent(out) :: zH(ndim_, ndim_)
+    call need(ndim_, numNeighborCells, 'TightBindingHamiltonian')
+    call forge_check(forge_tb_hamiltonian(ctx, zH, vk), 'TightBindingHamiltonian')
+  end subroutine TightBindingHamiltonian
+
+  ! Eigenpairs nLower..nUpper of the Hermitian matrix in A's lower triangle;
+  ! A(:, 1:nUpper-nLower+1) receives the eigenvectors.  A failure count is
+  ! reported as the reference reports zheevr's info, and the run continues.
+  subroutine zdiagRed(A, eigenvalues, ev, nLower_, nUpper_)
+    complex(8), intent(inout) :: A(:, :)
+    real(8), intent(inout) :: eigenvalues(:)
+    character(1), intent(in) :: ev
+    integer(8), intent(in) :: nLower_, nUpper_
+    integer(c_int32_t) :: info
+    integer(c_int) :: rc
+    rc = forge_zdiagred(A, int(size(A, 1), c_int64_t), eigenvalues, nLower_, nUpper_, info)
+    call forge_check(rc, 'zdiagRed')
+    if (info /= 0) write(*, *) 'zheev failed', info
+  end subroutine zdiagRed
+
+  subroutine Solve(zFockBulk, zFock, Potential, alpha_, Bands, zEigenvectors, Coords, MomentaValues, &
+                   nMomentaComponents, nLower_, nUpper_, ndim_, numk_, Nk_, numNeighborCells, &
+                   nUnitCell_1, nUnitCell_2, tn)
+    integer(dp), intent(in) :: ndim_, numk_, Nk_, nLower_, nUpper_, numNeighborCells
+    integer(dp), intent(in) :: nMomentaComponents(Nk_, 2)
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), MomentaValues(numk_, numk_, 2), Coords(ndim_, 3)
+    real(dp), intent(in) :: Potential(ndim_), alpha_
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
+    real(dp), intent(out) :: Bands(nUpper_ - nLower_ + 1, Nk_)
+    complex(dp), intent(out) :: zEigenvectors(ndim_, nUpper_ - nLower_ + 1, Nk_)
+    complex(dp), intent(out) :: zFockBulk(ndim_, ndim_, numNeighborCells)
+    integer(c_int) :: rc
+    call need(ndim_, numNeighborCells, 'Solve')
+    rc = forge_solve(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors)
+    call forge_check(rc, 'Solve')
+    if (rc > 0) write(*, *) 'zheev failed', rc
+  end subroutine Solve
+
+  subroutine GetFock(zFock, zSortedEigenvectors, ndim_, numb_, numk_, DegFactor, OccStates, &
+                     PartOccStates, Coords, MomentaValues, numNeighborCells, nUnitCell_1, &
+                     nUnitCell_2, nSortedMomenta)
+    integer(dp), intent(in) :: ndim_, numb_, numk_, OccStates, PartOccStates, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    integer(dp), intent(in) :: nSortedMomenta(numb_ * numk_ * numk_, 2)
+    real(dp), intent(in) :: Coords(ndim_, 3), MomentaValues(numk_, numk_, 2), DegFactor
+    complex(dp), intent(in) :: zSortedEigenvectors(ndim_, numb_ * numk_ * numk_)
+    complex(dp), intent(inout) :: zFock(ndim_, ndim_, numNeighborCells)
+    call need(ndim_, numNeighborCells, 'GetFock')
+    call forge_check(forge_getfock(ctx, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, &
+                                   OccStates, PartOccStates), 'GetFock')
+  end subroutine GetFock
+
+  ! Window states of all k flattened band-major (index (b-1) Nk + k) and
+  ! sorted by energy (heap sort); vectors and momenta follow their energies.
+  subroutine SortEnergies_1Spin(ndim_, numk_, numb_, Nk_, nMomentaComponents, zSortedEigenvectors, &
+                                SortedEnergies_1Spin, nSortedMomenta, zEigenvectors, Bands)
+    integer(dp), intent(in) :: numb_, numk_, Nk_, ndim_
+    integer(dp), intent(in) :: nMomentaComponents(Nk_, 2)
+    real(dp), intent(in) :: Bands(numb_, Nk_)
+    complex(dp), intent(in) :: zEigenvectors(ndim_, numb_, Nk_)
+    integer(dp), intent(inout) :: nSortedMomenta(numb_ * numk_ * numk_, 2)
+    real(dp), intent(inout) :: SortedEnergies_1Spin(numb_ * numk_ * numk_)
+    complex(dp), intent(inout) :: zSortedEigenvectors(ndim_, numb_ * numk_ * numk_)
+    integer(dp) :: N, i, b, k, ic
+    integer(dp), allocatable :: idx(:)
+    real(dp), allocatable :: key(:)
+    N = numb_ * Nk_
+    allocate(idx(N), key(N))
+    do b = 1, numb_
+      do k = 1, Nk_
+        ic = (b - 1) * Nk_ + k
+        key(ic) = Bands(b, k)
+        idx(ic) = ic
+      end do
+    end do
+    call heap_sort(key, idx, N)
+    do i = 1, N
+      b = (idx(i) - 1) / Nk_ + 1
+      k = idx(i) - (b - 1) * Nk_
+      SortedEnergies_1Spin(i) = key(i)
+      zSortedEigenvectors(:, i) = zEigenvectors(:, b, k)
+      nSortedMomenta(i, :) = nMomentaComponents(k, :)
+    end do
+  end subroutine SortEnergies_1Spin
+
+  ! Ascending heap sort of key with idx carried along.
+  subroutine heap_sort(key, idx, N)
+    integer(dp), intent(in) :: N
+    real(dp), intent(inout) :: key(N)
+    integer(dp), intent(inout) :: idx(N)
+    integer(dp) :: i, last
+    do i = N / 2, 1, -1
+      call sift(i, N)
+    end do
+    do last = N, 2, -1
+      call swap(1_dp, last)
+      call sift(1_dp, last - 1)
+    end do
+  contains
+    subroutine sift(root, hi)
+      integer(dp), intent(in) :: root, hi
+      integer(dp) :: r, c
+      r = root
+      do
+        c = 2 * r
+        if (c > hi) exit
+        if (c < hi) then
+          if (key(c + 1) > key(c)) c = c + 1
+        end if
+        if (key(r) >= key(c)) exit
+        call swap(r, c)
+        r = c
+      end do
+    end subroutine sift
+    subroutine swap(a, b)
+      integer(dp), intent(in) :: a, b
+      real(dp) :: t
+      integer(dp) :: ti
+      t = key(a); key(a) = key(b); key(b) = t
+      ti = idx(a); idx(a) = idx(b); idx(b) = ti
+    end subroutine swap
+  end subroutine heap_sort
+
+  subroutine GetKineticEnergy(fsum, Coords, zFock, Delta_, nUnitCell_1, nUnitCell_2, ndim_, &
+                              numNeighborCells, tn)
+    integer(dp), intent(in) :: ndim_, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), Coords(ndim_, 3), Delta_
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
+    real(dp), intent(out) :: fsum
+    real(dp) :: e(4), d(ndim_)
+    integer(dp) :: i
+    call need(ndim_, numNeighborCells, 'GetKineticEnergy')
+    ! the layer-bias term reads the site densities of zFock itself
+    do i = 1, ndim_
+      d(i) = real(zFock(i, i, 1), dp) - 0.5_dp
+    end do
+    call forge_check(forge_energies(ctx, zFock, d, e), 'GetKineticEnergy')
+    fsum = e(1)
+  end subroutine GetKineticEnergy
+
+  subroutine GetFockEnergy(fsum, Coords, zFock, nUnitCell_1, nUnitCell_2, ndim_, numNeighborCells, tn)
+    integer(dp), intent(in) :: ndim_, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), Coords(ndim_, 3)
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
+    real(dp), intent(out) :: fsum
+    real(dp) :: e(4), d(ndim_)
+    call need(ndim_, numNeighborCells, 'GetFockEnergy')
+    d = 0.0_dp
+    call forge_check(forge_energies(ctx, zFock, d, e), 'GetFockEnergy')
+    fsum = e(2)
+  end subroutine GetFockEnergy
+
+  subroutine GetHartreeHubbardEnergy(fsumH, fsumU, LongRange, dens1, dens2, ndim_)
+    integer(dp), intent(in) :: ndim_
+    real(dp), intent(in) :: LongRange(ndim_, ndim_)
+    real(dp), intent(in) :: dens1(ndim_), dens2(ndim_)
+    real(dp), intent(out) :: fsumH, fsumU
+    real(dp) :: e(2)
+    call need(ndim_, -1_dp, 'GetHartreeHubbardEnergy')
+    call forge_check(forge_hartree_hubbard(ctx, dens1, dens2, e), 'GetHartreeHubbardEnergy')
+    fsumH = e(1)
+    fsumU = e(2)
+  end subroutine GetHartreeHubbardEnergy
+
+  ! Optimal damping: the total energy is exact-quadratic in the mixing weight;
+  ! fit it through E(0) (the energies passed in), E(1) and E(1/2), take the
+  ! minimiser when it lies in (0, 1), else 1 if E(1) < E(0), else
+  ! StepAlternative; on return the energies are those of the mixed state.
+  subroutine OptimalStep(Step, HartreeEnergy, FockEnergy, HubbardEnergy, KineticEnergy, zFock, &
+                         zFockIn, Density, DensityIn, DensitySub, ndim_, numNeighborCells, Coords, &
+                         LongRange, Delta_, nUnitCell_1, nUnitCell_2, tn)
+    integer(dp), intent(in) :: ndim_, numNeighborCells
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells, numS)
+    complex(dp), intent(in) :: zFockIn(ndim_, ndim_, numNeighborCells, numS)
+    real(dp), intent(in) :: Coords(ndim_, 3), LongRange(ndim_, ndim_), tn(2, 3), Delta_
+    real(dp), intent(in) :: Density(ndim_, numS), DensityIn(ndim_, numS), DensitySub(ndim_)
+    real(dp), intent(inout) :: HartreeEnergy, FockEnergy(numS), HubbardEnergy, KineticEnergy(numS)
+    real(dp), intent(out) :: Step
+    complex(dp), allocatable :: zmix(:, :, :)
+    real(dp) :: d(ndim_), e(4), c0, c1, c2, E1, Eh
+    call need(ndim_, numNeighborCells, 'OptimalStep')
+    c0 = total(FockEnergy(1), KineticEnergy(1), HartreeEnergy, HubbardEnergy)
+    d = Density(:, 1) - DensitySub
+    call forge_check(forge_energies(ctx, zFock, d, e), 'OptimalStep')
+    E1 = total(e(2), e(1), e(3), e(4))
+    allocate(zmix(ndim_, ndim_, numNeighborCells))
+    zmix = 0.5_dp * (zFockIn(:, :, :, 1) + zFock(:, :, :, 1))
+    d = 0.5_dp * (Density(:, 1) + DensityIn(:, 1)) - DensitySub
+    call forge_check(forge_energies(ctx, zmix, d, e), 'OptimalStep')
+    Eh = total(e(2), e(1), e(3), e(4))
+    c1 = 4.0_dp * Eh - E1 - 3.0_dp * c0
+    c2 = E1 - c0 - c1
+    if (c2 > 0.0_dp .and. abs(-c1 / 2.0_dp / c2 - 0.5_dp) < 0.5_dp) then
+      Step = -c1 / 2.0_dp / c2
+    else if (E1 < c0) then
+      Step = 1.0_dp
+    else
+      Step = StepAlternative
+    end if
+    zmix = (1.0_dp - Step) * zFockIn(:, :, :, 1) + Step * zFock(:, :, :, 1)
+    d = (1.0_dp - Step) * DensityIn(:, 1) + Step * Density(:, 1) - DensitySub
+    call forge_check(forge_energies(ctx, zmix, d, e), 'OptimalStep')
+    KineticEnergy(1) = e(1)
+    FockEnergy(1) = e(2)
+    HartreeEnergy = e(3)
+    HubbardEnergy = e(4)
+  contains
+    real(dp) function total(ef, ek, eh_, eu)
+      real(dp), intent(in) :: ef, ek, eh_, eu
+      total = alpha * ef * real(3 - numS, dp) + ek * real(3 - numS, dp) + U * eu + alphaH * eh_
+    end function total
+  end subroutine OptimalStep
+
+end module forge_hartreefock

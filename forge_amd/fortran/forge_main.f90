@@ -1,0 +1,351 @@
+! forge_main.f90 -- Fortran host of the MI355X Hartree-Fock engine.
+!
+! Reproduces the call sequence of the reference driver (Main.f90:79-679) for
+! the spin-singlet case: geometry, LongRange, initial state (first
+! diagonalisation or the checkpoint read of Main.f90:187-235), energies and
+! potential, then the SCF loop (Solve, SortEnergies_1Spin, Fermi level,
+! GetFock, OptimalStep, mixing, potential) with the Energy and Convergence
+! rows of Main.f90:616-639.
+!
+!   mode=reference  every hot-path routine is the drop-in shim of
+!                   forge_hartreefock, called exactly where the reference calls
+!                   its own routine; host arrays cross the C ABI on each call;
+!   mode=device     the loop body runs device-resident (forge_scf_run); the
+!                   host only sets options and collects the rows;
+!   mode=geometry   writes the geometry and stops (no device needed).
+!
+! Arguments (key=value): ntheta nlayers numk nfilling itmax xi_nm epsilon U
+! Delta mode start (first | path of a checkpoint in the reference's Fock
+! format) occ mix kT beta etol out (trajectory file).
+program forge_main
+  use iso_c_binding
+  use forge_c
+  use forge_setup
+  use forge_geometry
+  use forge_hartreefock
+  implicit none
+
+  character(len=16) :: mode = 'reference'
+  character(len=512) :: start = 'first', outfile = 'forge_traj.txt'
+  integer(c_int32_t) :: occ = 0, mix = 0
+  real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp
+  integer(dp) :: nnc, it, n, nFermiLevel, nOcc, nPart, icount, i, nU
+  integer(dp), allocatable :: nUnitCell_1(:), nUnitCell_2(:), nMomentaComponents(:, :)
+  integer(dp), allocatable :: nSortedMomenta(:, :, :)
+  real(dp), allocatable :: Coords(:, :), MomentaValues(:, :, :), LongRange(:, :)
+  real(dp), allocatable :: Potential(:, :), Density(:, :), DensityIn(:, :), DensitySub(:)
+  real(dp), allocatable :: Bands(:, :, :), SortedEnergies(:, :), traj(:, :)
+  complex(dp), allocatable, target :: zFock(:, :, :, :)
+  complex(dp), allocatable :: zFockIn(:, :, :, :), zFockBulk(:, :, :, :)
+  complex(dp), allocatable :: zEigenvectors(:, :, :, :), zSorted(:, :, :)
+  real(dp) :: t1(2), t2(2), t3(2), g1(2), g12(2), tn(2, 3)
+  real(dp) :: KineticEnergy(numS), FockEnergy(numS), HartreeEnergy, HubbardEnergy, TotalEnergy
+  real(dp) :: KineticEnergyIn(numS), FockEnergyIn(numS), HartreeEnergyIn, HubbardEnergyIn
+  real(dp) :: TotalEnergyIn, FermiEnergy, DegFactor, Step, DensityConvergence
+  type(c_ptr) :: ctx
+  type(forge_scf_options_t) :: opt
+  integer(c_int64_t) :: done
+  integer :: fout
+
+  call parse_args()
+  call setup_init()
+  n = ndim
+  nU = nUpper - nLower + 1
+
+  ! ---- geometry (Main.f90:119-160)
+  nnc = NumberNeighborCells(numI)
+  allocate(nUnitCell_1(nnc), nUnitCell_2(nnc))
+  call OrderNeighborCells(nUnitCell_1, nUnitCell_2, numI, nnc)
+  allocate(Coords(n, 3), MomentaValues(numk, numk, 2), nMomentaComponents(Nk, 2))
+  call build_cell(Coords, t1, t2, t3, g1, g12)
+  call SampleBZ(nMomentaComponents, MomentaValues, numk, g1, g12)
+  tn = reshape([t1, t2, t3], [2, 3])
+  if (trim(mode) == 'geometry') then
+    call write_geometry()
+  else
+    call run_engine()
+  end if
+
+contains
+
+  subroutine run_engine()
+    if (forge_device_count() < 1) then
+      write(*, *) 'forge_main: no HIP device'
+      error stop 3
+    end if
+    call forge_check(forge_ctx_create(setup_params(nnc), Coords, tn, MomentaValues, &
+                                      nMomentaComponents, nUnitCell_1, nUnitCell_2, ctx), &
+                     'forge_ctx_create')
+    call forge_attach(ctx, n, nnc)
+
+    allocate(zFock(n, n, nnc, numS), zFockIn(n, n, nnc, numS))
+    allocate(Density(n, numS), DensityIn(n, numS), DensitySub(n), Potential(n, numS))
+    DensitySub = 0.5_dp
+    Potential = 0.0_dp
+    zFock = (0.0_dp, 0.0_dp)
+    if (trim(start) /= 'first') call read_fock(trim(start))
+
+    open(newunit=fout, file=trim(outfile), status='replace')
+    write(fout, '(A,I0,A,I0,A,I0,A,F4.1,A,A)') '# forge_main ntheta=', ntheta, ' nlayers=', nlayers, &
+        ' numk=', numk, ' nfilling=', nfilling, ' mode=', trim(mode)
+
+    if (trim(mode) == 'device') then
+      opt%occ_mode = occ; opt%mix_mode = mix; opt%kT = kT; opt%beta = beta; opt%etol = etol
+      call forge_check(forge_scf_set_options(ctx, opt), 'forge_scf_set_options')
+      if (trim(start) == 'first') then
+        call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
+      else
+        call forge_check(forge_scf_init(ctx, c_loc(zFock)), 'forge_scf_init')
+      end if
+      allocate(traj(12, itmax))
+      call forge_check(forge_scf_run(ctx, int(itmax, c_int64_t), traj, done), 'forge_scf_run')
+      write(fout, '(A)') '# Energy: it Etot 2Ekin 2EF EH EU'
+      do i = 1, done
+        write(fout, '(F5.1,3X,5(F24.9,3X))') traj(1, i), traj(2:6, i)
+      end do
+      write(fout, '(A)') '# Convergence: it maxdDensity Step dE'
+      do i = 1, done
+        write(fout, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') traj(1, i), traj(7, i), traj(8, i), traj(9, i)
+      end do
+    else
+      call reference_sequence()
+    end if
+    close(fout)
+    call forge_check(forge_ctx_destroy(ctx), 'forge_ctx_destroy')
+  end subroutine run_engine
+
+  subroutine reference_sequence()
+    real(dp), allocatable :: erow(:, :), crow(:, :)
+    allocate(LongRange(n, n), zFockBulk(n, n, nnc, numS), Bands(numb, Nk, numS))
+    allocate(zEigenvectors(n, numb, Nk, numS), zSorted(n, numb * Nk, numS))
+    allocate(SortedEnergies(numb * Nk, numS), nSortedMomenta(numb * Nk, 2, numS))
+    allocate(erow(6, itmax), crow(4, itmax))
+    call LongRangeInteraction(LongRange, Coords, n, t1, t2)
+
+    ! initial state: first diagonalisation (Main.f90:236-288) unless read
+    if (trim(start) == 'first') then
+      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
+                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+                 nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
+      call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
+                              SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
+                              zEigenvectors(:, :, :, 1), Bands(:, :, 1))
+      call fermi_level(SortedEnergies(:, 1), 1_dp)
+      zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
+      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
+                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+      zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+    end if
+    do i = 1, n
+      Density(i, 1) = real(zFock(i, i, 1, 1), dp)
+    end do
+    ! energies and potential of the initial state (Main.f90:290-327)
+    call GetKineticEnergy(KineticEnergy(1), Coords, zFock(:, :, :, 1), Delta, nUnitCell_1, &
+                          nUnitCell_2, n, nnc, tn)
+    call GetFockEnergy(FockEnergy(1), Coords, zFock(:, :, :, 1), nUnitCell_1, nUnitCell_2, n, nnc, tn)
+    call GetHartreeHubbardEnergy(HartreeEnergy, HubbardEnergy, LongRange, Density(:, 1) - DensitySub, &
+                                 Density(:, 1) - DensitySub, n)
+    TotalEnergy = total_energy()
+    call update_potential()
+    DensityIn = Density
+    zFockIn = zFock
+
+    it = 0
+    do while (it < itmax)
+      it = it + 1
+      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
+                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+                 nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
+      call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
+                              SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
+                              zEigenvectors(:, :, :, 1), Bands(:, :, 1))
+      call fermi_level(SortedEnergies(:, 1), numS)
+      zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
+      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
+                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+      zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+
+      HartreeEnergyIn = HartreeEnergy
+      FockEnergyIn = FockEnergy
+      HubbardEnergyIn = HubbardEnergy
+      KineticEnergyIn = KineticEnergy
+      TotalEnergyIn = TotalEnergy
+      do i = 1, n
+        Density(i, 1) = real(zFock(i, i, 1, 1), dp)
+      end do
+      if (mix == 0) then
+        call OptimalStep(Step, HartreeEnergy, FockEnergy, HubbardEnergy, KineticEnergy, zFock, &
+                         zFockIn, Density, DensityIn, DensitySub, n, nnc, Coords, LongRange, Delta, &
+                         nUnitCell_1, nUnitCell_2, tn)
+        TotalEnergy = total_energy()
+        zFock = Step * zFock + (1.0_dp - Step) * zFockIn
+      else
+        Step = beta
+        zFock = Step * zFock + (1.0_dp - Step) * zFockIn
+        do i = 1, n
+          Density(i, 1) = real(zFock(i, i, 1, 1), dp)
+        end do
+        call GetKineticEnergy(KineticEnergy(1), Coords, zFock(:, :, :, 1), Delta, nUnitCell_1, &
+                              nUnitCell_2, n, nnc, tn)
+        call GetFockEnergy(FockEnergy(1), Coords, zFock(:, :, :, 1), nUnitCell_1, nUnitCell_2, n, &
+                           nnc, tn)
+        call GetHartreeHubbardEnergy(HartreeEnergy, HubbardEnergy, LongRange, &
+                                     Density(:, 1) - DensitySub, Density(:, 1) - DensitySub, n)
+        TotalEnergy = total_energy()
+      end if
+      do i = 1, n
+        Density(i, 1) = real(zFock(i, i, 1, 1), dp)
+      end do
+      DensityConvergence = maxval(abs(Density(:, 1) - DensityIn(:, 1)))
+      DensityIn = Density
+      zFockIn = zFock
+      call update_potential()
+      erow(:, it) = [real(it, dp), TotalEnergyIn, 2.0_dp * KineticEnergyIn(1), &
+                     2.0_dp * FockEnergyIn(1), HartreeEnergyIn, HubbardEnergyIn]
+      crow(:, it) = [real(it, dp), DensityConvergence, Step, TotalEnergyIn - TotalEnergy]
+      write(*, '(A,I0,A,F20.9,A,ES10.3)') 'it=', it, ' E=', TotalEnergy, ' dE=', &
+          TotalEnergyIn - TotalEnergy
+      if (etol > 0.0_dp .and. abs(TotalEnergyIn - TotalEnergy) < etol) exit
+    end do
+    write(fout, '(A)') '# Energy: it Etot 2Ekin 2EF EH EU'
+    do i = 1, min(it, itmax)
+      write(fout, '(F5.1,3X,5(F24.9,3X))') erow(:, i)
+    end do
+    write(fout, '(A)') '# Convergence: it maxdDensity Step dE'
+    do i = 1, min(it, itmax)
+      write(fout, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') crow(:, i)
+    end do
+  end subroutine reference_sequence
+
+  ! Fermi level and degenerate shell (Main.f90:396-414): states within
+  ! EnergyTolerance of E_F share the remaining electrons equally.
+  subroutine fermi_level(S, mult)
+    real(dp), intent(in) :: S(:)
+    integer(dp), intent(in) :: mult
+    nFermiLevel = mult * nint(real(Nk * NeutralityPoint, dp) + real(Nk, dp) / 2.0_dp * nfilling, dp)
+    FermiEnergy = (S(nFermiLevel) + S(nFermiLevel + 1)) / 2.0_dp
+    nOcc = 0
+    nPart = 0
+    do icount = 1, Nk * numb
+      if (S(icount) - FermiEnergy < -EnergyTolerance) then
+        nOcc = nOcc + 1
+      else if (abs(S(icount) - FermiEnergy) < EnergyTolerance) then
+        nOcc = nOcc + 1
+        nPart = nPart + 1
+      end if
+    end do
+    DegFactor = 1.0_dp / real(nPart, dp) * real(nFermiLevel + nPart - nOcc, dp)
+  end subroutine fermi_level
+
+  real(dp) function total_energy()
+    total_energy = alpha * sum(FockEnergy) * real(3 - numS, dp) + &
+                   sum(KineticEnergy) * real(3 - numS, dp) + U * HubbardEnergy + alphaH * HartreeEnergy
+  end function total_energy
+
+  ! V = 2 alphaH L (D - 1/2) + U (D - 1/2)   (Main.f90:577-585)
+  subroutine update_potential()
+    Potential(:, 1) = 2.0_dp * alphaH * matmul(LongRange, Density(:, 1) - DensitySub) + &
+                      U * (Density(:, 1) - DensitySub)
+  end subroutine update_potential
+
+  ! Density matrix in the reference's checkpoint layout (Main.f90:187-235):
+  ! direct access, one complex(8) per record; cell 1 as its lower triangle by
+  ! rows (the upper triangle is the conjugate), then cells 2..nnc in full.
+  subroutine read_fock(path)
+    character(*), intent(in) :: path
+    integer :: fu, ios
+    integer(dp) :: rec, a, b, m
+    complex(dp) :: z
+    open(newunit=fu, file=path, form='unformatted', status='old', access='direct', recl=16, &
+         iostat=ios)
+    if (ios /= 0) then
+      write(*, *) 'forge_main: cannot open ', path
+      error stop 4
+    end if
+    rec = 0
+    do a = 1, n
+      do b = 1, a - 1
+        rec = rec + 1
+        read(fu, rec=rec) z
+        zFock(a, b, 1, 1) = z
+        zFock(b, a, 1, 1) = conjg(z)
+      end do
+      rec = rec + 1
+      read(fu, rec=rec) z
+      zFock(a, a, 1, 1) = z
+    end do
+    do m = 2, nnc
+      do a = 1, n
+        do b = 1, n
+          rec = rec + 1
+          read(fu, rec=rec) z
+          zFock(a, b, m, 1) = z
+        end do
+      end do
+    end do
+    close(fu)
+  end subroutine read_fock
+
+  ! mode=geometry: the host-side geometry as text (17 significant digits,
+  ! exact on read-back), without touching the device
+  subroutine write_geometry()
+    open(newunit=fout, file=trim(outfile), status='replace')
+    write(fout, '(A,2I8)') '# Coords', n, 3
+    do i = 1, n
+      write(fout, '(3ES26.17E3)') Coords(i, :)
+    end do
+    write(fout, '(A)') '# tn'
+    write(fout, '(6ES26.17E3)') tn
+    write(fout, '(A)') '# g1 g12'
+    write(fout, '(4ES26.17E3)') g1, g12
+    write(fout, '(A)') '# MomentaValues'
+    write(fout, '(4ES26.17E3)') MomentaValues
+    write(fout, '(A)') '# nMomentaComponents'
+    write(fout, '(2I8)') (nMomentaComponents(i, :), i = 1, Nk)
+    write(fout, '(A)') '# nUnitCell'
+    write(fout, '(2I8)') (nUnitCell_1(i), nUnitCell_2(i), i = 1, nnc)
+    close(fout)
+  end subroutine write_geometry
+
+  subroutine parse_args()
+    integer :: k, eq
+    character(len=512) :: arg, key, val
+    do k = 1, command_argument_count()
+      call get_command_argument(k, arg)
+      eq = index(arg, '=')
+      if (eq == 0) then
+        write(*, *) 'forge_main: expected key=value, got ', trim(arg)
+        error stop 1
+      end if
+      key = arg(1:eq - 1)
+      val = arg(eq + 1:)
+      select case (trim(key))
+      case ('ntheta'); read(val, *) ntheta
+      case ('nlayers'); read(val, *) nlayers
+      case ('numk'); read(val, *) numk
+      case ('nfilling'); read(val, *) nfilling
+      case ('itmax'); read(val, *) itmax
+      case ('xi_nm'); read(val, *) xi; xi = xi / 0.246_dp
+      case ('epsilon'); read(val, *) epsilon
+      case ('U'); read(val, *) U
+      case ('Delta'); read(val, *) Delta
+      case ('mode'); mode = val
+      case ('start'); start = val
+      case ('occ'); read(val, *) occ
+      case ('mix'); read(val, *) mix
+      case ('kT'); read(val, *) kT
+      case ('beta'); read(val, *) beta
+      case ('etol'); read(val, *) etol
+      case ('out'); outfile = val
+      case default
+        write(*, *) 'forge_main: unknown key ', trim(key)
+        error stop 1
+      end select
+    end do
+    if (trim(mode) /= 'reference' .and. trim(mode) /= 'device' .and. trim(mode) /= 'geometry') then
+      write(*, *) 'forge_main: mode must be reference, device or geometry'
+      error stop 1
+    end if
+  end subroutine parse_args
+
+end program forge_main

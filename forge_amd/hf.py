@@ -146,6 +146,14 @@ class Engine:
         L.check(self.lib.forge_energies(self.ctx, _p(rho), _p(d), _p(out)), "forge_energies")
         return out
 
+    def hartree_hubbard(self, dens1, dens2):
+        """(fsumH, fsumU) of GetHartreeHubbardEnergy (HartreeFock.f90:1066-1092)."""
+        out = np.zeros(2)
+        d1, d2 = _f(dens1, np.float64), _f(dens2, np.float64)
+        L.check(self.lib.forge_hartree_hubbard(self.ctx, _p(d1), _p(d2), _p(out)),
+                "forge_hartree_hubbard")
+        return out
+
     def total_energy(self, e):
         p = self.p
         return p.alpha * e[1] * 2.0 + e[0] * 2.0 + p.U * e[3] + p.alphaH * e[2]

@@ -47,6 +47,7 @@ _SIGS = {
     "forge_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "forge_getfock": (C.c_int, [vp, vp, vp, vp, dbl, i64, i64]),
     "forge_energies": (C.c_int, [vp, vp, vp, vp]),
+    "forge_hartree_hubbard": (C.c_int, [vp, vp, vp, vp]),
     "forge_scf_set_options": (C.c_int, [vp, vp]),
     "forge_scf_init": (C.c_int, [vp, vp]),
     "forge_scf_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),

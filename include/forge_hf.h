@@ -106,6 +106,11 @@ int forge_getfock(forge_ctx* ctx, forge_z* zFock, const forge_z* zSortedEigenvec
  * density matrix zFock and site densities dens (Density - DensitySub). */
 int forge_energies(forge_ctx* ctx, const forge_z* zFock, const double* dens, double* out4);
 
+/* GetHartreeHubbardEnergy(fsumH, fsumU, LongRange, dens1, dens2, ndim),
+ * HartreeFock.f90:1066-1092, with the context's device copy of LongRange:
+ * out2 = {1/2 (d1+d2).L.(d1+d2), d1.d2}. */
+int forge_hartree_hubbard(forge_ctx* ctx, const double* dens1, const double* dens2, double* out2);
+
 /* ---- device-resident SCF (the loop of Main.f90:364-679 on the device) ---- */
 
 int forge_scf_set_options(forge_ctx* ctx, const forge_scf_options* opt);
@@ -130,7 +135,7 @@ int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
  * [3] back-transformation, [4] density products, [5] energies + mixing,
  * [6] total, [7] launches of the assembly kernel, [8] summed time of the
  * assembly kernel (ms), [9] timed launches of the Householder hemv kernel
- * (every 16th column), [10] their summed time (ms), [11] their algorithmic
+ * (every 17th column), [10] their summed time (ms), [11] their algorithmic
  * bytes (8 m^2 per matrix, m the trailing order). */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
 

@@ -1,0 +1,103 @@
+"""The Fortran host (forge_amd/fortran, built into forge_amd/_lib/forge_main).
+
+CPU: the executable is built and its geometry (its own Fortran restatement of
+Geometry.f90) equals the reference's bit for bit on the fixture cases.
+GPU: both host modes reproduce the reference's Energy/Convergence rows --
+mode=reference calls the drop-in shims where Main.f90 calls the reference
+routines, mode=device runs the loop device-resident; the seeded case goes
+through the reference's checkpoint format (nRead = 1 path).
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from golden_io import case_params, load_case, load_traj
+from forge_amd.build import FORTRAN_EXE
+
+CASES = ["t1k3", "t2k3", "t3k6", "tri1k3", "t2k3xi10"]
+
+
+def _args(c):
+    return [f"ntheta={c['ntheta']}", f"nlayers={c['nlayers']}", f"numk={c['numk']}",
+            f"nfilling={c['nfilling']}", f"xi_nm={c['xi_nm']}"]
+
+
+def _read_geometry(path):
+    """Sections of the geometry file as flat arrays, in the order written
+    (Coords row by row, the rest in Fortran element order)."""
+    sec, out = None, {}
+    for line in open(path):
+        if line.startswith("#"):
+            sec = line.split()[1]
+            out[sec] = []
+        else:
+            out[sec].extend(float(x) for x in line.split())
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def test_fortran_host_is_built():
+    assert os.access(FORTRAN_EXE, os.X_OK), f"{FORTRAN_EXE} missing: run python -m forge_amd.build"
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fortran_geometry_matches_reference_bitwise(case, tmp_path):
+    f = load_case(case)
+    c = case_params(f)
+    out = tmp_path / "geom.txt"
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", *_args(c), f"out={out}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = _read_geometry(out)
+    np.testing.assert_array_equal(g["Coords"].reshape(-1, 3), f["Coords"])
+    np.testing.assert_array_equal(g["tn"], f["tn"].reshape(-1, order="F"))
+    np.testing.assert_array_equal(g["g1"], np.concatenate([f["g1"], f["g12"]]))
+    np.testing.assert_array_equal(g["MomentaValues"], f["MomentaValues"].reshape(-1, order="F"))
+    np.testing.assert_array_equal(g["nMomentaComponents"].reshape(-1, 2), f["nMomentaComponents"])
+    np.testing.assert_array_equal(g["nUnitCell"].reshape(-1, 2)[:, 0], f["nUnitCell_1"])
+    np.testing.assert_array_equal(g["nUnitCell"].reshape(-1, 2)[:, 1], f["nUnitCell_2"])
+
+
+def _traj(path):
+    eng, conv, cur = [], [], None
+    for line in open(path):
+        if line.startswith("# Energy"):
+            cur = eng
+        elif line.startswith("# Convergence"):
+            cur = conv
+        elif line.startswith("#"):
+            continue
+        else:
+            cur.append([float(x) for x in line.split()])
+    return np.array(eng), np.array(conv)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,mode", [("t2k3", "reference"), ("t2k3", "device"),
+                                       ("t2k3s", "reference"), ("t5k6s", "device")])
+def test_fortran_host_trajectory_matches_reference(case, mode, tmp_path):
+    from forge_amd.checkpoint import write_checkpoint
+    from forge_amd.geometry import build_geometry
+    from forge_amd.seed import seeded_rho
+    print("fortran host", case, mode, flush=True)
+    f = load_case(case)
+    c = case_params(f)
+    args = [FORTRAN_EXE, f"mode={mode}", *_args(c), f"itmax={c['itmax']}",
+            f"out={tmp_path / 'traj.txt'}"]
+    if c["seeded"]:
+        g = build_geometry(c["ntheta"], c["nlayers"], c["numk"])
+        ck = tmp_path / "start.ckpt"
+        write_checkpoint(str(ck), seeded_rho(g.ndim, g.nnc, c["nfilling"]))
+        args.append(f"start={ck}")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
+    np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)

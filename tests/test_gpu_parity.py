@@ -144,6 +144,16 @@ def test_energies_match_reference(case):
     out = e.energies(rho0, dens)
     np.testing.assert_allclose(out, f["energies0"][:4], rtol=1e-12, atol=1e-10)
     assert e.total_energy(out) == pytest.approx(f["energies0"][4], abs=1e-9)
+    # GetHartreeHubbardEnergy with the spin-singlet call of Main.f90:307 (dens1 = dens2)
+    hh = e.hartree_hubbard(dens, dens)
+    np.testing.assert_allclose(hh, f["energies0"][2:4], rtol=1e-12, atol=1e-10)
+    # unequal densities: 1/2 (d1+d2).L.(d1+d2) and d1.d2 against the reference's LongRange
+    rng = np.random.default_rng(7)
+    d2 = dens + 1e-3 * rng.standard_normal(dens.shape)
+    hh = e.hartree_hubbard(dens, d2)
+    Lr = f["LongRange"]
+    s = dens + d2
+    np.testing.assert_allclose(hh, [0.5 * s @ Lr @ s, dens @ d2], rtol=1e-12, atol=1e-10)
 
 
 @pytest.mark.parametrize("case", ["t1k3", "t1k3s", "t2k3", "t2k3s", "t2k3xi10", "t3k6", "tri1k3",
