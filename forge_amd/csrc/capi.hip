@@ -162,9 +162,13 @@ struct forge_ctx {
   int* gidx = nullptr;
   double *part = nullptr, *e5 = nullptr, *dots = nullptr;
   double *da = nullptr, *db = nullptr, *ya = nullptr, *yb = nullptr;
+  // FP32 copy of the eigensolver batch for the mixed-precision hemv
+  // (allocated on the first forge_scf_set_options with precision 1)
+  float2* S = nullptr;
   // SCF state
   forge_scf_options opt{};
   bool scf_ready = false;
+  bool mixed_done = false;  // precision 1: switched to FP64 for the rest of the solve
   double e[4] = {0, 0, 0, 0}, Etot = 0;
   std::vector<double> h_da;
   int it = 0;
@@ -346,7 +350,8 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
   hipStreamSynchronize(c->s);
   void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
-                  c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb};
+                  c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
+                  c->S};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (i64 q = 0; q < c->trd.nev; ++q) hipEventDestroy(c->trd.evs[q]);
@@ -420,6 +425,9 @@ extern "C" int forge_hamiltonian_hf(forge_ctx* c, forge_z* zH, const double* Pot
 static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   const i64 n = c->n;
   c->trd.K = K;
+  const bool mixed = c->opt.precision == 1 && !c->mixed_done && c->S;
+  c->trd.S = mixed ? c->S : nullptr;
+  c->trd.sS = (size_t)c->ldH * n;
   double hemv_ms = 0, hemv_bytes = 0;
   long hemv_n = 0;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[2], c->s));
@@ -772,9 +780,19 @@ extern "C" int forge_scf_set_options(forge_ctx* c, const forge_scf_options* opt)
     set_error("forge_scf_set_options: invalid option values");
     return -2;
   }
-  if (opt->precision != 0) {
-    set_error("forge_scf_set_options: only precision 0 (FP64) is available");
+  if (opt->precision < 0 || opt->precision > 1 || !(opt->switch_de >= 0.0)) {
+    set_error("forge_scf_set_options: precision must be 0 or 1 and switch_de >= 0");
     return -2;
+  }
+  if (opt->precision == 1 && !c->S) {
+    const size_t bytes = sizeof(float2) * (size_t)c->ldH * c->n * c->Kb;
+    size_t fr = 0, tot = 0;
+    FORGE_HIP(hipMemGetInfo(&fr, &tot));
+    if (bytes + ((size_t)256 << 20) > fr) {
+      set_error("forge_scf_set_options: not enough device memory for the FP32 copy of precision 1");
+      return -4;
+    }
+    CK(dalloc((void**)&c->S, bytes));
   }
   c->opt = *opt;
   return 0;
@@ -800,6 +818,7 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
   const i64 n = c->n, nnc = c->nnc;
   const size_t nn = (size_t)n * n;
   std::fill(c->tm, c->tm + 16, 0.0);
+  c->mixed_done = false;
   int fails = 0;
   if (rho_start) {
     FORGE_HIP(hipMemcpyAsync(c->rho, rho_start, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
@@ -840,6 +859,8 @@ extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64
   FORGE_HIP(hipEventRecord(c->ev[8], c->s));
   for (i64 itn = 0; itn < max_iter; ++itn) {
     std::vector<double> band;
+    const bool mixed = c->opt.precision == 1 && !c->mixed_done;
+    if (mixed) c->tm[12] += 1;
     int f = solve_device(c, c->rho, c->pot, band, timed);
     if (f < 0) return f;
     fails += f;
@@ -903,7 +924,16 @@ extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64
       r[5] = ein[3]; r[6] = dconv; r[7] = step; r[8] = EtotIn - c->Etot; r[9] = occ.EF;
       r[10] = (double)occ.nOcc; r[11] = (double)occ.nPart;
     }
-    if (c->opt.etol > 0 && std::fabs(EtotIn - c->Etot) < c->opt.etol) break;
+    const double dE = std::fabs(EtotIn - c->Etot);
+    if (mixed) {
+      // precision 1: FP64 from here on once the energy change reaches the
+      // switch threshold (default 1e-4 eV); convergence is only declared on
+      // FP64 iterations
+      const double sw = c->opt.switch_de > 0 ? c->opt.switch_de : 1e-4;
+      if (dE < sw || (c->opt.etol > 0 && dE < c->opt.etol)) c->mixed_done = true;
+      continue;
+    }
+    if (c->opt.etol > 0 && dE < c->opt.etol) break;
   }
   FORGE_HIP(hipEventRecord(c->ev[9], c->s));
   FORGE_HIP(hipEventSynchronize(c->ev[9]));

@@ -44,16 +44,19 @@ struct TridiagWork {
   z2* W;          // K x n x NB
   z2* P;          // hemv row partials: K x nt x n   (A v over column strip J, row r)
   z2* P2;         // hemv column partials: K x nt x nseg x TS (A^H v of strip J, segment s)
-  z2* tpart;      // K x nseg x 2NB: W^H v, V^H v partials per row segment
+  z2* tpart;      // K x nseg x PB x 2NB: W^H v, V^H v partials per row chunk
   double* npart;  // K x nblk: column-norm partials of k_trd_col
   z2* dpart;      // K x nblk: w^H v partials of k_trd_fin
   z2* scal;       // K x 4: tau, 1/(alpha - beta), beta of the current column
+  float2* S;      // null, or an FP32 copy of A (same lda, stride sS) that the hemv
+  size_t sS;      // streams instead of A; k_trd_her2k keeps it current
   hipEvent_t* evs;  // event pairs of the sampled hemv launches (created lazily)
   i64 nev;
 };
 constexpr int TRD_NB = 32;  // panel width
 constexpr int TRD_TS = 32;     // hemv column-strip width
 constexpr int TRD_SEGR = 2048; // hemv rows per block
+constexpr int TRD_PB = 4;      // hemv panel-product blocks per row segment
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,

@@ -24,7 +24,17 @@
 //   k_trd_fin    w = tau (A22 v - V W^H v - W V^H v), v written back, w^H v
 //                partials
 // The correction of the panel's last w is applied on the fly by k_trd_her2k.
+//
+// Mixed precision (TridiagWork::S set): the hemv streams an FP32 copy of the
+// trailing matrix, written by k_trd_to_fp32 before the first panel and by
+// k_trd_her2k after every trailing update, and accumulates in FP64 against the
+// FP64 Householder vector.  The hemv is bound by HBM bandwidth, so halving its
+// bytes halves its time; the reduction's backward error becomes that of the
+// FP32 rounding of A22 (|dA| <= 2^-24 |A|), while every other quantity (the
+// panel, W, tau, d, e) stays FP64.
 #include <hip/hip_ext.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -59,7 +69,7 @@ static TrdLayout layout(i64 n, int K) {
   L.A_W = take(sizeof(z2) * K * n * NB);
   L.A_P = take(sizeof(z2) * (size_t)K * nt * n);
   L.A_P2 = take(sizeof(z2) * (size_t)K * nt * ms * TS);
-  L.A_tp = take(sizeof(z2) * (size_t)K * ms * 2 * NB);
+  L.A_tp = take(sizeof(z2) * (size_t)K * ms * TRD_PB * 2 * NB);
   L.A_np = take(sizeof(double) * (size_t)K * nb);
   L.A_dp = take(sizeof(z2) * (size_t)K * nb);
   L.A_sc = take(sizeof(z2) * (size_t)K * 4);
@@ -85,6 +95,8 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   w.npart = (double*)(b + L.A_np);
   w.dpart = (z2*)(b + L.A_dp);
   w.scal = (z2*)(b + L.A_sc);
+  w.S = nullptr;
+  w.sS = 0;
   w.evs = nullptr;
   w.nev = 0;
   return 0;
@@ -126,15 +138,23 @@ __device__ inline Reflector reflector(z2 alpha, double xn2) {
   return R;
 }
 
+// Sum of nb partials by the first wave: lane l adds partials l, l + 64, ...,
+// then a fixed butterfly; one memory latency instead of nb dependent loads.
+__device__ inline double wave_sum_partials(const double* part, unsigned nb) {
+  double s = 0.0;
+  for (unsigned b = threadIdx.x; b < nb; b += 64) s += part[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return s;
+}
+
 // Reflector of column j from the norm partials of k_trd_col (fixed order).
 // All threads of the block return the same value.
 __device__ inline Reflector reflector_of(const z2* A, i64 lda, i64 n, i64 j, const double* npart) {
   __shared__ Reflector Rs;
-  if (threadIdx.x == 0) {
-    const unsigned nb = (unsigned)nblk_of(n - j);
-    double xn2 = 0.0;
-    for (unsigned b = 0; b < nb; ++b) xn2 += npart[b];
-    Rs = reflector(A[(j + 1) + lda * j], xn2);
+  if (threadIdx.x < 64) {
+    const double xn2 = wave_sum_partials(npart, (unsigned)nblk_of(n - j));
+    if (threadIdx.x == 0) Rs = reflector(A[(j + 1) + lda * j], xn2);
   }
   __syncthreads();
   Reflector R = Rs;
@@ -146,11 +166,19 @@ __device__ inline Reflector reflector_of(const z2* A, i64 lda, i64 n, i64 j, con
 // k_trd_fin (its grid had nblk_of(n - j - 1) blocks).
 __device__ inline z2 alpha_of(i64 n, i64 j, z2 tau, const z2* dpart) {
   __shared__ z2 as;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
     const unsigned nb = (unsigned)nblk_of(n - j - 1);
-    z2 s = mk(0.0, 0.0);
-    for (unsigned b = 0; b < nb; ++b) s = s + dpart[b];
-    as = (-0.5) * (tau * s);
+    double sx = 0.0, sy = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += 64) {
+      sx += dpart[b].x;
+      sy += dpart[b].y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sx += __shfl_xor(sx, off, 64);
+      sy += __shfl_xor(sy, off, 64);
+    }
+    if (threadIdx.x == 0) as = (-0.5) * (tau * mk(sx, sy));
   }
   __syncthreads();
   z2 a = as;
@@ -206,7 +234,7 @@ __global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z
 }
 
 // ---------------------------------------------------------------- K2: hemv
-// grid (nt + 1, nseg(m, 0), K).  Strip blocks (J < nt) stream column strip J
+// grid (PB + nt, nseg(m, 0), K).  Strip blocks (x = PB + J) stream column strip J
 // (32 columns, c0 = base + 32 J) over rows [32 J + s SEGR, min(m, 32 J + (s+1) SEGR))
 // of the trailing matrix.  Lane l of a wave covers row (l mod 16) and the 8
 // columns 8 (l / 16) .. + 7 of a 16-row step, so one load instruction reads
@@ -214,23 +242,34 @@ __global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z
 // the 4 waves take interleaved 16-row steps:
 //   P[J][r]      = sum_c A(r, c) v_c                (c <= r inside the diagonal tile)
 //   P2[J][s][c]  = sum_r conj(A(r, c)) v_r          (r >  c inside the diagonal tile)
-// Blocks J == nt form the W^H v and V^H v partials of row segment s.
+// Blocks x < PB form the W^H v and V^H v partials of chunk x of row segment s.
+__device__ inline z2 widen(const z2 a) { return a; }
+__device__ inline z2 widen(const float2 a) { return mk((double)a.x, (double)a.y); }
+
+// TM: element type of the streamed matrix M (z2: M = A; float2: the FP32 copy)
+template <typename TM>
 __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, const z2* A_,
-                                                  size_t sA, i64 lda, const z2* W_,
-                                                  const double* npart_, z2* P_, z2* P2_, z2* tpart_,
-                                                  z2* scal_, i64 ntmax, i64 msmax, i64 nbmax) {
+                                                  size_t sA, i64 lda, const TM* M_, size_t sM,
+                                                  const z2* W_, const double* npart_, z2* P_,
+                                                  z2* P2_, z2* tpart_, z2* scal_, i64 ntmax,
+                                                  i64 msmax, i64 nbmax) {
   const int k = blockIdx.z;
-  const i64 J = blockIdx.x, s = blockIdx.y;
+  const i64 s = blockIdx.y;
   const i64 base = j + 1, m = n - base, nt = nt_of(m);
   const z2* A = A_ + k * sA;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (J == nt) {
-    // ---- W^H v and V^H v over rows [s SEGR, (s+1) SEGR) of the trailing part
-    const i64 rs = s * SEGR;
-    if (rs >= m) return;
-    const i64 re = rs + SEGR < m ? rs + SEGR : m;
-    const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
-    if (s == 0 && threadIdx.x == 0) {  // published for k_trd_fin, which rewrites A(j+1, j)
+  if (blockIdx.x < TRD_PB) {
+    // ---- W^H v and V^H v over rows [rs, re) of the trailing part: chunk pb
+    // of segment s.  These blocks come first in the grid and are small, so
+    // they finish under the strip blocks instead of forming the tail.
+    const int pb = blockIdx.x;
+    const i64 rs = s * SEGR + pb * (SEGR / TRD_PB);
+    const i64 re = rs + SEGR / TRD_PB < m ? rs + SEGR / TRD_PB : m;
+    const bool publish = s == 0 && pb == 0;
+    Reflector R{};
+    if (publish || (jj > 0 && rs < m))  // uniform over the block
+      R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
+    if (publish && threadIdx.x == 0) {  // published for k_trd_fin, which rewrites A(j+1, j)
       scal_[k * 4 + 0] = R.tau;
       scal_[k * 4 + 1] = R.scale;
       scal_[k * 4 + 2] = mk(R.beta, 0.0);
@@ -253,7 +292,9 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
         }
       }
     }
-    z2* tp = tpart_ + ((size_t)k * msmax + s) * 2 * NB;
+    // written also for empty chunks: k_trd_fin sums every chunk of the
+    // segments that hold rows
+    z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
@@ -271,8 +312,9 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
     }
     return;
   }
+  const i64 J = (i64)blockIdx.x - TRD_PB;  // column strip
   const i64 rs = TS * J + s * SEGR;  // first row (relative to base) of this block
-  if (J > nt || rs >= m) return;
+  if (J >= nt || rs >= m) return;
   const i64 re = rs + SEGR < m ? rs + SEGR : m;
   const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
   const z2 scale = R.scale;
@@ -294,10 +336,20 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = mk(0.0, 0.0);
   z2* P = P_ + ((size_t)k * ntmax + J) * n;
-  const z2* colp = A + (size_t)lda * (c0 + cb);
+  const TM* colp = M_ + k * sM + (size_t)lda * (c0 + cb);
   // row steps start at an absolute multiple of 16 rows, so with lda a multiple
   // of 16 every 16-row column chunk is one aligned 256-byte segment
   const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+  // The FP32 copy is read one step ahead (8-byte loads alone leave too few
+  // bytes in flight per wave); the FP64 loads gain nothing from it.
+  constexpr bool PF = sizeof(TM) == 8;
+  TM nxt[8];
+  if (PF) {
+    const i64 rr = ra0 + 16 * wv + rsub;
+    const bool ok = rr >= rs && rr < re;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? colp[base + rr + (size_t)lda * q] : TM{};
+  }
   for (i64 rr0 = ra0 + 16 * wv; rr0 < re; rr0 += 64) {
     const i64 rr = rr0 + rsub;  // row relative to base
     const bool valid = rr >= rs && rr < re;
@@ -305,9 +357,18 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
     const z2 vr = !valid ? mk(0.0, 0.0) : (rr == 0 ? mk(1.0, 0.0) : A[r + lda * j] * scale);
     const i64 dr = rr - TS * J;  // < 32 inside the diagonal tile
     z2 a[8];
+    if (PF) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-      a[q] = (valid && cb + q < ncol) ? colp[r + (size_t)lda * q] : mk(0.0, 0.0);
+      for (int q = 0; q < 8; ++q) a[q] = widen(nxt[q]);
+      const i64 rn = rr + 64;
+      const bool ok = rn >= rs && rn < re;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? colp[base + rn + (size_t)lda * q] : TM{};
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        a[q] = (valid && cb + q < ncol) ? widen(colp[r + (size_t)lda * q]) : mk(0.0, 0.0);
+    }
     z2 y = mk(0.0, 0.0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -357,9 +418,9 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
   __shared__ z2 t1[NB], t2[NB];
   __shared__ double sh[256];
   if (threadIdx.x < 2 * NB) {
-    const z2* tp = tpart_ + (size_t)k * msmax * 2 * NB;
+    const z2* tp = tpart_ + (size_t)k * msmax * TRD_PB * 2 * NB;
     z2 a = mk(0.0, 0.0);
-    for (i64 s = 0; s < ns0; ++s) a = a + tp[s * 2 * NB + threadIdx.x];
+    for (i64 s = 0; s < ns0 * TRD_PB; ++s) a = a + tp[s * 2 * NB + threadIdx.x];
     if (threadIdx.x < NB) t1[threadIdx.x] = a;
     else t2[threadIdx.x - NB] = a;
   }
@@ -401,10 +462,12 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
 // (added here while staging).  64x64 output tiles (lower tiles only), 256
 // threads with 4x4 complex outputs each, panel columns staged 8 at a time.
 __global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp, z2* A_,
-                                                   size_t sA, i64 lda, const z2* W_,
-                                                   const z2* tau_, const z2* dpart_, i64 nbmax) {
+                                                   size_t sA, i64 lda, float2* S_, size_t sS,
+                                                   const z2* W_, const z2* tau_, const z2* dpart_,
+                                                   i64 nbmax) {
   const int k = blockIdx.y;
   z2* A = A_ + k * sA;
+  float2* S = S_ ? S_ + k * sS : nullptr;
   const z2* W = W_ + (size_t)k * n * NB;
   const i64 jl = i0 + nbp - 1;  // last reflector of the panel
   const z2 alpha = alpha_of(n, jl, tau_[(size_t)k * n + jl], dpart_ + (size_t)k * nbmax);
@@ -463,8 +526,25 @@ __global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const i64 r = r0 + tx + 16 * u, c = c0 + ty + 16 * v;
-      if (r < n && c < n && r >= c) A[r + lda * c] = A[r + lda * c] - acc[u][v];
+      if (r < n && c < n && r >= c) {
+        const z2 a = A[r + lda * c] - acc[u][v];
+        A[r + lda * c] = a;
+        if (S) S[r + lda * c] = make_float2((float)a.x, (float)a.y);
+      }
     }
+}
+
+// S = fp32(A) on the lower triangle (column c, rows c..n-1); grid
+// (cdiv(n, 256), ny, K), columns blockIdx.y, blockIdx.y + ny, ...
+__global__ void __launch_bounds__(256) k_trd_to_fp32(i64 n, const z2* A_, size_t sA, i64 lda,
+                                                     float2* S_, size_t sS) {
+  for (i64 c = blockIdx.y; c < n; c += gridDim.y) {
+    const i64 r = c + (i64)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) {
+      const z2 a = A_[blockIdx.z * sA + r + lda * c];
+      S_[blockIdx.z * sS + r + lda * c] = make_float2((float)a.x, (float)a.y);
+    }
+  }
 }
 
 __global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* d_) {
@@ -476,8 +556,9 @@ __global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* 
 // Timing (hemv_ms != nullptr): the hemv launches of every TIME_STRIDE-th
 // column carry a pair of HIP events (hipExtLaunchKernel), so the events take
 // the dispatch's own begin and end timestamps; their summed time, count and
-// algorithmic bytes (8 m^2 per matrix: the lower triangle of the trailing
-// matrix read once) are accumulated into the outputs.  The pairs form a ring
+// algorithmic bytes (the lower triangle of the m x m trailing matrix read
+// once: 8 m^2 per matrix in FP64, 4 m^2 from the FP32 copy) are accumulated
+// into the outputs.  The pairs form a ring
 // of TIME_RING entries; a pair is read back before it is reused, which waits
 // only for a launch TIME_RING * TIME_STRIDE columns back.  The stride is
 // coprime with the panel width, so the sample visits every panel position
@@ -493,6 +574,21 @@ static int read_pair(TridiagWork& w, i64 q, double* ms) {
   return 0;
 }
 
+// hemv launch; with events e0/e1 they take the dispatch's begin and end
+template <typename TM>
+static void launch_hemv(dim3 g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, i64 n, i64 j, i64 i0,
+                        int jj, const TridiagWork& w, const TM* M, size_t sM, i64 ntmax, i64 msmax,
+                        i64 nbmax) {
+  if (e0)
+    hipExtLaunchKernelGGL(k_trd_hemv<TM>, g, dim3(256), 0, s, e0, e1, 0, n, j, i0, jj,
+                          (const z2*)w.A, w.sA, w.lda, M, sM, (const z2*)w.W,
+                          (const double*)w.npart, w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
+  else
+    hipLaunchKernelGGL(k_trd_hemv<TM>, g, dim3(256), 0, s, n, j, i0, jj, (const z2*)w.A, w.sA,
+                       w.lda, M, sM, (const z2*)w.W, (const double*)w.npart, w.P, w.P2, w.tpart,
+                       w.scal, ntmax, msmax, nbmax);
+}
+
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes) {
   const i64 n = w.n;
@@ -506,6 +602,11 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   }
   i64 nrec = 0;
   double bytes = 0.0, tms = 0.0;
+  if (w.S) {
+    hipLaunchKernelGGL(k_trd_to_fp32, dim3((unsigned)cdiv(n, 256), (unsigned)std::min<i64>(n, 32768), K),
+                       dim3(256), 0, s, n, w.A, w.sA, w.lda, w.S, w.sS);
+    FORGE_LAUNCH();
+  }
   for (i64 i0 = 0; i0 < n - 1; i0 += NB) {
     const int nbp = (int)((n - 1 - i0) < NB ? (n - 1 - i0) : NB);
     for (int jj = 0; jj < nbp; ++jj) {
@@ -514,19 +615,24 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
                          w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
       const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
-      const dim3 hgrid((unsigned)nt + 1, (unsigned)ms, K);
-      if (timed && (j % TIME_STRIDE == 0)) {
+      const dim3 hgrid((unsigned)(nt + TRD_PB), (unsigned)ms, K);
+      const bool tj = timed && (j % TIME_STRIDE == 0);
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (tj) {
         const i64 q = nrec % TIME_RING;
         if (nrec >= TIME_RING && read_pair(w, q, &tms)) return -1;
-        hipExtLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, w.evs[2 * q], w.evs[2 * q + 1],
-                              0, n, j, i0, jj, w.A, w.sA, w.lda, w.W, w.npart, w.P, w.P2, w.tpart,
-                              w.scal, ntmax, msmax, nbmax);
+        e0 = w.evs[2 * q];
+        e1 = w.evs[2 * q + 1];
         ++nrec;
-        bytes += 8.0 * (double)m * (double)m * K;
-      } else {
-        hipLaunchKernelGGL(k_trd_hemv, hgrid, dim3(256), 0, s, n, j, i0, jj, w.A, w.sA, w.lda, w.W,
-                           w.npart, w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
+        // algorithmic bytes: the lower triangle of the m x m trailing matrix
+        bytes += (w.S ? 4.0 : 8.0) * (double)m * (double)m * K;
       }
+      if (w.S)
+        launch_hemv<float2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const float2*)w.S, w.sS, ntmax,
+                            msmax, nbmax);
+      else
+        launch_hemv<z2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const z2*)w.A, w.sA, ntmax, msmax,
+                        nbmax);
       FORGE_LAUNCH();
       hipLaunchKernelGGL(k_trd_fin, dim3(cdiv(m, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
@@ -535,7 +641,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     }
     const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
     hipLaunchKernelGGL(k_trd_her2k, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s, n,
-                       p, i0, nbp, w.A, w.sA, w.lda, w.W, w.tau, w.dpart, nbmax);
+                       p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
     FORGE_LAUNCH();
   }
   hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);

@@ -28,6 +28,7 @@ module forge_c
     integer(c_int32_t) :: occ_mode = 0, mix_mode = 0
     real(c_double) :: kT = 0.0_c_double, beta = 0.3_c_double, etol = 0.0_c_double
     integer(c_int32_t) :: precision = 0, timing = 0
+    real(c_double) :: switch_de = 0.0_c_double
   end type forge_scf_options_t
 
   interface

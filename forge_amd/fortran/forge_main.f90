@@ -16,7 +16,7 @@
 !
 ! Arguments (key=value): ntheta nlayers numk nfilling itmax xi_nm epsilon U
 ! Delta mode start (first | path of a checkpoint in the reference's Fock
-! format) occ mix kT beta etol out (trajectory file).
+! format) occ mix kT beta etol precision switch_de out (trajectory file).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -27,8 +27,8 @@ program forge_main
 
   character(len=16) :: mode = 'reference'
   character(len=512) :: start = 'first', outfile = 'forge_traj.txt'
-  integer(c_int32_t) :: occ = 0, mix = 0
-  real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp
+  integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
+  real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp, switch_de = 0.0_dp
   integer(dp) :: nnc, it, n, nFermiLevel, nOcc, nPart, icount, i, nU
   integer(dp), allocatable :: nUnitCell_1(:), nUnitCell_2(:), nMomentaComponents(:, :)
   integer(dp), allocatable :: nSortedMomenta(:, :, :)
@@ -91,6 +91,7 @@ contains
 
     if (trim(mode) == 'device') then
       opt%occ_mode = occ; opt%mix_mode = mix; opt%kT = kT; opt%beta = beta; opt%etol = etol
+      opt%precision = prec; opt%switch_de = switch_de
       call forge_check(forge_scf_set_options(ctx, opt), 'forge_scf_set_options')
       if (trim(start) == 'first') then
         call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
@@ -336,6 +337,8 @@ contains
       case ('kT'); read(val, *) kT
       case ('beta'); read(val, *) beta
       case ('etol'); read(val, *) etol
+      case ('precision'); read(val, *) prec
+      case ('switch_de'); read(val, *) switch_de
       case ('out'); outfile = val
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)

@@ -44,9 +44,11 @@ typedef struct {
   double kT;          /* eV, occ_mode 2 */
   double beta;        /* linear mixing weight, mix_mode 1 */
   double etol;        /* > 0: forge_scf_run stops once |E_in - E_new| < etol (eV per cell) */
-  int32_t precision;  /* 0: FP64 everywhere; 1: FP64 reduction + tridiagonal solve, FP32-split
-                         back-transform and density products */
+  int32_t precision;  /* 0: FP64 everywhere; 1: mixed -- the Householder hemv streams an FP32
+                         copy of the trailing matrix (FP64 accumulation, every other step
+                         FP64) until |E_in - E_new| < switch_de, then FP64 to convergence */
   int32_t timing;     /* nonzero: per-stage HIP-event timings (forge_scf_timings) */
+  double switch_de;   /* eV, precision 1 (0: 1e-4) */
 } forge_scf_options;
 
 typedef struct forge_ctx forge_ctx;
@@ -136,7 +138,8 @@ int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
  * [6] total, [7] launches of the assembly kernel, [8] summed time of the
  * assembly kernel (ms), [9] timed launches of the Householder hemv kernel
  * (every 17th column), [10] their summed time (ms), [11] their algorithmic
- * bytes (8 m^2 per matrix, m the trailing order). */
+ * bytes (8 m^2 per matrix in FP64, 4 m^2 from the FP32 copy; m the trailing
+ * order), [12] iterations run with precision 1 still in its mixed phase. */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
 
 #ifdef __cplusplus
