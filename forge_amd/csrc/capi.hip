@@ -813,6 +813,8 @@ static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation
   return 0;
 }
 
+static int init_state_of_rho(forge_ctx* c);
+
 extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
   if (!c) { set_error("forge_scf_init: null context"); return -2; }
   const i64 n = c->n, nnc = c->nnc;
@@ -832,6 +834,13 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
     CK(window_step(c, band, occ));
     FORGE_HIP(hipMemcpyAsync(c->rho, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToDevice, c->s));
   }
+  CK(init_state_of_rho(c));
+  return fails;
+}
+
+// energies of c->rho and the potential it defines (Main.f90:290-346)
+static int init_state_of_rho(forge_ctx* c) {
+  const i64 n = c->n;
   CK(site_density(n, c->rho, c->da, c->s));
   double pc[13];
   CK(energy_pieces(c, c->rho, nullptr, pc));
@@ -843,7 +852,88 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
   FORGE_HIP(hipStreamSynchronize(c->s));
   c->it = 0;
   c->scf_ready = true;
+  return 0;
+}
+
+// rho(i, i, 0) = diag_i; rho(row_q, col_q, cell_q) += delta_q (distinct entries)
+__global__ void k_apply_guess(i64 n, const double* diag, i64 nupd, const i64* row, const i64* col,
+                              const i64* cell, const z2* delta, z2* rho) {
+  const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) rho[t + n * t] = mk(diag[t], 0.0);
+  if (t < nupd) {
+    z2* e = rho + (size_t)cell[t] * n * n + row[t] + (size_t)n * col[t];
+    *e = *e + delta[t];
+  }
+}
+
+// one diagonalisation of H[c->rho] at every k; the state becomes its
+// occupied density
+static int guess_diagonalisation(forge_ctx* c) {
+  const size_t tot = (size_t)c->nnc * c->n * c->n;
+  CK(init_state_of_rho(c));
+  std::vector<double> band;
+  const int fails = solve_device(c, c->rho, c->pot, band, false);
+  if (fails < 0) return fails;
+  Occupation occ;
+  CK(window_step(c, band, occ));
+  FORGE_HIP(hipMemcpyAsync(c->rho, c->rho_new, sizeof(z2) * tot, hipMemcpyDeviceToDevice, c->s));
+  CK(init_state_of_rho(c));
   return fails;
+}
+
+extern "C" int forge_scf_init_guess_update(forge_ctx* c, const double* diag, int64_t nupd,
+                                           const int64_t* row, const int64_t* col,
+                                           const int64_t* cell, const forge_z* delta) {
+  if (!c || !diag || nupd < 0 || (nupd > 0 && (!row || !col || !cell || !delta))) {
+    set_error("forge_scf_init_guess_update: invalid argument");
+    return -2;
+  }
+  if (!c->scf_ready) { set_error("forge_scf_init_guess_update: call forge_scf_init first"); return -5; }
+  const i64 n = c->n;
+  for (i64 q = 0; q < nupd; ++q)
+    if (row[q] < 0 || row[q] >= n || col[q] < 0 || col[q] >= n || cell[q] < 0 || cell[q] >= c->nnc) {
+      set_error("forge_scf_init_guess_update: entry out of range");
+      return -2;
+    }
+  std::fill(c->tm, c->tm + 16, 0.0);
+  c->mixed_done = false;
+  double* dd = nullptr;
+  i64* ix = nullptr;
+  z2* dv = nullptr;
+  const i64 m = std::max<i64>(nupd, 1);
+  int rc = [&]() -> int {
+    CK(dalloc((void**)&dd, sizeof(double) * n));
+    CK(dalloc((void**)&ix, sizeof(i64) * 3 * m));
+    CK(dalloc((void**)&dv, sizeof(z2) * m));
+    FORGE_HIP(hipMemcpyAsync(dd, diag, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+    if (nupd > 0) {
+      FORGE_HIP(hipMemcpyAsync(ix, row, sizeof(i64) * nupd, hipMemcpyHostToDevice, c->s));
+      FORGE_HIP(hipMemcpyAsync(ix + m, col, sizeof(i64) * nupd, hipMemcpyHostToDevice, c->s));
+      FORGE_HIP(hipMemcpyAsync(ix + 2 * m, cell, sizeof(i64) * nupd, hipMemcpyHostToDevice, c->s));
+      FORGE_HIP(hipMemcpyAsync(dv, delta, sizeof(z2) * nupd, hipMemcpyHostToDevice, c->s));
+    }
+    hipLaunchKernelGGL(k_apply_guess, dim3((unsigned)cdiv(std::max<i64>(n, nupd), 256)), dim3(256), 0,
+                       c->s, n, dd, nupd, ix, ix + m, ix + 2 * m, dv, c->rho);
+    FORGE_LAUNCH();
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    return 0;
+  }();
+  if (dd) (void)hipFree(dd);
+  if (ix) (void)hipFree(ix);
+  if (dv) (void)hipFree(dv);
+  if (rc != 0) return rc;
+  return guess_diagonalisation(c);
+}
+
+extern "C" int forge_scf_init_guess(forge_ctx* c, const forge_z* rho_guess) {
+  if (!c || !rho_guess) { set_error("forge_scf_init_guess: null argument"); return -2; }
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  std::fill(c->tm, c->tm + 16, 0.0);
+  c->mixed_done = false;
+  // H[rho_guess]: its potential, then one diagonalisation at every k
+  FORGE_HIP(hipMemcpyAsync(c->rho, rho_guess, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  return guess_diagonalisation(c);
 }
 
 extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64_t* iters_done) {

@@ -1,0 +1,137 @@
+"""Training of the learned-guess MLP on SCF solutions of this engine (GPU).
+
+    python -m forge_amd.guess_train [--cells 6-15] [--fillings=-2,-1,0,1,2]
+        [--data PATH] [--out forge_amd/data/guess_mlp.npz] [--steps 4000]
+
+1. Data: for every cell index m and filling nu, the first diagonalisation
+   (rho_TB) and a plain FP64 SCF from the seeded start (linear mixing 0.3,
+   Fermi smearing 1 meV, |dE| < 1e-6 eV), then the descriptors and targets of
+   every site (forge_amd.learned).  Cell index max(cells) is held out for
+   validation.
+2. Training: Adam on the library's MLP kernels (forward, backward and update
+   on the GPU), standardised inputs and targets.
+3. Report: validation loss and the SCF iterations from the learned start
+   against the seeded start on the held-out cell.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import learned as Lr
+from .hf import Engine, Setup
+from .mlp import Mlp, loss_grad
+from .seed import seeded_rho
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCF = dict(occ_mode=2, mix_mode=1, kT=0.001, beta=0.3, etol=1e-6)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def solve_system(m: int, nu: float, max_iter: int = 250):
+    e = Engine(Setup(ntheta=m, numk=3, nfilling=nu, xi_nm=40.0))
+    e.set_options(**SCF)
+    t0 = time.time()
+    e.scf_init(None)
+    rho_tb = e.density()
+    e.scf_init(seeded_rho(e.n, e.nnc, nu))
+    tr = e.scf_run(max_iter)
+    rho = e.density()
+    X = Lr.descriptors(e.geom, nu)
+    Y, M = Lr.targets(e.geom, rho, rho_tb)
+    info = (m, nu, e.n, len(tr), tr[-1, 1] - tr[-1, 8], tr[-1, 8], time.time() - t0)
+    e.close()
+    return X, Y, M, info
+
+
+def generate(cells, fillings, path):
+    Xs, Ys, Ms, sys_id, infos = [], [], [], [], []
+    for m in cells:
+        for nu in fillings:
+            X, Y, M, info = solve_system(m, nu)
+            say("  m=%2d nu=%+.0f n=%5d: %3d iterations, E=%.6f eV, last dE %+.1e (%.1fs)" % info)
+            Xs.append(X); Ys.append(Y); Ms.append(M)
+            sys_id.append(np.full(len(X), len(infos)))
+            infos.append(info)
+    np.savez(path, X=np.concatenate(Xs), Y=np.concatenate(Ys), M=np.concatenate(Ms),
+             sys=np.concatenate(sys_id), info=np.array(infos))
+    return path
+
+
+def iterations_from(e, start_fn, max_iter=250):
+    t0 = time.time()
+    start_fn()
+    tr = e.scf_run(max_iter)
+    return len(tr), tr[-1, 1] - tr[-1, 8], time.time() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cells", default="6-15")
+    ap.add_argument("--fillings", default="-2,-1,0,1,2")
+    ap.add_argument("--data", default="gpurun_out/guess_data.npz")
+    ap.add_argument("--out", default=os.path.join(HERE, "data", "guess_mlp.npz"))
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--reuse-data", action="store_true",
+                    help="train on existing data files (comma-separated --data)")
+    ap.add_argument("--data-only", action="store_true", help="generate the data and stop")
+    a = ap.parse_args()
+    lo, hi = (int(x) for x in a.cells.split("-"))
+    cells = list(range(lo, hi + 1))
+    fillings = [float(x) for x in a.fillings.split(",")]
+    paths = a.data.split(",")
+    if not a.reuse_data:
+        os.makedirs(os.path.dirname(os.path.abspath(paths[0])), exist_ok=True)
+        say(f"[data] {len(cells)} cells x {len(fillings)} fillings")
+        generate(cells, fillings, paths[0])
+        if a.data_only:
+            return 0
+    parts = [np.load(p) for p in paths]
+    X = np.concatenate([f["X"] for f in parts])
+    Y = np.concatenate([f["Y"] for f in parts])
+    M = np.concatenate([f["M"] for f in parts])
+    offs = np.cumsum([0] + [len(f["info"]) for f in parts[:-1]])
+    sid = np.concatenate([f["sys"] + o for f, o in zip(parts, offs)])
+    info = np.concatenate([f["info"] for f in parts])
+    held = info[:, 0] == cells[-1]
+    val = np.isin(sid, np.nonzero(held)[0])
+    say(f"[train] {int((~val).sum())} training sites, {int(val.sum())} validation sites "
+        f"(m = {cells[-1]} held out)")
+    t0 = time.time()
+    model, hist = Mlp.fit(X[~val], Y[~val], M[~val], steps=a.steps, batch=a.batch, lr=a.lr)
+    say(f"[train] {a.steps} Adam steps in {time.time() - t0:.1f}s; batch loss "
+        f"{hist[:100].mean():.4f} (first 100) -> {hist[-100:].mean():.4f} (last 100)")
+    for tag, sel in (("train", ~val), ("validation", val)):
+        Xn = (X[sel] - model.xm) / model.xs
+        Yn = (Y[sel] - model.ym) / model.ys
+        lval, _ = loss_grad(model.params, Xn, Yn, M[sel])
+        yp = model.predict(X[sel])
+        r2 = 1 - ((yp - Y[sel]) ** 2).sum(0) / np.maximum(((Y[sel] - Y[sel].mean(0)) ** 2).sum(0), 1e-30)
+        say(f"[train] {tag}: standardised loss {lval:.4f}; R^2 per output {np.round(r2, 3)}")
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    model.save(a.out)
+    say(f"[train] model written to {a.out}")
+    # the held-out cell at nu = +1: seeded start against the learned start
+    m = cells[-1]
+    e = Engine(Setup(ntheta=m, numk=3, nfilling=1.0, xi_nm=40.0))
+    e.set_options(**SCF)
+    its_s = iterations_from(e, lambda: e.scf_init(seeded_rho(e.n, e.nnc, 1.0)))
+    its_tb = iterations_from(e, lambda: e.scf_init(None))
+    its_l = iterations_from(e, lambda: Lr.learned_start(e, model, 1.0))
+    for tag, r in (("seeded", its_s), ("first diagonalisation", its_tb), ("learned", its_l)):
+        say(f"[check] m={m} nu=+1 start {tag:22s}: {r[0]:3d} iterations, E={r[1]:.6f} eV ({r[2]:.1f}s)")
+    e.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -168,6 +168,21 @@ class Engine:
         rho = None if rho_start is None else _f(rho_start, np.complex128)
         return L.check(self.lib.forge_scf_init(self.ctx, _p(rho)), "forge_scf_init")
 
+    def scf_init_guess(self, rho_guess):
+        """Start from the occupied density of H[rho_guess] (one diagonalisation)."""
+        rho = _f(rho_guess, np.complex128)
+        return L.check(self.lib.forge_scf_init_guess(self.ctx, _p(rho)), "forge_scf_init_guess")
+
+    def scf_init_guess_update(self, diag, row, col, cell, delta):
+        """As scf_init_guess, the guess given as changes to the current state
+        (diagonal values and distinct (row, col, cell) increments, 0-based)."""
+        d = _f(diag, np.float64)
+        r, c, k = (np.ascontiguousarray(a, np.int64) for a in (row, col, cell))
+        v = np.ascontiguousarray(delta, np.complex128)
+        return L.check(self.lib.forge_scf_init_guess_update(self.ctx, _p(d), int(r.size), _p(r),
+                                                            _p(c), _p(k), _p(v)),
+                       "forge_scf_init_guess_update")
+
     def scf_run(self, max_iter):
         traj = np.zeros((max(max_iter, 1), 12))
         done = C.c_int64(0)

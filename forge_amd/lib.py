@@ -50,9 +50,15 @@ _SIGS = {
     "forge_hartree_hubbard": (C.c_int, [vp, vp, vp, vp]),
     "forge_scf_set_options": (C.c_int, [vp, vp]),
     "forge_scf_init": (C.c_int, [vp, vp]),
+    "forge_scf_init_guess": (C.c_int, [vp, vp]),
+    "forge_scf_init_guess_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),
     "forge_scf_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),
     "forge_scf_get_density": (C.c_int, [vp, vp]),
     "forge_scf_timings": (C.c_int, [vp, vp]),
+    "forge_mlp_param_count": (i64, [i64, i64]),
+    "forge_mlp_loss_grad": (C.c_int, [vp, i64, i64, vp, vp, vp, i64, vp, vp]),
+    "forge_mlp_train": (C.c_int, [vp, i64, i64, vp, vp, vp, i64, i64, i64, dbl, C.c_uint64, vp]),
+    "forge_mlp_infer": (C.c_int, [vp, i64, i64, vp, i64, i32, vp]),
 }
 
 _lib = None

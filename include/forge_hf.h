@@ -123,6 +123,21 @@ int forge_scf_set_options(forge_ctx* ctx, const forge_scf_options* opt);
  * (Main.f90:290-346). */
 int forge_scf_init(forge_ctx* ctx, const forge_z* rho_start);
 
+/* Initial state from a proposed density (the learned guess; no reference
+ * counterpart, it enters at the initial-state block Main.f90:187-288): one
+ * diagonalisation of H[rho_guess] at every k, and the state is the occupied
+ * density it yields, built as the first diagonalisation builds it from the
+ * tight-binding Hamiltonian (Main.f90:238-282); then as forge_scf_init. */
+int forge_scf_init_guess(forge_ctx* ctx, const forge_z* rho_guess);
+
+/* The same starting from the current state's density on the device (normally
+ * the first diagonalisation of forge_scf_init(ctx, NULL)): its cell-0
+ * diagonal is set to diag (ndim values) and the nupd distinct entries
+ * (row, col, cell), 0-based, are incremented by delta; no ndim^2 host copy. */
+int forge_scf_init_guess_update(forge_ctx* ctx, const double* diag, int64_t nupd,
+                                const int64_t* row, const int64_t* col, const int64_t* cell,
+                                const forge_z* delta);
+
 /* Runs up to max_iter SCF iterations (Main.f90:365-679).  traj receives one
  * row of 12 doubles per iteration: it, E_in, 2Ekin_in, 2EF_in, EH_in, EU_in,
  * max|dDensity|, Step, E_in - E_new, E_Fermi, nOcc, nPart.  *iters_done is
@@ -141,6 +156,30 @@ int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
  * bytes (8 m^2 per matrix in FP64, 4 m^2 from the FP32 copy; m the trailing
  * order), [12] iterations run with precision 1 still in its mixed phase. */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
+
+/* ---- MLP of the learned initial guess (no reference counterpart) ----
+ * 3 layers of width 128 with ReLU: Y = relu(relu(X W1 + b1) W2 + b2) W3 + b3,
+ * X (N x din, row-major, din <= 32), Y (N x dout, dout <= 16), parameters
+ * packed [W1 (din x 128), b1, W2 (128 x 128), b2, W3 (128 x dout), b3], W
+ * row-major (in x out), FP32. */
+int64_t forge_mlp_param_count(int64_t din, int64_t dout);
+
+/* Masked mean-squared loss sum(mask (Y - T)^2) / sum(mask) of the N rows and
+ * its gradient with respect to the parameters (FP32 kernels). */
+int forge_mlp_loss_grad(const float* params, int64_t din, int64_t dout, const float* X,
+                        const float* T, const float* mask, int64_t N, float* loss, float* grad);
+
+/* Adam (lr, beta1 0.9, beta2 0.999, eps 1e-8) on mini-batches of `batch` rows
+ * drawn without replacement in epochs (order from splitmix64(seed)); params
+ * in/out; loss_hist (steps, may be NULL) gets each step's batch loss. */
+int forge_mlp_train(float* params, int64_t din, int64_t dout, const float* X, const float* T,
+                    const float* mask, int64_t N, int64_t steps, int64_t batch, double lr,
+                    uint64_t seed, float* loss_hist);
+
+/* Y = network(X): bf16 != 0 on the matrix cores with bf16 operands and FP32
+ * accumulation, else the FP32 training kernels. */
+int forge_mlp_infer(const float* params, int64_t din, int64_t dout, const float* X, int64_t N,
+                    int32_t bf16, float* Y);
 
 #ifdef __cplusplus
 }
