@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: wall-time to a converged SCF (|E_in - E_new| < 1e-6 eV per moire
 cell) for 1.05 deg twisted bilayer graphene (cell index 31, 11908 pz orbitals,
-3 x 3 k grid, filling +1, seeded start, linear mixing 0.3, Fermi smearing
-kT = 1 meV) on one MI355X, plain FP64 GPU path.
+3 x 3 k grid, filling +1, linear mixing 0.3, Fermi smearing kT = 1 meV) on one
+MI355X: the mixed-precision + learned-guess path against the plain FP64 path.
 
-Default mode: a step is ONE SCF iteration of that solve (assembly of all nine
-H(k), nine Hermitian eigensolves, density products, energies, mixing).  W
-warm-up iterations and K timed iterations run from the seeded start; the
-reported value is the projected time to convergence = measured seconds per
-iteration x the iteration count of a complete recorded solve of the same
-workload (ITERS_TO_CONVERGE, log in profiles/scf_convergence_plain_fp64_n11908_k3.txt).
---full: a step is the complete solve, timed for real (about 20 minutes).
+  plain FP64      seeded start (uniform + 1e-3 Gaussian), every step FP64
+  mixed+learned   learned start (first diagonalisation, MLP site charges and
+                  bond shifts on the matrix cores, one diagonalisation of the
+                  proposed Hamiltonian), then SCF iterations with the FP32-copy
+                  hemv while |dE| > 1e-4 eV and FP64 iterations to convergence
 
-Inputs are synthetic (geometry from the cell index, seeded Gaussian start
-density); nothing is read from disk.  The JSON line carries the roofline of the
-dominant kernel (k_trd_hemv, HBM-bound), measured with HIP events on the
-library's stream inside the timed region, and a CPU baseline: the oracle
-(restatement of the reference's OpenMP/MKL CPU path) timed on a bounded sample.
+Default mode (a few minutes): a step is one SCF iteration.  W warm-up and K
+timed plain-FP64 iterations from the seeded start, one timed mixed-phase
+iteration and one timed learned start; the value is the projected time to
+convergence of the mixed+learned path = t(learned start) + N_mixed t(mixed
+iteration) + N_fp64 t(FP64 iteration), with the iteration counts of complete
+recorded solves of the same workload (MIXED_LEARNED_ITERS, PLAIN_ITERS; logs
+in profiles/).  --full: the complete solve(s), timed.
+
+The JSON line carries the roofline of the dominant kernel (k_trd_hemv, HBM
+bound; HIP events on the library's stream inside the timed region) and a CPU
+baseline: the oracle (restatement of the reference's OpenMP/MKL path) timed on
+a bounded sample.  Inputs are synthetic; nothing is read from disk except the
+trained network.
 """
 from __future__ import annotations
 
@@ -30,9 +36,15 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-# complete plain-FP64 solve of the default workload: 69 iterations reached
-# |dE| = 2.4e-6 eV with a contraction of 0.87 per iteration -> 75 iterations
-ITERS_TO_CONVERGE = 75
+# plain-FP64 solve of the default workload: 69 iterations reached |dE| =
+# 2.4e-6 eV with a contraction of 0.87 per iteration -> at least 75 iterations
+# (a lower bound: the slow mode the mixed+learned solve shows at the end
+# contracts by only 0.965 per iteration)
+PLAIN_ITERS = 75
+# complete mixed+learned solve of the default workload (bench.py --full,
+# switch at 1e-4 eV): 21 mixed-phase and 52 FP64 iterations
+MIXED_LEARNED_ITERS = {"mixed": 21, "fp64": 52}
+MODEL = os.path.join(REPO, "forge_amd", "data", "guess_mlp.npz")
 
 WORKLOADS = {
     # name: (ntheta, nlayers, numk, nfilling)
@@ -45,17 +57,25 @@ WORKLOADS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tbg_1.05deg_n11908_k3", choices=sorted(WORKLOADS))
     ap.add_argument("--full", action="store_true", help="time complete solves to convergence")
+    ap.add_argument("--path", default="both", choices=["both", "mixed_learned", "plain"],
+                    help="--full: which complete solves to run")
     ap.add_argument("--etol", type=float, default=1e-6)
-    ap.add_argument("--max-iter", type=int, default=120)
+    ap.add_argument("--switch-de", type=float, default=1e-4)
+    ap.add_argument("--max-iter", type=int, default=150)
     ap.add_argument("--beta", type=float, default=0.3)
     ap.add_argument("--kT", type=float, default=0.001)
+    ap.add_argument("--model", default=MODEL)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-ntheta", type=int, default=11)
     return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(args, iters: int, n_full: int):
@@ -84,9 +104,23 @@ def cpu_baseline(args, iters: int, n_full: int):
         "kind": "port",
         "sample": (f"one SCF iteration of the oracle at cell index {nt} (n={s.n}, {numk}x{numk} k, "
                    f"{nthreads} OpenMP threads over k, MKL sequential per k) took {dt:.2f} s; "
-                   f"scaled by (n/{s.n})^3 = {scale:.1f} to n={n_full} and by {iters} iterations "
-                   f"(extrapolated)"),
+                   f"scaled by (n/{s.n})^3 = {scale:.1f} to n={n_full} and by {iters} iterations of "
+                   f"the plain path (extrapolated)"),
     }
+
+
+def roofline(stage, kernel, bytes_per_elem):
+    """Sampled hemv launches (every 17th column, HIP events on the dispatch):
+    their summed algorithmic bytes over their summed duration."""
+    ms, launches, byts = stage[10], int(stage[9]), stage[11]
+    if launches == 0:
+        return None
+    achieved = byts / (ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "timed_launches": launches, "avg_launch_us": round(ms / launches * 1e3, 2),
+            "algorithmic_bytes_per_launch": round(byts / launches),
+            "bytes_per_matrix_element": bytes_per_elem}
 
 
 def main():
@@ -98,70 +132,106 @@ def main():
     import numpy as np
 
     from forge_amd.hf import Engine, Setup
+    from forge_amd.learned import learned_start
     from forge_amd.lib import device_count
+    from forge_amd.mlp import Mlp
     from forge_amd.seed import seeded_rho
 
     if device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(3)
     ntheta, nlayers, numk, nfill = WORKLOADS[args.workload]
-    setup = Setup(ntheta=ntheta, nlayers=nlayers, numk=numk, nfilling=nfill, xi_nm=40.0)
-    eng = Engine(setup)
-    n, K = eng.n, eng.Nk
-    start = seeded_rho(n, eng.nnc, nfill)
-    eng.set_options(occ_mode=2, mix_mode=1, kT=args.kT, beta=args.beta, etol=0.0, timing=1)
+    eng = Engine(Setup(ntheta=ntheta, nlayers=nlayers, numk=numk, nfilling=nfill, xi_nm=40.0))
+    model = Mlp.load(args.model)
+    n = eng.n
+    seeded = seeded_rho(n, eng.nnc, nfill)
+    base = dict(occ_mode=2, mix_mode=1, kT=args.kT, beta=args.beta, timing=1)
 
     def iterate(tag, count, stop_on_conv):
+        """count single-iteration runs; stage timings summed; the mixed-phase
+        count is accumulated in [12]"""
         rows, tm = [], np.zeros(16)
-        for it in range(count):
+        for _ in range(count):
+            t0 = time.perf_counter()
             r = eng.scf_run(1)
             rows.append(r[0])
-            tm += eng.timings()
-            print(f"[bench] {tag} it {len(rows)}: E_tot {r[0, 1]:.9f} dE {r[0, 8]:+.3e} "
-                  f"max|dn| {r[0, 6]:.2e}", file=sys.stderr, flush=True)
-            if stop_on_conv and abs(r[0, 8]) < args.etol:
+            t = eng.timings()
+            tm += t
+            log(f"{tag} it {len(rows)}: E_tot {r[0, 1]:.9f} dE {r[0, 8]:+.3e} max|dn| {r[0, 6]:.2e} "
+                f"({'mixed' if t[12] > 0 else 'fp64'}, {time.perf_counter() - t0:.2f}s)")
+            if stop_on_conv and abs(r[0, 8]) < args.etol and t[12] == 0:
                 break
         return np.array(rows), tm
 
-    stage = np.zeros(16)
-    if args.full:
-        for w in range(args.warmup):
-            eng.scf_init(start)
-            iterate(f"warmup {w + 1}", args.max_iter, True)
-        walls, iters = [], []
-        for st in range(args.steps):
-            eng.scf_init(start)
-            t0 = time.perf_counter()
-            traj, tm = iterate(f"solve {st + 1}", args.max_iter, True)
-            walls.append(time.perf_counter() - t0)
-            stage += tm
-            iters.append(len(traj))
-        wall = float(np.mean(walls))
-        timed_iters = sum(iters)
-        per_iter = sum(walls) / timed_iters
-        value, method = wall, "measured"
-        ms_per_step = wall * 1e3
-        iters_conv = iters[-1]
-        converged = bool(abs(traj[-1, 8]) < args.etol)
-    else:
-        eng.scf_init(start)
-        iterate("warmup", args.warmup, False)
+    def timed_learned_start():
         t0 = time.perf_counter()
-        traj, tm = iterate("timed", args.steps, False)
-        wall = time.perf_counter() - t0
-        stage += tm
+        learned_start(eng, model, nfill, bf16=True)
+        dt = time.perf_counter() - t0
+        log(f"learned start (first diagonalisation + MLP + guess diagonalisation): {dt:.2f}s")
+        return dt
+
+    if args.full:
+        results = {}
+        if args.path in ("both", "mixed_learned"):
+            eng.set_options(**base, precision=1, switch_de=args.switch_de, etol=0.0)
+            t0 = time.perf_counter()
+            t_init = timed_learned_start()
+            traj, tm = iterate("mixed+learned", args.max_iter, True)
+            results["mixed_learned"] = (time.perf_counter() - t0, traj, tm, t_init)
+        if args.path in ("both", "plain"):
+            eng.set_options(**base, precision=0, etol=0.0)
+            t0 = time.perf_counter()
+            eng.scf_init(seeded)
+            traj, tm = iterate("plain", args.max_iter, True)
+            results["plain"] = (time.perf_counter() - t0, traj, tm, 0.0)
+        key = "mixed_learned" if "mixed_learned" in results else "plain"
+        wall, traj, stage, t_init = results[key]
+        value, method = wall, "measured"
         timed_iters = len(traj)
-        per_iter = wall / timed_iters
-        iters_conv = ITERS_TO_CONVERGE
-        value, method = per_iter * iters_conv, "projected"
-        ms_per_step = per_iter * 1e3
+        ms_per_step = (wall - t_init) / max(timed_iters, 1) * 1e3
+        converged = bool(abs(traj[-1, 8]) < args.etol)
+        extra = {}
+        for k, (w, tr, tm, ti) in results.items():
+            extra[f"{k}_s"] = round(w, 1)
+            extra[f"{k}_iterations"] = len(tr)
+            extra[f"{k}_final_energy_eV"] = round(float(tr[-1, 1] - tr[-1, 8]), 7)
+            if k == "mixed_learned":
+                extra["mixed_learned_mixed_phase_iterations"] = int(tm[12])
+                extra["mixed_learned_start_s"] = round(ti, 2)
+        if len(results) == 2:
+            extra["speedup_vs_plain"] = round(results["plain"][0] / results["mixed_learned"][0], 3)
+        rf = roofline(stage, f"k_trd_hemv ({key} solve, FP32-copy and FP64 launches)", 0)
+    else:
+        # plain FP64 iterations from the seeded start
+        eng.set_options(**base, precision=0, etol=0.0)
+        eng.scf_init(seeded)
+        iterate("plain warmup", args.warmup, False)
+        t0 = time.perf_counter()
+        traj, stage = iterate("plain timed", args.steps, False)
+        t64 = (time.perf_counter() - t0) / max(len(traj), 1)
+        # one mixed-phase iteration (far from convergence: the switch is not reached)
+        eng.set_options(**base, precision=1, switch_de=args.switch_de, etol=0.0)
+        t0 = time.perf_counter()
+        _, tmmx = iterate("mixed timed", 1, False)
+        tmx = time.perf_counter() - t0
+        t_init = timed_learned_start()
+        nm, nf = MIXED_LEARNED_ITERS["mixed"], MIXED_LEARNED_ITERS["fp64"]
+        value = t_init + nm * tmx + nf * t64
+        plain = PLAIN_ITERS * t64
+        method = "projected"
+        timed_iters = len(traj)
+        ms_per_step = t64 * 1e3
         converged = None
-    # hemv launches of every 17th column carry HIP events on the library's
-    # stream; their algorithmic bytes (8 m^2 per matrix) are summed by the library
-    hemv_ms, hemv_launches, hemv_bytes = stage[10], int(stage[9]), stage[11]
-    avg_launch_ms = hemv_ms / max(hemv_launches, 1)
-    bytes_per_launch = hemv_bytes / max(hemv_launches, 1)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if hemv_launches else 0.0
+        rf = roofline(stage, "k_trd_hemv<FP64>", 16)
+        extra = {
+            "plain_fp64_projected_s": round(plain, 1),
+            "speedup_vs_plain": round(plain / value, 3),
+            "iterations": {"plain_fp64": PLAIN_ITERS, "mixed_learned_mixed_phase": nm,
+                           "mixed_learned_fp64_phase": nf},
+            "ms_per_iteration": {"fp64": round(t64 * 1e3, 1), "mixed": round(tmx * 1e3, 1)},
+            "learned_start_s": round(t_init, 2),
+            "roofline_mixed_phase": roofline(tmmx, "k_trd_hemv<FP32 copy>", 8),
+        }
     out = {
         "metric": "wall-time to SCF converged 1e-6 eV, 1.05deg TBG; mixed+learned vs plain FP64 GPU",
         "value": round(value, 3),
@@ -174,16 +244,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (geometry from the cell index; seeded Gaussian start density)",
+        "data": "synthetic (geometry from the cell index; seeded Gaussian start; trained MLP weights)",
         "config": {
             "workload": args.workload,
             "ndim": n, "k_grid": f"{numk}x{numk}", "nfilling": nfill, "layers": nlayers,
-            "path": "plain FP64 GPU (blocked Householder + divide-and-conquer), seeded start",
+            "path": "mixed precision (FP32-copy hemv until |dE| < %g eV) + learned start; "
+                    "plain FP64 from the seeded start as the comparison" % args.switch_de,
             "value_method": method,
-            "step": "one complete SCF solve" if args.full else "one SCF iteration",
-            "iterations_to_converge": iters_conv,
+            "step": "one SCF iteration",
             "converged": converged,
-            "ms_per_iteration": round(per_iter * 1e3, 1),
             "mixing": f"linear {args.beta}", "smearing_eV": args.kT, "etol_eV": args.etol,
             "stage_ms_per_iteration": {
                 "assembly": round(stage[0] / timed_iters, 1),
@@ -193,23 +262,13 @@ def main():
                 "density_products": round(stage[4] / timed_iters, 1),
                 "energies_mixing": round(stage[5] / timed_iters, 1),
             },
+            **extra,
         },
-        "roofline": {
-            "kernel": "k_trd_hemv",
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "timed_launches": hemv_launches,
-            "avg_launch_us": round(avg_launch_ms * 1e3, 2),
-            "algorithmic_bytes_per_launch": round(bytes_per_launch),
-        },
+        "roofline": rf,
     }
     eng.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, iters_conv, n)
+        out["cpu_baseline"] = cpu_baseline(args, PLAIN_ITERS, n)
     print(json.dumps(out), flush=True)
 
 
