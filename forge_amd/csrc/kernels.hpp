@@ -55,7 +55,7 @@ struct TridiagWork {
 };
 constexpr int TRD_NB = 32;  // panel width
 constexpr int TRD_TS = 32;     // hemv column-strip width
-constexpr int TRD_SEGR = 2048; // hemv rows per block
+constexpr int TRD_SEGR = 1024; // hemv rows per block (more blocks for small trailing orders)
 constexpr int TRD_PB = 4;      // hemv panel-product blocks per row segment
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);

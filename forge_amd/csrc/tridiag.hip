@@ -49,13 +49,18 @@ __host__ __device__ static inline i64 nt_of(i64 m) { return (m + TS - 1) / TS; }
 // row segments of strip J (rows 32J .. m-1)
 __host__ __device__ static inline i64 nseg_of(i64 m, i64 J) { return (m - TS * J + SEGR - 1) / SEGR; }
 __host__ __device__ static inline i64 nblk_of(i64 m) { return (m + 255) / 256; }
+// rows per k_trd_fin block and its block count
+constexpr int FIN_ROWS = 64;
+__host__ __device__ static inline i64 nfin_of(i64 m) { return (m + FIN_ROWS - 1) / FIN_ROWS; }
+// per-matrix stride of the norm (k_trd_col) and dot (k_trd_fin) partials
+__host__ __device__ static inline i64 npart_stride(i64 n) { return nfin_of(n) + 1; }
 
 struct TrdLayout {
   size_t A_d, A_e, A_tau, A_W, A_P, A_P2, A_tp, A_np, A_dp, A_sc, total;
 };
 
 static TrdLayout layout(i64 n, int K) {
-  i64 nt = nt_of(n), ms = nseg_of(n, 0), nb = nblk_of(n) + 1;
+  i64 nt = nt_of(n), ms = nseg_of(n, 0), nb = npart_stride(n);
   TrdLayout L;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -163,11 +168,11 @@ __device__ inline Reflector reflector_of(const z2* A, i64 lda, i64 n, i64 j, con
 }
 
 // -tau_j/2 (w_j^H v_j) of the finished column j from the partials of
-// k_trd_fin (its grid had nblk_of(n - j - 1) blocks).
+// k_trd_fin (its grid had nfin_of(n - j - 1) blocks).
 __device__ inline z2 alpha_of(i64 n, i64 j, z2 tau, const z2* dpart) {
   __shared__ z2 as;
   if (threadIdx.x < 64) {
-    const unsigned nb = (unsigned)nblk_of(n - j - 1);
+    const unsigned nb = (unsigned)nfin_of(n - j - 1);
     double sx = 0.0, sy = 0.0;
     for (unsigned b = threadIdx.x; b < nb; b += 64) {
       sx += dpart[b].x;
@@ -425,21 +430,44 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
     else t2[threadIdx.x - NB] = a;
   }
   __syncthreads();
-  const i64 r = base + (i64)blockIdx.x * 256 + threadIdx.x;
+  // FIN_ROWS rows per block, 4 lanes per row: lane g sums the strip partials
+  // J = g, g + 4, ... and panel columns q = g, g + 4, ... (two accumulators
+  // each, so several independent loads are in flight), then the 4 lane sums
+  // are combined by a fixed butterfly
+  const int g = threadIdx.x & 3;
+  const i64 r = base + (i64)blockIdx.x * FIN_ROWS + (threadIdx.x >> 2);
   z2 dot = mk(0.0, 0.0);
-  if (r < n) {
-    const i64 rel = r - base, R = rel / TS, rr = rel % TS;
-    const z2* P = P_ + (size_t)k * ntmax * n;
-    z2 y = mk(0.0, 0.0);
-    for (i64 J = 0; J <= R; ++J) y = y + P[(size_t)J * n + rel];
+  const bool live = r < n;
+  z2 y = mk(0.0, 0.0);
+  i64 rel = 0, R = 0, rr = 0;
+  if (live) {
+    rel = r - base;
+    R = rel / TS;
+    rr = rel % TS;
+    const z2* P = P_ + (size_t)k * ntmax * n + rel;
+    z2 y2 = mk(0.0, 0.0);
+    i64 J = g;
+    for (; J + 4 <= R; J += 8) {
+      y = y + P[(size_t)J * n];
+      y2 = y2 + P[(size_t)(J + 4) * n];
+    }
+    if (J <= R) y = y + P[(size_t)J * n];
+    z2 p1 = mk(0.0, 0.0), p2 = mk(0.0, 0.0);
+    for (int q = g; q < jj; q += 4) {
+      p1 = p1 + A[r + lda * (i0 + q)] * t1[q];
+      p2 = p2 + W[r + n * q] * t2[q];
+    }
+    y = (y + y2) - (p1 + p2);
+  }
+  y.x += __shfl_xor(y.x, 1, 64);
+  y.y += __shfl_xor(y.y, 1, 64);
+  y.x += __shfl_xor(y.x, 2, 64);
+  y.y += __shfl_xor(y.y, 2, 64);
+  if (live && g == 0) {
     const i64 nseg = nseg_of(m, R);
     const z2* P2 = P2_ + ((size_t)k * ntmax + R) * msmax * TS;
     for (i64 s = 0; s < nseg; ++s) y = y + P2[s * TS + rr];
     const z2 v = (r == base) ? mk(1.0, 0.0) : A[r + lda * j] * scale;
-    for (int q = 0; q < jj; ++q) {
-      y = y - A[r + lda * (i0 + q)] * t1[q];
-      y = y - W[r + n * q] * t2[q];
-    }
     const z2 w = tau * y;
     W[r + n * jj] = w;
     A[r + lda * j] = v;
@@ -593,7 +621,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes) {
   const i64 n = w.n;
   const int K = w.K;
-  const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = nblk_of(n) + 1;
+  const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = npart_stride(n);
   const bool timed = hemv_ms != nullptr;
   if (timed && w.nev == 0) {
     w.evs = new hipEvent_t[2 * TIME_RING];
@@ -634,7 +662,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
         launch_hemv<z2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const z2*)w.A, w.sA, ntmax, msmax,
                         nbmax);
       FORGE_LAUNCH();
-      hipLaunchKernelGGL(k_trd_fin, dim3(cdiv(m, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
+      hipLaunchKernelGGL(k_trd_fin, dim3((unsigned)nfin_of(m), K), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
                          msmax, nbmax);
       FORGE_LAUNCH();
