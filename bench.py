@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: wall-time to a converged SCF (|E_in - E_new| < 1e-6 eV per moire
 cell) for 1.05 deg twisted bilayer graphene (cell index 31, 11908 pz orbitals,
-3 x 3 k grid, filling +1, linear mixing 0.3, Fermi smearing kT = 1 meV) on one
-MI355X: the mixed-precision + learned-guess path against the plain FP64 path.
+3 x 3 k grid, filling +1, Fermi smearing kT = 1 meV) on one MI355X: the
+mixed-precision + learned-guess path against the plain FP64 path.
 
-  plain FP64      seeded start (uniform + 1e-3 Gaussian), every step FP64
+  plain FP64      seeded start (uniform + 1e-3 Gaussian), linear mixing 0.3,
+                  every step FP64
   mixed+learned   learned start (first diagonalisation, MLP site charges and
                   bond shifts on the matrix cores, one diagonalisation of the
-                  proposed Hamiltonian), then SCF iterations with the FP32-copy
-                  hemv while |dE| > 1e-4 eV and FP64 iterations to convergence
+                  proposed Hamiltonian), optimal-damping mixing (the
+                  reference's OptimalStep), SCF iterations with the FP32-copy
+                  hemv while |dE| > 1e-4 eV and FP64 iterations to convergence;
+                  the same path with linear mixing 0.3 is reported beside it
 
 Default mode (a few minutes): a step is one SCF iteration.  W warm-up and K
 timed plain-FP64 iterations from the seeded start, one timed mixed-phase
@@ -16,7 +19,8 @@ iteration and one timed learned start; the value is the projected time to
 convergence of the mixed+learned path = t(learned start) + N_mixed t(mixed
 iteration) + N_fp64 t(FP64 iteration), with the iteration counts of complete
 recorded solves of the same workload (MIXED_LEARNED_ITERS, PLAIN_ITERS; logs
-in profiles/).  --full: the complete solve(s), timed.
+in profiles/).  An iteration costs the same with either mixer.  --full: the
+complete solve(s), timed.
 
 The JSON line carries the roofline of the dominant kernel (k_trd_hemv, HBM
 bound; HIP events on the library's stream inside the timed region) and a CPU
@@ -41,9 +45,9 @@ HBM_PEAK_GBS = 8000.0
 # (a lower bound: the slow mode the mixed+learned solve shows at the end
 # contracts by only 0.965 per iteration)
 PLAIN_ITERS = 75
-# complete mixed+learned solve of the default workload (bench.py --full,
-# switch at 1e-4 eV): 21 mixed-phase and 52 FP64 iterations
-MIXED_LEARNED_ITERS = {"mixed": 21, "fp64": 52}
+# complete mixed+learned solves of the default workload (bench.py --full
+# --mixer oda|linear, switch at 1e-4 eV): mixed-phase and FP64 iterations
+MIXED_LEARNED_ITERS = {"oda": {"mixed": 10, "fp64": 22}, "linear": {"mixed": 21, "fp64": 52}}
 MODEL = os.path.join(REPO, "forge_amd", "data", "guess_mlp.npz")
 
 WORKLOADS = {
@@ -67,6 +71,9 @@ def parse():
     ap.add_argument("--switch-de", type=float, default=1e-4)
     ap.add_argument("--max-iter", type=int, default=150)
     ap.add_argument("--beta", type=float, default=0.3)
+    ap.add_argument("--mixer", default="oda", choices=["linear", "oda"],
+                    help="--full: mixer of the mixed+learned solve (the plain path keeps linear "
+                         "mixing); oda = the reference's OptimalStep")
     ap.add_argument("--kT", type=float, default=0.001)
     ap.add_argument("--model", default=MODEL)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -173,7 +180,8 @@ def main():
     if args.full:
         results = {}
         if args.path in ("both", "mixed_learned"):
-            eng.set_options(**base, precision=1, switch_de=args.switch_de, etol=0.0)
+            ml = {**base, "mix_mode": 0 if args.mixer == "oda" else 1}
+            eng.set_options(**ml, precision=1, switch_de=args.switch_de, etol=0.0)
             t0 = time.perf_counter()
             t_init = timed_learned_start()
             traj, tm = iterate("mixed+learned", args.max_iter, True)
@@ -215,19 +223,23 @@ def main():
         _, tmmx = iterate("mixed timed", 1, False)
         tmx = time.perf_counter() - t0
         t_init = timed_learned_start()
-        nm, nf = MIXED_LEARNED_ITERS["mixed"], MIXED_LEARNED_ITERS["fp64"]
-        value = t_init + nm * tmx + nf * t64
+        proj = {mx: t_init + c["mixed"] * tmx + c["fp64"] * t64
+                for mx, c in MIXED_LEARNED_ITERS.items()}
+        value = proj[args.mixer]
         plain = PLAIN_ITERS * t64
         method = "projected"
         timed_iters = len(traj)
         ms_per_step = t64 * 1e3
         converged = None
         rf = roofline(stage, "k_trd_hemv<FP64>", 16)
+        other = "linear" if args.mixer == "oda" else "oda"
         extra = {
             "plain_fp64_projected_s": round(plain, 1),
             "speedup_vs_plain": round(plain / value, 3),
-            "iterations": {"plain_fp64": PLAIN_ITERS, "mixed_learned_mixed_phase": nm,
-                           "mixed_learned_fp64_phase": nf},
+            f"mixed_learned_{other}_projected_s": round(proj[other], 1),
+            f"speedup_vs_plain_{other}": round(plain / proj[other], 3),
+            "iterations": {"plain_fp64_linear": PLAIN_ITERS,
+                           **{f"mixed_learned_{mx}": c for mx, c in MIXED_LEARNED_ITERS.items()}},
             "ms_per_iteration": {"fp64": round(t64 * 1e3, 1), "mixed": round(tmx * 1e3, 1)},
             "learned_start_s": round(t_init, 2),
             "roofline_mixed_phase": roofline(tmmx, "k_trd_hemv<FP32 copy>", 8),
@@ -253,7 +265,10 @@ def main():
             "value_method": method,
             "step": "one SCF iteration",
             "converged": converged,
-            "mixing": f"linear {args.beta}", "smearing_eV": args.kT, "etol_eV": args.etol,
+            "mixing": {"plain_fp64": f"linear {args.beta}",
+                       "mixed_learned": "optimal damping (OptimalStep)" if args.mixer == "oda"
+                       else f"linear {args.beta}"},
+            "smearing_eV": args.kT, "etol_eV": args.etol,
             "stage_ms_per_iteration": {
                 "assembly": round(stage[0] / timed_iters, 1),
                 "tridiagonal_reduction": round(stage[1] / timed_iters, 1),
