@@ -253,7 +253,9 @@ __device__ inline z2 widen(const float2 a) { return mk((double)a.x, (double)a.y)
 
 // TM: element type of the streamed matrix M (z2: M = A; float2: the FP32 copy)
 template <typename TM>
-__global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, const z2* A_,
+// 3 waves per SIMD (at most 168 VGPRs, no spills): the loads in flight, not
+// the arithmetic, bound this kernel (4 waves need spills and run slower)
+__global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, const z2* A_,
                                                   size_t sA, i64 lda, const TM* M_, size_t sM,
                                                   const z2* W_, const double* npart_, z2* P_,
                                                   z2* P2_, z2* tpart_, z2* scal_, i64 ntmax,
@@ -281,38 +283,40 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
     }
     const z2* W = W_ + (size_t)k * n * NB;
     const int qb = wv * 8;  // 8 panel columns per wave, lanes over rows
-    z2 t1[8], t2[8];
+    // two panel columns per pass keep this branch's registers below the
+    // strip branch's, which sets the kernel's occupancy
+    z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
+    for (int q0 = 0; q0 < 8; q0 += 2) {
+      z2 t1[2] = {mk(0.0, 0.0), mk(0.0, 0.0)}, t2[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
+      if (qb + q0 < jj) {
+        for (i64 rr = rs + lane; rr < re; rr += 64) {
+          const i64 r = base + rr;
+          const z2 vr = (rr == 0) ? mk(1.0, 0.0) : A[r + lda * j] * R.scale;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) t1[q] = t2[q] = mk(0.0, 0.0);
-    if (qb < jj) {
-      for (i64 rr = rs + lane; rr < re; rr += 64) {
-        const i64 r = base + rr;
-        const z2 vr = (rr == 0) ? mk(1.0, 0.0) : A[r + lda * j] * R.scale;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          if (qb + q < jj) {
-            t1[q] = t1[q] + cmul(W[r + n * (qb + q)], vr);
-            t2[q] = t2[q] + cmul(A[r + lda * (i0 + qb + q)], vr);
+          for (int q = 0; q < 2; ++q) {
+            if (qb + q0 + q < jj) {
+              t1[q] = t1[q] + cmul(W[r + n * (qb + q0 + q)], vr);
+              t2[q] = t2[q] + cmul(A[r + lda * (i0 + qb + q0 + q)], vr);
+            }
           }
         }
       }
-    }
-    // written also for empty chunks: k_trd_fin sums every chunk of the
-    // segments that hold rows
-    z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
+      // written also for empty chunks: k_trd_fin sums every chunk of the
+      // segments that hold rows
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
+      for (int q = 0; q < 2; ++q) {
+        double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-        c += __shfl_xor(c, off, 64);
-        d += __shfl_xor(d, off, 64);
-      }
-      if (lane == 0) {
-        tp[qb + q] = mk(a, b);
-        tp[NB + qb + q] = mk(c, d);
+        for (int off = 1; off < 64; off <<= 1) {
+          a += __shfl_xor(a, off, 64);
+          b += __shfl_xor(b, off, 64);
+          c += __shfl_xor(c, off, 64);
+          d += __shfl_xor(d, off, 64);
+        }
+        if (lane == 0) {
+          tp[qb + q0 + q] = mk(a, b);
+          tp[NB + qb + q0 + q] = mk(c, d);
+        }
       }
     }
     return;
@@ -334,9 +338,8 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
   __syncthreads();
   const int rsub = lane & 15, cg = lane >> 4;  // row in the step, column group
   const int cb = cg * 8;                       // first column of this lane's group
-  z2 vc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) vc[q] = vJ[cb + q];
+  // v of the strip's columns stays in LDS (a broadcast read per use), which
+  // keeps the kernel within 128 VGPRs
   z2 acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = mk(0.0, 0.0);
@@ -375,12 +378,22 @@ __global__ void __launch_bounds__(256) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, 
         a[q] = (valid && cb + q < ncol) ? widen(colp[r + (size_t)lda * q]) : mk(0.0, 0.0);
     }
     z2 y = mk(0.0, 0.0);
+    if (rr0 >= TS * J + TS) {
+      // below the diagonal tile (wave-uniform): every element counts in both
+      // products (rows outside [rs, re) were loaded as zeros)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const z2 zero = mk(0.0, 0.0);
-      const int c = cb + q;
-      y = y + ((c <= dr) ? a[q] : zero) * vc[q];
-      acc[q] = acc[q] + cmul((c < dr) ? a[q] : zero, vr);
+      for (int q = 0; q < 8; ++q) {
+        y = y + a[q] * vJ[cb + q];
+        acc[q] = acc[q] + cmul(a[q], vr);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const z2 zero = mk(0.0, 0.0);
+        const int c = cb + q;
+        y = y + ((c <= dr) ? a[q] : zero) * vJ[c];
+        acc[q] = acc[q] + cmul((c < dr) ? a[q] : zero, vr);
+      }
     }
     // row sum over the 4 column groups (lanes l, l^16, l^32, l^48)
     y.x += __shfl_xor(y.x, 16, 64);
