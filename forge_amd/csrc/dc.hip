@@ -22,9 +22,13 @@
 #include <utility>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "kernels.hpp"
 
 namespace forge {
+
+rocblas_handle blas_handle(hipStream_t s);
 
 constexpr int LEAF = 32;
 constexpr double EPSU = 1.1102230246251565e-16;  // unit roundoff, 2^-53
@@ -668,13 +672,38 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
     i64 ldo = n;
     // Q_out for the top merge is n x mout with ld n (caller buffer K x n x mout)
     if (top) {
-      unsigned tiles = (unsigned)(((lhalf + 63) / 64) * ((lmout + 63) / 64));
-      // caller layout: stride n*mout per matrix; use a per-matrix pointer offset
-      for (int k = 0; k < K; ++k) {
-        hipLaunchKernelGGL(k_dc_gemm, dim3(tiles, 2 * nm, 1), dim3(256), 0, s, n, md,
-                           P.Qa + (size_t)k * n * n, P.Wm + (size_t)k * n * n,
-                           Qout + (size_t)k * n * mout, ldo);
-        FORGE_LAUNCH();
+      // the top merge is the bulk of the work: its two half products go to
+      // the FP64 matrix cores (rocBLAS), all K matrices in one batched call
+      // each; caller layout: stride n * mout per matrix
+      if (nm == 1) {
+        const MergeDesc& M = all[levels[lv].first];
+        rocblas_handle h = blas_handle(s);
+        if (!h) return -1;
+        const double one = 1.0, zero = 0.0;
+        for (int half = 0; half < 2; ++half) {
+          const int rb = half ? M.n1 : 0, rn = half ? M.n2 : M.n1;
+          if (rn == 0 || M.mout == 0) continue;
+          const size_t a0 = (size_t)(M.o + rb) + (size_t)n * (M.o + rb);
+          const size_t b0 = (size_t)(M.o + rb) + (size_t)n * M.o;
+          const size_t c0 = (size_t)(M.o + rb) + (size_t)ldo * M.o;
+          const rocblas_status st = rocblas_dgemm_strided_batched(
+              h, rocblas_operation_none, rocblas_operation_none, rn, M.mout, rn, &one, P.Qa + a0,
+              (rocblas_int)n, (rocblas_stride)((size_t)n * n), P.Wm + b0, (rocblas_int)n,
+              (rocblas_stride)((size_t)n * n), &zero, Qout + c0, (rocblas_int)ldo,
+              (rocblas_stride)((size_t)n * mout), K);
+          if (st != rocblas_status_success) {
+            set_error(std::string("rocblas_dgemm_strided_batched: ") + rocblas_status_to_string(st));
+            return -1;
+          }
+        }
+      } else {
+        unsigned tiles = (unsigned)(((lhalf + 63) / 64) * ((lmout + 63) / 64));
+        for (int k = 0; k < K; ++k) {
+          hipLaunchKernelGGL(k_dc_gemm, dim3(tiles, 2 * nm, 1), dim3(256), 0, s, n, md,
+                             P.Qa + (size_t)k * n * n, P.Wm + (size_t)k * n * n,
+                             Qout + (size_t)k * n * mout, ldo);
+          FORGE_LAUNCH();
+        }
       }
       hipLaunchKernelGGL(k_dc_evcopy, dim3(cdiv(lmout, 256), nm, K), dim3(256), 0, s, n, md, P,
                          evals);

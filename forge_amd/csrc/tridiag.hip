@@ -344,7 +344,10 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = mk(0.0, 0.0);
   z2* P = P_ + ((size_t)k * ntmax + J) * n;
-  const TM* colp = M_ + k * sM + (size_t)lda * (c0 + cb);
+  // uniform base (strip start) and 32-bit per-lane offsets: the loads use a
+  // scalar base, which saves a VGPR pair per column address
+  const TM* strip = M_ + k * sM + (size_t)lda * c0;
+  const unsigned lda32 = (unsigned)lda, cofs = (unsigned)cb * lda32;
   // row steps start at an absolute multiple of 16 rows, so with lda a multiple
   // of 16 every 16-row column chunk is one aligned 256-byte segment
   const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
@@ -356,7 +359,7 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     const i64 rr = ra0 + 16 * wv + rsub;
     const bool ok = rr >= rs && rr < re;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? colp[base + rr + (size_t)lda * q] : TM{};
+    for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rr) + lda32 * q] : TM{};
   }
   for (i64 rr0 = ra0 + 16 * wv; rr0 < re; rr0 += 64) {
     const i64 rr = rr0 + rsub;  // row relative to base
@@ -371,11 +374,11 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
       const i64 rn = rr + 64;
       const bool ok = rn >= rs && rn < re;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? colp[base + rn + (size_t)lda * q] : TM{};
+      for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rn) + lda32 * q] : TM{};
     } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        a[q] = (valid && cb + q < ncol) ? widen(colp[r + (size_t)lda * q]) : mk(0.0, 0.0);
+        a[q] = (valid && cb + q < ncol) ? widen(strip[cofs + (unsigned)r + lda32 * q]) : mk(0.0, 0.0);
     }
     z2 y = mk(0.0, 0.0);
     if (rr0 >= TS * J + TS) {
