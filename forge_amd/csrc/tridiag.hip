@@ -503,12 +503,24 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
 // ---------------------------------------------------------------- her2k
 // A(p:n, p:n) -= V W^H + W V^H on the lower triangle, V = A(p:n, i0:i0+nbp),
 // W = W(p:n, 0:nbp), whose last column still lacks its -tau/2 (w^H v) v term
-// (added here while staging).  64x64 output tiles (lower tiles only), 256
-// threads with 4x4 complex outputs each, panel columns staged 8 at a time.
-__global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp, z2* A_,
-                                                   size_t sA, i64 lda, float2* S_, size_t sS,
-                                                   const z2* W_, const z2* tau_, const z2* dpart_,
-                                                   i64 nbmax) {
+// (added here while staging); 64x64 output tiles, lower tiles only.
+// On the FP64 matrix cores (v_mfma_f64_16x16x4f64).  With
+// X = [Vr Vi Wr Wi] for the tile's rows and, for its columns,
+// Yre = [Wr Wi Vr Vi], Yim = [-Wi Wr -Vi Vr]:
+//   Re(V W^H + W V^H) = X Yre^T,   Im(V W^H + W V^H) = X Yim^T,
+// two real products with k = 4 nbp.  A 64x64 tile per block; each wave owns
+// a 32x32 quadrant as 2x2 blocks of 16x16 (real and imaginary accumulators).
+// Panel columns are staged 8 at a time as their four real components; the
+// k-step of 4 covers panel columns q0 + 4 (lane / 16) ... of one component.
+// MFMA 16x16x4 f64 lane layout (checked on the device): A row lane % 16,
+// k lane / 16; B k lane / 16, column lane % 16; D row lane / 16 + 4 i,
+// column lane % 16 for output i = 0..3.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, int nbp, z2* A_,
+                                                        size_t sA, i64 lda, float2* S_,
+                                                        size_t sS, const z2* W_, const z2* tau_,
+                                                        const z2* dpart_, i64 nbmax) {
   const int k = blockIdx.y;
   z2* A = A_ + k * sA;
   float2* S = S_ ? S_ + k * sS : nullptr;
@@ -521,13 +533,16 @@ __global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp
   while (I * (I + 1) / 2 > t) --I;
   const i64 J = t - I * (I + 1) / 2;
   const i64 r0 = p + I * 64, c0 = p + J * 64;
-  __shared__ z2 Vr[8][64], Wr[8][64], Vc[8][64], Wc[8][64];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  z2 acc[4][4];
+  // [component: Vr Vi Wr Wi][panel column 0..7][tile row / column]
+  __shared__ double Xr[4][8][65], Xc[4][8][65];  // padded: the 4 k rows of a step hit different banks
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int qr = wv >> 1, qc = wv & 1;  // the wave's quadrant
+  const int l16 = lane & 15, lk = lane >> 4;
+  f64x4 re[2][2], im[2][2];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = mk(0.0, 0.0);
+    for (int v = 0; v < 2; ++v) re[u][v] = im[u][v] = f64x4{0.0, 0.0, 0.0, 0.0};
   const z2 zero = mk(0.0, 0.0);
   for (int q0 = 0; q0 < nbp; q0 += 8) {
     __syncthreads();
@@ -543,39 +558,50 @@ __global__ void __launch_bounds__(256) k_trd_her2k(i64 n, i64 p, i64 i0, int nbp
         wr = wr + alpha * vr;
         wc = wc + alpha * vc;
       }
-      Vr[qq][rr] = vr;
-      Wr[qq][rr] = wr;
-      Vc[qq][rr] = vc;
-      Wc[qq][rr] = wc;
+      Xr[0][qq][rr] = vr.x; Xr[1][qq][rr] = vr.y; Xr[2][qq][rr] = wr.x; Xr[3][qq][rr] = wr.y;
+      Xc[0][qq][rr] = vc.x; Xc[1][qq][rr] = vc.y; Xc[2][qq][rr] = wc.x; Xc[3][qq][rr] = wc.y;
     }
     __syncthreads();
+    // component pairs: X comp -> (Yre comp, sign), (Yim comp, sign)
+    //   Vr: (Wr, +) (Wi, -);  Vi: (Wi, +) (Wr, +);  Wr: (Vr, +) (Vi, -);  Wi: (Vi, +) (Vr, +)
 #pragma unroll
-    for (int qq = 0; qq < 8; ++qq) {
-      z2 va[4], wa[4], vb[4], wb[4];
+    for (int comp = 0; comp < 4; ++comp) {
+      const int cre = comp ^ 2, cim = (comp ^ 2) ^ 1;
+      const double sim = (comp & 1) ? 1.0 : -1.0;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        va[u] = Vr[qq][tx + 16 * u];
-        wa[u] = Wr[qq][tx + 16 * u];
-        vb[u] = Vc[qq][ty + 16 * u];
-        wb[u] = Wc[qq][ty + 16 * u];
+      for (int kq = 0; kq < 8; kq += 4) {
+        const int q = kq + lk;
+        double a[2], bre[2], bim[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) a[u] = Xr[comp][q][qr * 32 + u * 16 + l16];
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          bre[v] = Xc[cre][q][qc * 32 + v * 16 + l16];
+          bim[v] = sim * Xc[cim][q][qc * 32 + v * 16 + l16];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            re[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bre[v], re[u][v], 0, 0, 0);
+            im[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bim[v], im[u][v], 0, 0, 0);
+          }
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = acc[u][v] + mulc(va[u], wb[v]) + mulc(wa[u], vb[v]);
     }
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const i64 r = r0 + tx + 16 * u, c = c0 + ty + 16 * v;
-      if (r < n && c < n && r >= c) {
-        const z2 a = A[r + lda * c] - acc[u][v];
-        A[r + lda * c] = a;
-        if (S) S[r + lda * c] = make_float2((float)a.x, (float)a.y);
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const i64 r = r0 + qr * 32 + u * 16 + lk + 4 * i, c = c0 + qc * 32 + v * 16 + l16;
+        if (r < n && c < n && r >= c) {
+          const z2 a = A[r + lda * c] - mk(re[u][v][i], im[u][v][i]);
+          A[r + lda * c] = a;
+          if (S) S[r + lda * c] = make_float2((float)a.x, (float)a.y);
+        }
       }
-    }
 }
 
 // S = fp32(A) on the lower triangle (column c, rows c..n-1); grid
@@ -684,8 +710,8 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
       FORGE_LAUNCH();
     }
     const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
-    hipLaunchKernelGGL(k_trd_her2k, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s, n,
-                       p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
+    hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s,
+                       n, p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
     FORGE_LAUNCH();
   }
   hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);
