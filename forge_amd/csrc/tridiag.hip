@@ -9,7 +9,7 @@
 //   reflector       beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha)/beta,
 //                   v = (1, x / (alpha - beta)), e_j = beta
 //   w               = tau (A22 v - V (W^H v) - W (V^H v)),   w += -tau/2 (w^H v) v
-//   panel end       A22 -= V W^H + W V^H   (lower triangle, k_trd_her2k)
+//   panel end       A22 -= V W^H + W V^H   (lower triangle, k_trd_her2k_mfma)
 //
 // Each column is three launches over all K matrices, with no atomics and no
 // device-scope fences (the L2 caches of the XCDs are not coherent with each
@@ -23,11 +23,11 @@
 //                partials in dedicated blocks
 //   k_trd_fin    w = tau (A22 v - V W^H v - W V^H v), v written back, w^H v
 //                partials
-// The correction of the panel's last w is applied on the fly by k_trd_her2k.
+// The correction of the panel's last w is applied on the fly by k_trd_her2k_mfma.
 //
 // Mixed precision (TridiagWork::S set): the hemv streams an FP32 copy of the
 // trailing matrix, written by k_trd_to_fp32 before the first panel and by
-// k_trd_her2k after every trailing update, and accumulates in FP64 against the
+// k_trd_her2k_mfma after every trailing update, and accumulates in FP64 against the
 // FP64 Householder vector.  The hemv is bound by HBM bandwidth, so halving its
 // bytes halves its time; the reduction's backward error becomes that of the
 // FP32 rounding of A22 (|dA| <= 2^-24 |A|), while every other quantity (the
