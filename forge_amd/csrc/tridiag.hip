@@ -673,7 +673,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   i64 nrec = 0;
   double bytes = 0.0, tms = 0.0;
   if (w.S) {
-    hipLaunchKernelGGL(k_trd_to_fp32, dim3((unsigned)cdiv(n, 256), (unsigned)std::min<i64>(n, 32768), K),
+    hipLaunchKernelGGL(k_trd_to_fp32, dim3((unsigned)cdiv(n, 256), (unsigned)std::min<i64>(n, 512), K),
                        dim3(256), 0, s, n, w.A, w.sA, w.lda, w.S, w.sS);
     FORGE_LAUNCH();
   }
@@ -713,6 +713,11 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s,
                        n, p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
     FORGE_LAUNCH();
+    // bound the launches in flight (about 400): the host otherwise runs up to
+    // 3 n launches ahead, which rocprofv3's dispatch tracing does not survive
+    // (host fault in its record buffers beyond ~15000 pending dispatches);
+    // the GPU idles only for the next launch's latency, ~100 times per call
+    if ((i0 / NB) % 4 == 3) FORGE_HIP(hipStreamSynchronize(s));
   }
   hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);
   FORGE_LAUNCH();
