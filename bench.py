@@ -10,7 +10,8 @@ mixed-precision + learned-guess path against the plain FP64 path.
                   bond shifts on the matrix cores, one diagonalisation of the
                   proposed Hamiltonian), optimal-damping mixing (the
                   reference's OptimalStep), SCF iterations with the FP32-copy
-                  hemv while |dE| > 2e-6 eV and FP64 iterations to convergence;
+                  hemv and the FP32 back-transformation while |dE| > 2e-6 eV
+                  and FP64 iterations to convergence;
                   the same path with linear mixing 0.3 is reported beside it
 
 Default mode (a few minutes): a step is one SCF iteration.  W warm-up and K
@@ -272,7 +273,7 @@ def main():
         "config": {
             "workload": args.workload,
             "ndim": n, "k_grid": f"{numk}x{numk}", "nfilling": nfill, "layers": nlayers,
-            "path": "mixed precision (FP32-copy hemv until |dE| < %g eV) + learned start; "
+            "path": "mixed precision (FP32-copy hemv and FP32 back-transformation until |dE| < %g eV) + learned start; "
                     "plain FP64 from the seeded start as the comparison" % args.switch_de,
             "value_method": method,
             "step": "one SCF iteration",

@@ -4,8 +4,15 @@
 // LapackRoutines.f90:414-426).  Reflectors are applied in blocks of BT in the
 // compact WY form  H_i0 ... H_{i0+b-1} = I - V T V^H  (T upper triangular,
 // LAPACK zlarft 'F','C'), last block first:
-//     Y = V^H C,  Y <- T Y,  C <- C - V Y          (rocBLAS zgemm, batched over k)
+//     Y = V^H C,  Y <- T Y,  C <- C - V Y          (rocBLAS gemm, batched over k)
 // The clean V (unit diagonal, zeros above) and T are built by small kernels.
+//
+// FP32 variant (the mixed-precision phase of the SCF): the same blocks in
+// complex FP32 (cgemm on the FP32 matrix cores, V, T and C rounded to FP32),
+// then every column is scaled to unit 2-norm in FP64 and written to the FP64
+// output.  Eigenvector errors then enter the band energies only at second
+// order: for psi = psi* + d, <psi|H|psi> / <psi|psi> - eps = O(|d|^2), while
+// an unnormalised column would carry the first-order term 2 eps Re<psi*|d>.
 #include <rocblas/rocblas.h>
 
 #include "kernels.hpp"
@@ -20,18 +27,20 @@ size_t backtr_workspace_bytes(i64 n, int K, i64 m) {
   return sizeof(z2) * per * K + 4096;
 }
 
+template <typename E>
 __global__ void k_bt_promote(size_t nm, const double* __restrict__ Z, size_t ldZ, size_t sZ,
-                             i64 n, i64 m, z2* __restrict__ C, size_t ldC, size_t sC) {
+                             i64 n, i64 m, E* __restrict__ C, size_t ldC, size_t sC) {
   int k = blockIdx.y;
   for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nm; q += (size_t)gridDim.x * 256) {
     i64 r = (i64)(q % n), c = (i64)(q / n);
-    C[k * sC + r + ldC * c] = mk(Z[k * sZ + r + ldZ * c], 0.0);
+    C[k * sC + r + ldC * c] = narrow<E>(mk(Z[k * sZ + r + ldZ * c], 0.0));
   }
 }
 
 // V(r', q) for rows r = i0+1+r' of reflector j = i0+q: 1 at r = j+1, A(r, j) below.
+template <typename E>
 __global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, size_t sA, i64 lda,
-                            z2* __restrict__ V, size_t sV) {
+                            E* __restrict__ V, size_t sV) {
   int k = blockIdx.y;
   i64 mt = n - i0 - 1;
   for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < (size_t)mt * nb;
@@ -42,34 +51,60 @@ __global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, siz
     if (r == j + 1) v = mk(1.0, 0.0);
     else if (r > j + 1) v = A[k * sA + r + lda * j];
     else v = mk(0.0, 0.0);
-    V[k * sV + rr + (size_t)n * c] = v;
+    V[k * sV + rr + (size_t)n * c] = narrow<E>(v);
   }
 }
 
 // T (nb x nb, upper) from G = V^H V and tau, zlarft forward/columnwise:
 //   T(i,i) = tau_i,  T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).
-__global__ void k_bt_tbuild(int nb, i64 i0, i64 n, const z2* __restrict__ G, const z2* __restrict__ tau,
-                            z2* __restrict__ T) {
+// The recurrence runs in FP64 on the stored (FP64 or FP32) entries.
+template <typename E>
+__global__ void k_bt_tbuild(int nb, i64 i0, i64 n, const E* __restrict__ G, const z2* __restrict__ tau,
+                            E* __restrict__ T) {
   int k = blockIdx.x;
-  const z2* g = G + (size_t)k * BT * BT;
-  z2* t = T + (size_t)k * BT * BT;
+  const E* g = G + (size_t)k * BT * BT;
+  E* t = T + (size_t)k * BT * BT;
   const z2* tk = tau + (size_t)k * n + i0;
   __shared__ z2 col[BT];
-  for (int q = threadIdx.x; q < BT * BT; q += blockDim.x) t[q] = mk(0.0, 0.0);
+  for (int q = threadIdx.x; q < BT * BT; q += blockDim.x) t[q] = narrow<E>(mk(0.0, 0.0));
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
     // col(0:i) = -tau_i G(0:i, i)
-    for (int r = threadIdx.x; r < i; r += blockDim.x) col[r] = (-1.0) * (tk[i] * g[r + BT * i]);
+    for (int r = threadIdx.x; r < i; r += blockDim.x) col[r] = (-1.0) * (tk[i] * widen(g[r + BT * i]));
     __syncthreads();
     // T(0:i, i) = T(0:i,0:i) col
     for (int r = threadIdx.x; r < i; r += blockDim.x) {
       z2 s = mk(0.0, 0.0);
-      for (int c = r; c < i; ++c) s = s + t[r + BT * c] * col[c];
-      t[r + BT * i] = s;
+      for (int c = r; c < i; ++c) s = s + widen(t[r + BT * c]) * col[c];
+      t[r + BT * i] = narrow<E>(s);
     }
-    if (threadIdx.x == 0) t[i + BT * i] = tk[i];
+    if (threadIdx.x == 0) t[i + BT * i] = narrow<E>(tk[i]);
     __syncthreads();
   }
+}
+
+// C(:, j) = C32(:, j) / |C32(:, j)|_2 in FP64.  grid (m, K), 256 threads.
+__global__ void __launch_bounds__(256) k_bt_normalise(i64 n, const float2* __restrict__ C32, size_t ldC32,
+                                                      size_t sC32, z2* __restrict__ C, size_t ldC,
+                                                      size_t sC) {
+  const int k = blockIdx.y;
+  const i64 j = blockIdx.x;
+  const float2* c32 = C32 + k * sC32 + ldC32 * j;
+  z2* c64 = C + k * sC + ldC * j;
+  double ss = 0.0;
+  for (i64 r = threadIdx.x; r < n; r += 256) {
+    const float2 v = c32[r];
+    ss += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = ss;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double inv = red[0] > 0.0 ? 1.0 / sqrt(red[0]) : 0.0;
+  for (i64 r = threadIdx.x; r < n; r += 256) c64[r] = inv * widen(c32[r]);
 }
 
 #define RB(call)                                                              \
@@ -83,57 +118,81 @@ __global__ void k_bt_tbuild(int nb, i64 i0, i64 n, const z2* __restrict__ G, con
 
 rocblas_handle blas_handle(hipStream_t s);
 
-int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
-                  size_t ldC, size_t sC, void* work, hipStream_t s) {
+// rocBLAS complex gemm for each storage type
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, i64 M,
+                              i64 N, i64 Kd, double alpha, const z2* A, i64 lda, size_t sA,
+                              const z2* B, i64 ldb, size_t sB, double beta, z2* C, i64 ldc,
+                              size_t sC, int batch) {
+  const rocblas_double_complex a(alpha, 0.0), b(beta, 0.0);
+  return rocblas_zgemm_strided_batched(
+      h, ta, tb, (rocblas_int)M, (rocblas_int)N, (rocblas_int)Kd, &a,
+      (const rocblas_double_complex*)A, (rocblas_int)lda, (rocblas_stride)sA,
+      (const rocblas_double_complex*)B, (rocblas_int)ldb, (rocblas_stride)sB, &b,
+      (rocblas_double_complex*)C, (rocblas_int)ldc, (rocblas_stride)sC, batch);
+}
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, i64 M,
+                              i64 N, i64 Kd, double alpha, const float2* A, i64 lda, size_t sA,
+                              const float2* B, i64 ldb, size_t sB, double beta, float2* C,
+                              i64 ldc, size_t sC, int batch) {
+  const rocblas_float_complex a((float)alpha, 0.0f), b((float)beta, 0.0f);
+  return rocblas_cgemm_strided_batched(
+      h, ta, tb, (rocblas_int)M, (rocblas_int)N, (rocblas_int)Kd, &a,
+      (const rocblas_float_complex*)A, (rocblas_int)lda, (rocblas_stride)sA,
+      (const rocblas_float_complex*)B, (rocblas_int)ldb, (rocblas_stride)sB, &b,
+      (rocblas_float_complex*)C, (rocblas_int)ldc, (rocblas_stride)sC, batch);
+}
+
+// C (ldC, stride sC per matrix) <- Q Z, blocks last to first, in storage type E.
+template <typename E>
+static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, E* C,
+                   size_t ldC, size_t sC, void* work, hipStream_t s) {
   const i64 n = t.n;
   const int K = t.K;
-  z2* V = (z2*)work;
-  z2* G = V + (size_t)n * BT * K;
-  z2* T = G + (size_t)BT * BT * K;
-  z2* Y = T + (size_t)BT * BT * K;
-  z2* Y2 = Y + (size_t)BT * m * K;
+  E* V = (E*)work;
+  E* G = V + (size_t)n * BT * K;
+  E* T = G + (size_t)BT * BT * K;
+  E* Y = T + (size_t)BT * BT * K;
+  E* Y2 = Y + (size_t)BT * m * K;
   const size_t sV = (size_t)n * BT, sG = (size_t)BT * BT, sY = (size_t)BT * m;
-  hipLaunchKernelGGL(k_bt_promote, dim3(2048, K), dim3(256), 0, s, (size_t)n * m, Z, ldZ, sZ, n,
+  hipLaunchKernelGGL(k_bt_promote<E>, dim3(2048, K), dim3(256), 0, s, (size_t)n * m, Z, ldZ, sZ, n,
                      m, C, ldC, sC);
   FORGE_LAUNCH();
   if (n < 2) return 0;
   rocblas_handle h = blas_handle(s);
   if (!h) return -1;
-  const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
+  const auto CT = rocblas_operation_conjugate_transpose, NO = rocblas_operation_none;
   const i64 nblocks = (n - 1 + BT - 1) / BT;
   for (i64 bq = nblocks - 1; bq >= 0; --bq) {
     const i64 i0 = bq * BT;
     const int nb = (int)((n - 1 - i0) < BT ? (n - 1 - i0) : BT);
     const i64 mt = n - i0 - 1;
-    hipLaunchKernelGGL(k_bt_vclean, dim3(cdiv(mt * nb, 256) < 4096 ? cdiv(mt * nb, 256) : 4096, K),
+    hipLaunchKernelGGL(k_bt_vclean<E>, dim3(cdiv(mt * nb, 256) < 4096 ? cdiv(mt * nb, 256) : 4096, K),
                        dim3(256), 0, s, n, i0, nb, t.A, t.sA, t.lda, V, sV);
     FORGE_LAUNCH();
     // G = V^H V
-    RB(rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                                     nb, nb, (rocblas_int)mt, &one, (const rocblas_double_complex*)V,
-                                     (rocblas_int)n, sV, (const rocblas_double_complex*)V,
-                                     (rocblas_int)n, sV, &zero, (rocblas_double_complex*)G, BT, sG,
-                                     K));
-    hipLaunchKernelGGL(k_bt_tbuild, dim3(K), dim3(128), 0, s, nb, i0, n, G, t.tau, T);
+    RB(gemm_sb(h, CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K));
+    hipLaunchKernelGGL(k_bt_tbuild<E>, dim3(K), dim3(128), 0, s, nb, i0, n, G, t.tau, T);
     FORGE_LAUNCH();
     // Y = V^H C(i0+1:n, :)
-    RB(rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                                     nb, (rocblas_int)m, (rocblas_int)mt, &one,
-                                     (const rocblas_double_complex*)V, (rocblas_int)n, sV,
-                                     (const rocblas_double_complex*)(C + i0 + 1), (rocblas_int)ldC,
-                                     sC, &zero, (rocblas_double_complex*)Y, BT, sY, K));
+    RB(gemm_sb(h, CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K));
     // Y2 = T Y
-    RB(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nb,
-                                     (rocblas_int)m, nb, &one, (const rocblas_double_complex*)T, BT,
-                                     sG, (const rocblas_double_complex*)Y, BT, sY, &zero,
-                                     (rocblas_double_complex*)Y2, BT, sY, K));
+    RB(gemm_sb(h, NO, NO, nb, m, nb, 1.0, T, BT, sG, Y, BT, sY, 0.0, Y2, BT, sY, K));
     // C(i0+1:n, :) -= V Y2
-    RB(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none,
-                                     (rocblas_int)mt, (rocblas_int)m, nb, &mone,
-                                     (const rocblas_double_complex*)V, (rocblas_int)n, sV,
-                                     (const rocblas_double_complex*)Y2, BT, sY, &one,
-                                     (rocblas_double_complex*)(C + i0 + 1), (rocblas_int)ldC, sC,
-                                     K));
+    RB(gemm_sb(h, NO, NO, mt, m, nb, -1.0, V, n, sV, Y2, BT, sY, 1.0, C + i0 + 1, ldC, sC, K));
+  }
+  return 0;
+}
+
+int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
+                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32) {
+  if (!C32) return apply_q<z2>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s);
+  const size_t ld32 = (size_t)t.n, s32 = (size_t)t.n * m;
+  int r = apply_q<float2>(t, Z, ldZ, sZ, m, C32, ld32, s32, work, s);
+  if (r) return r;
+  if (m > 0) {
+    hipLaunchKernelGGL(k_bt_normalise, dim3((unsigned)m, t.K), dim3(256), 0, s, t.n, C32, ld32, s32,
+                       C, ldC, sC);
+    FORGE_LAUNCH();
   }
   return 0;
 }

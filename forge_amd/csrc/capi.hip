@@ -422,10 +422,15 @@ extern "C" int forge_hamiltonian_hf(forge_ctx* c, forge_z* zH, const double* Pot
 
 // Eigenpairs 1..mout of the Kb' matrices in c->H (lower): evals (Kb x n),
 // Cv (Kb x n x nU, ld n).  Returns D&C failures (> 0) or < 0.
+// precision 1 before the switch: FP32 copy in the hemv and FP32 back-transform
+static bool mixed_phase(const forge_ctx* c) {
+  return c->opt.precision == 1 && !c->mixed_done && c->S;
+}
+
 static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   const i64 n = c->n;
   c->trd.K = K;
-  const bool mixed = c->opt.precision == 1 && !c->mixed_done && c->S;
+  const bool mixed = mixed_phase(c);
   c->trd.S = mixed ? c->S : nullptr;
   c->trd.sS = (size_t)c->ldH * n;
   double hemv_ms = 0, hemv_bytes = 0;
@@ -437,7 +442,10 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   int fails = dc_solve(c->trd.d, c->trd.e, n, K, mout, c->evals, c->Zq, c->dc_ws, c->s);
   if (fails < 0) return fails;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[4], c->s));
-  CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws, c->s));
+  // mixed phase: the FP32 copy is free once the reduction is done and holds
+  // the FP32 eigenvectors (n x mout x K <= ldH x n x K)
+  CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
+                   c->s, mixed ? c->S : nullptr));
   if (timed) {
     FORGE_HIP(hipEventRecord(c->ev[5], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[5]));

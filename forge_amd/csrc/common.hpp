@@ -45,6 +45,14 @@ __host__ __device__ inline z2 mulc(z2 a, z2 b) {
 __host__ __device__ inline z2 cmul(z2 a, z2 b) {
   return z2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
+// element conversions between FP64 (z2) and FP32 (float2) complex storage
+__device__ inline z2 widen(const z2 a) { return a; }
+__device__ inline z2 widen(const float2 a) { return mk((double)a.x, (double)a.y); }
+template <typename E> __device__ inline E narrow(z2 a);
+template <> __device__ inline z2 narrow<z2>(z2 a) { return a; }
+template <> __device__ inline float2 narrow<float2>(z2 a) {
+  return make_float2((float)a.x, (float)a.y);
+}
 
 // ---------------------------------------------------------------- pair functions
 struct PairParams {

@@ -53,6 +53,8 @@ rocblas_handle blas_handle(hipStream_t s);
 
 // rho (nnc + 1 matrices: the cells, then the R scratch) receives
 // sum_k ph_c(k) conj(R_k); accumulate = false overwrites at the first k.
+// The product stays FP64 in every phase: rounding of R enters the energies at
+// first order (E_kin and the Hartree terms are linear in rho).
 int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
                  const double* phase_cells, z2* rho, bool accumulate, hipStream_t s) {
   const size_t nn = (size_t)n * n;

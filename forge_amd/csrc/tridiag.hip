@@ -248,9 +248,6 @@ __global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z
 //   P[J][r]      = sum_c A(r, c) v_c                (c <= r inside the diagonal tile)
 //   P2[J][s][c]  = sum_r conj(A(r, c)) v_r          (r >  c inside the diagonal tile)
 // Blocks x < PB form the W^H v and V^H v partials of chunk x of row segment s.
-__device__ inline z2 widen(const z2 a) { return a; }
-__device__ inline z2 widen(const float2 a) { return mk((double)a.x, (double)a.y); }
-
 // TM: element type of the streamed matrix M (z2: M = A; float2: the FP32 copy)
 template <typename TM>
 // 3 waves per SIMD (at most 168 VGPRs, no spills): the loads in flight, not

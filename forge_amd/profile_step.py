@@ -2,7 +2,7 @@
 start and N SCF iterations of one system, with no timing of its own.
 
     python -m forge_amd.profile_step [--ntheta 31] [--nlayers 2] [--numk 3]
-        [--nfilling 1] [--iters 1] [--precision 0|1]
+        [--nfilling 1] [--iters 1] [--precision 0|1] [--stages]
 
 Run under rocprofv3 (kernel trace, counters) to attribute device time and
 HBM traffic to kernels for exactly the work of N iterations.
@@ -25,12 +25,14 @@ def main():
     ap.add_argument("--nfilling", type=float, default=1.0)
     ap.add_argument("--iters", type=int, default=1)
     ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--stages", action="store_true",
+                    help="time the stages of each iteration (device events)")
     a = ap.parse_args()
     t0 = time.perf_counter()
     e = Engine(Setup(ntheta=a.ntheta, nlayers=a.nlayers, numk=a.numk, nfilling=a.nfilling,
                      xi_nm=40.0))
     e.set_options(occ_mode=2, mix_mode=1, kT=0.001, beta=0.3, precision=a.precision,
-                  switch_de=1e-12)
+                  switch_de=1e-12, timing=1 if a.stages else 0)
     e.scf_init(seeded_rho(e.n, e.nnc, a.nfilling))
     print(f"[profile_step] n={e.n} Nk={e.Nk} set-up {time.perf_counter() - t0:.1f}s", flush=True)
     for it in range(a.iters):
@@ -38,6 +40,12 @@ def main():
         r = e.scf_run(1)
         print(f"[profile_step] iteration {it + 1}: E_tot {r[0, 1]:.9f} dE {r[0, 8]:+.3e} "
               f"({time.perf_counter() - t1:.2f}s)", flush=True)
+        if a.stages:
+            t = e.timings()
+            print("[profile_step]   stage ms: " + " ".join(
+                f"{name} {t[q]:.1f}" for q, name in enumerate(
+                    ("assembly", "reduction", "tridiag_eig", "back_transform", "density",
+                     "energies_mixing"))), flush=True)
     e.close()
     return 0
 
