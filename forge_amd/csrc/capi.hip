@@ -510,8 +510,10 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
 // Solve on the device: rho_dev (nnc cells) and pot_dev in, bulk density
 // (unnormalised, all cells) into c->rho_new, window bands to hband (nb x Nk,
 // host) and window eigenvectors to c->Cwin.
+// eig_ready: a sweep has already assembled and solved all Nk matrices (one
+// batch) into c->evals / c->Cv; only the densities and bands are formed here.
 static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
-                        std::vector<double>& hband, bool timed) {
+                        std::vector<double>& hband, bool timed, bool eig_ready = false) {
   const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nU = c->nU, nb = c->nb;
   const size_t nn = (size_t)n * n;
   hband.assign(nb * Nk, 0.0);
@@ -519,21 +521,23 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
   int fails = 0;
   for (i64 k0 = 0; k0 < Nk; k0 += c->Kb) {
     const int K = (int)std::min<i64>(c->Kb, Nk - k0);
-    if (timed) FORGE_HIP(hipEventRecord(c->ev[0], c->s));
-    CK(assemble(n, (int)nnc, K, rho_dev, c->V, c->T, pot_dev, c->bias, c->P.alpha,
-                c->ph_asm + k0 * nnc * 2, c->H, c->ldH, (size_t)c->ldH * n, c->s));
-    if (timed) {
-      FORGE_HIP(hipEventRecord(c->ev[1], c->s));
-      FORGE_HIP(hipEventSynchronize(c->ev[1]));
-      float a;
-      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-      c->tm[0] += a;
-      c->tm[7] += 1;
-      c->tm[8] += a;
+    if (!eig_ready) {
+      if (timed) FORGE_HIP(hipEventRecord(c->ev[0], c->s));
+      CK(assemble(n, (int)nnc, K, rho_dev, c->V, c->T, pot_dev, c->bias, c->P.alpha,
+                  c->ph_asm + k0 * nnc * 2, c->H, c->ldH, (size_t)c->ldH * n, c->s));
+      if (timed) {
+        FORGE_HIP(hipEventRecord(c->ev[1], c->s));
+        FORGE_HIP(hipEventSynchronize(c->ev[1]));
+        float a;
+        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        c->tm[0] += a;
+        c->tm[7] += 1;
+        c->tm[8] += a;
+      }
+      int f = eig_batch(c, K, nU, timed);
+      if (f < 0) return f;
+      fails += f;
     }
-    int f = eig_batch(c, K, nU, timed);
-    if (f < 0) return f;
-    fails += f;
     if (timed) FORGE_HIP(hipEventRecord(c->ev[6], c->s));
     // bulk density: bands 1..nLower-1, phases of k0..k0+K-1, summed over k
     // (the first k of the first batch overwrites)
@@ -944,94 +948,107 @@ extern "C" int forge_scf_init_guess(forge_ctx* c, const forge_z* rho_guess) {
   return guess_diagonalisation(c);
 }
 
+// One SCF iteration of Main.f90:380-600: solve at every k, Fermi level and
+// window density, energies, OptimalStep or linear mixing, new potential.
+// traj_row (12 values) may be null; *stop: converged (FP64 iteration with
+// |dE| < etol).  eig_ready: see solve_device.  Returns D&C failures or < 0.
+static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, bool* stop) {
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  *stop = false;
+  std::vector<double> hda(n), hdb(n);
+  std::vector<double> band;
+  const bool mixed = c->opt.precision == 1 && !c->mixed_done;
+  if (mixed) c->tm[12] += 1;
+  int fails = solve_device(c, c->rho, c->pot, band, timed, eig_ready);
+  if (fails < 0) return fails;
+  if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
+  Occupation occ;
+  CK(window_step(c, band, occ));
+  if (timed) {
+    FORGE_HIP(hipEventRecord(c->ev[11], c->s));
+    FORGE_HIP(hipEventSynchronize(c->ev[11]));
+    float a;
+    hipEventElapsedTime(&a, c->ev[10], c->ev[11]);
+    c->tm[4] += a;
+    FORGE_HIP(hipEventRecord(c->ev[12], c->s));
+  }
+  CK(site_density(n, c->rho_new, c->db, c->s));
+  double pc[13];
+  CK(energy_pieces(c, c->rho, c->rho_new, pc));
+  double ein[4] = {c->e[0], c->e[1], c->e[2], c->e[3]};
+  const double EtotIn = c->Etot;
+  double step;
+  double e1[4], eh[4];
+  energies_at(pc, 1.0, e1);
+  energies_at(pc, 0.5, eh);
+  if (c->opt.mix_mode == 0) {
+    // OptimalStep, HartreeFock.f90:1162-1208
+    double c0 = EtotIn, E1 = total_energy(c->P, e1), Eh = total_energy(c->P, eh);
+    double c1 = 4.0 * Eh - E1 - 3.0 * c0, c2 = E1 - c0 - c1;
+    if (c2 > 0 && std::fabs(-c1 / 2.0 / c2 - 0.5) < 0.5) step = -c1 / 2.0 / c2;
+    else if (E1 < c0) step = 1.0;
+    else step = c->P.step_alt;
+  } else {
+    step = c->opt.beta;
+  }
+  energies_at(pc, step, c->e);
+  c->Etot = total_energy(c->P, c->e);
+  // DensityConvergence = max |D_mixed - D_in| (Main.f90:564-571)
+  FORGE_HIP(hipMemcpyAsync(hda.data(), c->da, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(hdb.data(), c->db, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  CK(mix(nnc * nn, c->rho, c->rho_new, step, c->s));
+  hipLaunchKernelGGL(k_potential, dim3(cdiv(n, 256)), dim3(256), 0, c->s, n, c->ya, c->yb, c->da,
+                     c->db, step, c->P.alphaH, c->P.U, c->pot, c->da);
+  FORGE_LAUNCH();
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  double dconv = 0;
+  for (i64 q = 0; q < n; ++q) {
+    double dmix = (1.0 - step) * hda[q] + step * hdb[q];
+    dconv = std::max(dconv, std::fabs(dmix - hda[q]));
+  }
+  if (timed) {
+    FORGE_HIP(hipEventRecord(c->ev[13], c->s));
+    FORGE_HIP(hipEventSynchronize(c->ev[13]));
+    float a;
+    hipEventElapsedTime(&a, c->ev[12], c->ev[13]);
+    c->tm[5] += a;
+  }
+  ++c->it;
+  if (traj_row) {
+    double* r = traj_row;
+    r[0] = (double)c->it; r[1] = EtotIn; r[2] = 2 * ein[0]; r[3] = 2 * ein[1]; r[4] = ein[2];
+    r[5] = ein[3]; r[6] = dconv; r[7] = step; r[8] = EtotIn - c->Etot; r[9] = occ.EF;
+    r[10] = (double)occ.nOcc; r[11] = (double)occ.nPart;
+  }
+  const double dE = std::fabs(EtotIn - c->Etot);
+  if (mixed) {
+    // precision 1: FP64 from here on once the energy change reaches the
+    // switch threshold (default 1e-4 eV); convergence is only declared on
+    // FP64 iterations
+    const double sw = c->opt.switch_de > 0 ? c->opt.switch_de : 1e-4;
+    if (dE < sw || (c->opt.etol > 0 && dE < c->opt.etol)) c->mixed_done = true;
+    return fails;
+  }
+  if (c->opt.etol > 0 && dE < c->opt.etol) *stop = true;
+  return fails;
+}
+
 extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64_t* iters_done) {
   if (!c || max_iter < 0) { set_error("forge_scf_run: invalid argument"); return -2; }
   if (!c->scf_ready) { set_error("forge_scf_run: call forge_scf_init first"); return -5; }
-  const i64 n = c->n, nnc = c->nnc;
-  const size_t nn = (size_t)n * n;
   std::fill(c->tm, c->tm + 16, 0.0);
   const bool timed = c->opt.timing != 0;
   int fails = 0;
   i64 done = 0;
-  std::vector<double> hda(n), hdb(n);
   FORGE_HIP(hipEventRecord(c->ev[8], c->s));
   for (i64 itn = 0; itn < max_iter; ++itn) {
-    std::vector<double> band;
-    const bool mixed = c->opt.precision == 1 && !c->mixed_done;
-    if (mixed) c->tm[12] += 1;
-    int f = solve_device(c, c->rho, c->pot, band, timed);
+    bool stop = false;
+    int f = scf_step(c, traj ? traj + 12 * itn : nullptr, timed, false, &stop);
     if (f < 0) return f;
     fails += f;
-    if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
-    Occupation occ;
-    CK(window_step(c, band, occ));
-    if (timed) {
-      FORGE_HIP(hipEventRecord(c->ev[11], c->s));
-      FORGE_HIP(hipEventSynchronize(c->ev[11]));
-      float a;
-      hipEventElapsedTime(&a, c->ev[10], c->ev[11]);
-      c->tm[4] += a;
-      FORGE_HIP(hipEventRecord(c->ev[12], c->s));
-    }
-    CK(site_density(n, c->rho_new, c->db, c->s));
-    double pc[13];
-    CK(energy_pieces(c, c->rho, c->rho_new, pc));
-    double ein[4] = {c->e[0], c->e[1], c->e[2], c->e[3]};
-    const double EtotIn = c->Etot;
-    double step;
-    double e1[4], eh[4];
-    energies_at(pc, 1.0, e1);
-    energies_at(pc, 0.5, eh);
-    if (c->opt.mix_mode == 0) {
-      // OptimalStep, HartreeFock.f90:1162-1208
-      double c0 = EtotIn, E1 = total_energy(c->P, e1), Eh = total_energy(c->P, eh);
-      double c1 = 4.0 * Eh - E1 - 3.0 * c0, c2 = E1 - c0 - c1;
-      if (c2 > 0 && std::fabs(-c1 / 2.0 / c2 - 0.5) < 0.5) step = -c1 / 2.0 / c2;
-      else if (E1 < c0) step = 1.0;
-      else step = c->P.step_alt;
-    } else {
-      step = c->opt.beta;
-    }
-    energies_at(pc, step, c->e);
-    c->Etot = total_energy(c->P, c->e);
-    // DensityConvergence = max |D_mixed - D_in| (Main.f90:564-571)
-    FORGE_HIP(hipMemcpyAsync(hda.data(), c->da, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
-    FORGE_HIP(hipMemcpyAsync(hdb.data(), c->db, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
-    CK(mix(nnc * nn, c->rho, c->rho_new, step, c->s));
-    hipLaunchKernelGGL(k_potential, dim3(cdiv(n, 256)), dim3(256), 0, c->s, n, c->ya, c->yb, c->da,
-                       c->db, step, c->P.alphaH, c->P.U, c->pot, c->da);
-    FORGE_LAUNCH();
-    FORGE_HIP(hipStreamSynchronize(c->s));
-    double dconv = 0;
-    for (i64 q = 0; q < n; ++q) {
-      double dmix = (1.0 - step) * hda[q] + step * hdb[q];
-      dconv = std::max(dconv, std::fabs(dmix - hda[q]));
-    }
-    if (timed) {
-      FORGE_HIP(hipEventRecord(c->ev[13], c->s));
-      FORGE_HIP(hipEventSynchronize(c->ev[13]));
-      float a;
-      hipEventElapsedTime(&a, c->ev[12], c->ev[13]);
-      c->tm[5] += a;
-    }
-    ++c->it;
     ++done;
-    if (traj) {
-      double* r = traj + 12 * itn;
-      r[0] = (double)c->it; r[1] = EtotIn; r[2] = 2 * ein[0]; r[3] = 2 * ein[1]; r[4] = ein[2];
-      r[5] = ein[3]; r[6] = dconv; r[7] = step; r[8] = EtotIn - c->Etot; r[9] = occ.EF;
-      r[10] = (double)occ.nOcc; r[11] = (double)occ.nPart;
-    }
-    const double dE = std::fabs(EtotIn - c->Etot);
-    if (mixed) {
-      // precision 1: FP64 from here on once the energy change reaches the
-      // switch threshold (default 1e-4 eV); convergence is only declared on
-      // FP64 iterations
-      const double sw = c->opt.switch_de > 0 ? c->opt.switch_de : 1e-4;
-      if (dE < sw || (c->opt.etol > 0 && dE < c->opt.etol)) c->mixed_done = true;
-      continue;
-    }
-    if (c->opt.etol > 0 && dE < c->opt.etol) break;
+    if (stop) break;
   }
   FORGE_HIP(hipEventRecord(c->ev[9], c->s));
   FORGE_HIP(hipEventSynchronize(c->ev[9]));
@@ -1053,4 +1070,205 @@ extern "C" int forge_scf_timings(forge_ctx* c, double* out16) {
   if (!c || !out16) { set_error("forge_scf_timings: null argument"); return -2; }
   for (int q = 0; q < 16; ++q) out16[q] = c->tm[q];
   return 0;
+}
+
+// ---------------------------------------------------------------- twist-angle sweeps
+// Several systems of different orders (e.g. TBG cells m = 8..14, each with its
+// own filling and k grid) advance their SCF iterations together.  Per
+// iteration all their H(k) go into ONE eigensolver batch: every matrix gets an
+// Nmax x Nmax slot (Nmax the largest order) and is assembled into the slot's
+// trailing n_i x n_i corner; the slots are ordered by decreasing n_i, so the
+// blocked Householder reduction runs one set of launches per column over the
+// prefix of matrices that have started (first_col = Nmax - n_i), all with the
+// same trailing order, and spends no work on the padding.  The tridiagonal
+// eigensolver and back-transformation then run per system (batched over its
+// k-points) on the compacted d, e, tau, and each system finishes its
+// iteration (densities, Fermi level, energies, mixing) on its own buffers.
+// A system that has converged leaves the batch.
+struct forge_sweep {
+  std::vector<forge_ctx*> sys;
+  i64 Nmax = 0, ldH = 0;
+  int Ktot = 0;
+  z2* H = nullptr;
+  void* trd_ws = nullptr;
+  TridiagWork trd{};
+  hipStream_t s = nullptr;
+  std::vector<i64> first_col;
+  std::vector<std::pair<int, int>> assigned;  // (system, first slot) of the last iteration
+};
+
+// zero columns [0, c1) of K consecutive slots (all Nmax rows)
+__global__ void k_sweep_clear_cols(i64 nrow, i64 c1, z2* H, i64 ld, size_t sH) {
+  z2* h = H + blockIdx.z * sH;
+  for (i64 c = blockIdx.y; c < c1; c += gridDim.y)
+    for (i64 r = (i64)blockIdx.x * 256 + threadIdx.x; r < nrow; r += (i64)gridDim.x * 256)
+      h[r + ld * c] = mk(0.0, 0.0);
+}
+
+// d, e, tau of system-local columns 0..ni-1 (sweep columns s0..) of K slots
+__global__ void k_sweep_gather(i64 Nmax, i64 s0, i64 ni, const double* __restrict__ d,
+                               const double* __restrict__ e, const z2* __restrict__ tau,
+                               double* __restrict__ di, double* __restrict__ ei,
+                               z2* __restrict__ ti) {
+  const int k = blockIdx.y;
+  for (i64 t = (i64)blockIdx.x * 256 + threadIdx.x; t < ni; t += (i64)gridDim.x * 256) {
+    const size_t src = (size_t)k * Nmax + s0 + t, dst = (size_t)k * ni + t;
+    di[dst] = d[src];
+    ei[dst] = t < ni - 1 ? e[src] : 0.0;
+    ti[dst] = t < ni - 1 ? tau[src] : mk(0.0, 0.0);
+  }
+}
+
+extern "C" int forge_sweep_destroy(forge_sweep* w);
+
+extern "C" int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sweep** out) {
+  if (!ctxs || nsys < 1 || !out) { set_error("forge_sweep_create: invalid argument"); return -2; }
+  *out = nullptr;
+  auto* w = new forge_sweep();
+  for (i64 i = 0; i < nsys; ++i) {
+    forge_ctx* c = ctxs[i];
+    if (!c) { delete w; set_error("forge_sweep_create: null context"); return -2; }
+    if (c->Kb < c->Nk) {
+      delete w;
+      set_error("forge_sweep_create: every system must solve all its k-points in one batch");
+      return -4;
+    }
+    w->sys.push_back(c);
+    w->Nmax = std::max<i64>(w->Nmax, c->n);
+    w->Ktot += (int)c->Nk;
+  }
+  w->ldH = pad_ld(w->Nmax);
+  const size_t sH = (size_t)w->ldH * w->Nmax;
+  FORGE_HIP(hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking));
+  if (dalloc((void**)&w->H, sizeof(z2) * sH * w->Ktot) ||
+      dalloc(&w->trd_ws, tridiag_workspace_bytes(w->Nmax, w->Ktot))) {
+    forge_sweep_destroy(w);
+    return -4;
+  }
+  FORGE_HIP(hipMemsetAsync(w->H, 0, sizeof(z2) * sH * w->Ktot, w->s));
+  CK(tridiag_bind(w->trd, w->trd_ws, w->Nmax, w->Ktot));
+  w->trd.A = w->H;
+  w->trd.sA = sH;
+  w->trd.lda = w->ldH;
+  FORGE_HIP(hipStreamSynchronize(w->s));
+  *out = w;
+  return 0;
+}
+
+extern "C" int forge_sweep_destroy(forge_sweep* w) {
+  if (!w) return 0;
+  if (w->s) hipStreamSynchronize(w->s);
+  if (w->H) hipFree(w->H);
+  if (w->trd_ws) hipFree(w->trd_ws);
+  if (w->s) hipStreamDestroy(w->s);
+  delete w;
+  return 0;
+}
+
+// the batched eigensolve of one sweep iteration over the systems in `act`
+static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
+  const size_t sH = (size_t)w->ldH * w->Nmax;
+  std::vector<std::pair<int, int>> assign;
+  w->first_col.clear();
+  int slot = 0;
+  for (int i : act) {
+    forge_ctx* c = w->sys[i];
+    assign.push_back({i, slot});
+    for (i64 k = 0; k < c->Nk; ++k) w->first_col.push_back(w->Nmax - c->n);
+    slot += (int)c->Nk;
+  }
+  // a slot's columns left of its matrix must be zero; they change only when
+  // the assignment of systems to slots does
+  if (assign != w->assigned) {
+    for (auto& a : assign) {
+      forge_ctx* c = w->sys[a.first];
+      const i64 s0 = w->Nmax - c->n;
+      if (s0 == 0) continue;
+      hipLaunchKernelGGL(k_sweep_clear_cols, dim3((unsigned)cdiv(w->Nmax, 256),
+                                                  (unsigned)std::min<i64>(s0, 1024), (unsigned)c->Nk),
+                         dim3(256), 0, w->s, w->Nmax, s0, w->H + a.second * sH, w->ldH, sH);
+      FORGE_LAUNCH();
+    }
+    w->assigned = assign;
+  }
+  for (auto& a : assign) {
+    forge_ctx* c = w->sys[a.first];
+    const i64 s0 = w->Nmax - c->n;
+    CK(assemble(c->n, (int)c->nnc, (int)c->Nk, c->rho, c->V, c->T, c->pot, c->bias, c->P.alpha,
+                c->ph_asm, w->H + a.second * sH + s0 + w->ldH * s0, w->ldH, sH, w->s));
+  }
+  w->trd.K = slot;
+  w->trd.first_col = w->first_col.data();
+  w->trd.S = nullptr;
+  CK(tridiag(w->trd, w->s, nullptr, nullptr, nullptr));
+  int fails = 0;
+  for (auto& a : assign) {
+    forge_ctx* c = w->sys[a.first];
+    const i64 n = c->n, s0 = w->Nmax - n;
+    const int K = (int)c->Nk;
+    const size_t q0 = (size_t)a.second;
+    hipLaunchKernelGGL(k_sweep_gather, dim3((unsigned)cdiv(n, 256), K), dim3(256), 0, w->s, w->Nmax,
+                       s0, n, w->trd.d + q0 * w->Nmax, w->trd.e + q0 * w->Nmax,
+                       w->trd.tau + q0 * w->Nmax, c->trd.d, c->trd.e, c->trd.tau);
+    FORGE_LAUNCH();
+    int f = dc_solve(c->trd.d, c->trd.e, n, K, c->nU, c->evals, c->Zq, c->dc_ws, w->s);
+    if (f < 0) return f;
+    fails += f;
+    // reflectors of this system: its slots' trailing corners, compacted tau
+    TridiagWork v = c->trd;
+    v.n = n;
+    v.K = K;
+    v.A = w->H + q0 * sH + s0 + w->ldH * s0;
+    v.sA = sH;
+    v.lda = w->ldH;
+    CK(backtransform(v, c->Zq, n, (size_t)n * c->nU, c->nU, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
+                     w->s));
+  }
+  return fails;
+}
+
+extern "C" int forge_sweep_run(forge_sweep* w, int64_t max_iter, double* traj, int64_t* iters_done) {
+  if (!w || max_iter < 0) { set_error("forge_sweep_run: invalid argument"); return -2; }
+  const int ns = (int)w->sys.size();
+  for (forge_ctx* c : w->sys) {
+    if (!c->scf_ready) { set_error("forge_sweep_run: call forge_scf_init on every system"); return -5; }
+    if (c->opt.precision != 0) { set_error("forge_sweep_run: sweeps run in FP64 (precision 0)"); return -2; }
+    FORGE_HIP(hipStreamSynchronize(c->s));
+  }
+  // every system's work goes to the sweep's stream for the run
+  std::vector<hipStream_t> own(ns);
+  for (int i = 0; i < ns; ++i) {
+    own[i] = w->sys[i]->s;
+    w->sys[i]->s = w->s;
+    std::fill(w->sys[i]->tm, w->sys[i]->tm + 16, 0.0);
+  }
+  std::vector<i64> done(ns, 0);
+  std::vector<bool> stopped(ns, false);
+  int fails = 0, rc = 0;
+  for (i64 itn = 0; itn < max_iter && rc >= 0; ++itn) {
+    std::vector<int> act;
+    for (int i = 0; i < ns; ++i)
+      if (!stopped[i]) act.push_back(i);
+    if (act.empty()) break;
+    std::stable_sort(act.begin(), act.end(),
+                     [&](int a, int b) { return w->sys[a]->n > w->sys[b]->n; });
+    int f = sweep_solve(w, act);
+    if (f < 0) { rc = f; break; }
+    fails += f;
+    for (int i : act) {
+      bool stop = false;
+      forge_ctx* c = w->sys[i];
+      f = scf_step(c, traj ? traj + ((size_t)i * max_iter + itn) * 12 : nullptr, false, true, &stop);
+      if (f < 0) { rc = f; break; }
+      fails += f;
+      ++done[i];
+      if (stop) stopped[i] = true;
+    }
+  }
+  for (int i = 0; i < ns; ++i) {
+    w->sys[i]->s = own[i];
+    if (iters_done) iters_done[i] = done[i];
+  }
+  FORGE_HIP(hipStreamSynchronize(w->s));
+  return rc < 0 ? rc : fails;
 }

@@ -52,6 +52,12 @@ struct TridiagWork {
   size_t sS;      // streams instead of A; k_trd_her2k keeps it current
   hipEvent_t* evs;  // event pairs of the sampled hemv launches (created lazily)
   i64 nev;
+  // Uneven sizes (null: all K matrices are n x n).  Matrix k holds an
+  // n_k x n_k matrix in the trailing corner of its n x n slot (first_col[k] =
+  // n - n_k, non-decreasing in k, the rest of the slot's columns zero); it is
+  // reduced from column first_col[k] on, so at column j the matrices with
+  // first_col <= j are a prefix of the batch and share the trailing order.
+  const i64* first_col;  // host array of K entries
 };
 constexpr int TRD_NB = 32;  // panel width
 constexpr int TRD_TS = 32;     // hemv column-strip width

@@ -104,6 +104,7 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   w.sS = 0;
   w.evs = nullptr;
   w.nev = 0;
+  w.first_col = nullptr;
   return 0;
 }
 
@@ -614,6 +615,11 @@ __global__ void __launch_bounds__(256) k_trd_to_fp32(i64 n, const z2* A_, size_t
   }
 }
 
+__global__ void k_trd_clear(size_t count, z2* __restrict__ p) {
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < count; q += (size_t)gridDim.x * 256)
+    p[q] = mk(0.0, 0.0);
+}
+
 __global__ void k_trd_lastdiag(i64 n, const z2* A_, size_t sA, i64 lda, double* d_) {
   int k = blockIdx.x;
   if (threadIdx.x == 0) d_[(size_t)k * n + n - 1] = A_[k * sA + (n - 1) + lda * (n - 1)].x;
@@ -669,6 +675,24 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   }
   i64 nrec = 0;
   double bytes = 0.0, tms = 0.0;
+  // uneven sizes: a matrix joins at its first column with W, tau and the w^H v
+  // partials of the columns before it zero, so those columns act as identity
+  // reflectors in the panel's column updates, panel products and her2k (their
+  // Householder vectors lie in the slot's zero columns)
+  int Kj = K;
+  if (w.first_col) {
+    for (int k = 1; k < K; ++k)
+      if (w.first_col[k] < w.first_col[k - 1] || w.first_col[k] < 0 || w.first_col[k] >= n) {
+        set_error("tridiag: first_col must be non-decreasing and below n");
+        return -2;
+      }
+    const size_t cW = (size_t)K * n * NB, cT = (size_t)K * n, cD = (size_t)K * nbmax;
+    hipLaunchKernelGGL(k_trd_clear, dim3(1024), dim3(256), 0, s, cW, w.W);
+    hipLaunchKernelGGL(k_trd_clear, dim3(256), dim3(256), 0, s, cT, w.tau);
+    hipLaunchKernelGGL(k_trd_clear, dim3(64), dim3(256), 0, s, cD, w.dpart);
+    FORGE_LAUNCH();
+    Kj = 0;
+  }
   if (w.S) {
     hipLaunchKernelGGL(k_trd_to_fp32, dim3((unsigned)cdiv(n, 256), (unsigned)std::min<i64>(n, 512), K),
                        dim3(256), 0, s, n, w.A, w.sA, w.lda, w.S, w.sS);
@@ -678,11 +702,14 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     const int nbp = (int)((n - 1 - i0) < NB ? (n - 1 - i0) : NB);
     for (int jj = 0; jj < nbp; ++jj) {
       const i64 j = i0 + jj;
-      hipLaunchKernelGGL(k_trd_col, dim3(cdiv(n - j, 256), K), dim3(256), 0, s, n, j, i0, jj, w.A,
+      if (w.first_col)
+        while (Kj < K && w.first_col[Kj] <= j) ++Kj;
+      if (Kj == 0) continue;
+      hipLaunchKernelGGL(k_trd_col, dim3(cdiv(n - j, 256), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
       const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
-      const dim3 hgrid((unsigned)(nt + TRD_PB), (unsigned)ms, K);
+      const dim3 hgrid((unsigned)(nt + TRD_PB), (unsigned)ms, Kj);
       const bool tj = timed && (j % TIME_STRIDE == 0);
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tj) {
@@ -692,7 +719,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
         e1 = w.evs[2 * q + 1];
         ++nrec;
         // algorithmic bytes: the lower triangle of the m x m trailing matrix
-        bytes += (w.S ? 4.0 : 8.0) * (double)m * (double)m * K;
+        bytes += (w.S ? 4.0 : 8.0) * (double)m * (double)m * Kj;
       }
       if (w.S)
         launch_hemv<float2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const float2*)w.S, w.sS, ntmax,
@@ -701,13 +728,14 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
         launch_hemv<z2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const z2*)w.A, w.sA, ntmax, msmax,
                         nbmax);
       FORGE_LAUNCH();
-      hipLaunchKernelGGL(k_trd_fin, dim3((unsigned)nfin_of(m), K), dim3(256), 0, s, n, j, i0, jj, w.A,
+      hipLaunchKernelGGL(k_trd_fin, dim3((unsigned)nfin_of(m), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
                          msmax, nbmax);
       FORGE_LAUNCH();
     }
     const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
-    hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), K), dim3(256), 0, s,
+    if (Kj == 0) continue;
+    hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), Kj), dim3(256), 0, s,
                        n, p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
     FORGE_LAUNCH();
     // bound the launches in flight (about 400): the host otherwise runs up to

@@ -203,6 +203,44 @@ class Engine:
         return out
 
 
+class Sweep:
+    """SCF of several systems of different orders advanced together, their
+    eigensolves batched into one uneven-size reduction per iteration
+    (forge_sweep_*).  The engines stay owned by the caller; set their options
+    (FP64) and initial states before run()."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        self.lib = L.load()
+        arr = (C.c_void_p * len(self.engines))(*[e.ctx.value for e in self.engines])
+        sw = C.c_void_p()
+        L.check(self.lib.forge_sweep_create(arr, len(self.engines), C.byref(sw)),
+                "forge_sweep_create")
+        self.sw = sw
+
+    def run(self, max_iter):
+        """-> list of per-system trajectories (rows as Engine.scf_run)."""
+        ns = len(self.engines)
+        traj = np.zeros((ns, max(max_iter, 1), 12))
+        done = np.zeros(ns, np.int64)
+        rc = L.check(self.lib.forge_sweep_run(self.sw, int(max_iter), _p(traj), _p(done)),
+                     "forge_sweep_run")
+        if rc > 0:
+            print("zheev failed", rc)
+        return [traj[i, :done[i]] for i in range(ns)]
+
+    def close(self):
+        if self.sw:
+            self.lib.forge_sweep_destroy(self.sw)
+            self.sw = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def zdiagred(A, il, iu):
     """zdiagRed (LapackRoutines.f90:381-433) on the GPU: eigenpairs il..iu of the
     Hermitian matrix whose lower triangle A holds -> (eigenvalues, vectors, info)."""

@@ -55,6 +55,9 @@ _SIGS = {
     "forge_scf_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),
     "forge_scf_get_density": (C.c_int, [vp, vp]),
     "forge_scf_timings": (C.c_int, [vp, vp]),
+    "forge_sweep_create": (C.c_int, [vp, i64, C.POINTER(vp)]),
+    "forge_sweep_destroy": (C.c_int, [vp]),
+    "forge_sweep_run": (C.c_int, [vp, i64, vp, vp]),
     "forge_mlp_param_count": (i64, [i64, i64]),
     "forge_mlp_loss_grad": (C.c_int, [vp, i64, i64, vp, vp, vp, i64, vp, vp]),
     "forge_mlp_train": (C.c_int, [vp, i64, i64, vp, vp, vp, i64, i64, i64, dbl, C.c_uint64, vp]),

@@ -147,6 +147,25 @@ int forge_scf_run(forge_ctx* ctx, int64_t max_iter, double* traj, int64_t* iters
 /* Copies the current density matrix (ndim x ndim x nnc) to the host. */
 int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
 
+/* ---- Twist-angle sweeps (no reference counterpart: the reference runs one
+ * system per process, Main.f90:34-685) ----
+ * A sweep advances the SCF of nsys systems of different orders together (e.g.
+ * TBG cells m = 8..14, each with its own filling): per iteration all their
+ * H(k) are reduced to tridiagonal form in ONE batch (uneven sizes; see
+ * TridiagWork::first_col), then every system finishes its iteration as
+ * forge_scf_run would.  The contexts stay owned by the caller; each needs
+ * forge_scf_set_options (precision 0) and forge_scf_init before
+ * forge_sweep_run, and must hold all its k-points in one eigensolver batch. */
+typedef struct forge_sweep forge_sweep;
+int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sweep** out);
+int forge_sweep_destroy(forge_sweep* sweep);
+
+/* Up to max_iter iterations of every system; a system stops once it
+ * converges (its opt.etol) and leaves the batch.  traj (nsys x max_iter x 12,
+ * may be NULL) gets forge_scf_run's rows per system, iters_done (nsys) the
+ * iterations each system ran. */
+int forge_sweep_run(forge_sweep* sweep, int64_t max_iter, double* traj, int64_t* iters_done);
+
 /* Per-stage device time of the last forge_scf_run (milliseconds, HIP events):
  * [0] assembly, [1] tridiagonal reduction, [2] tridiagonal eigensolver,
  * [3] back-transformation, [4] density products, [5] energies + mixing,

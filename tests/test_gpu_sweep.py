@@ -1,0 +1,79 @@
+"""Twist-angle sweeps: systems of different orders whose eigensolves share one
+uneven-size batch per SCF iteration (forge_sweep_*) against the same systems
+solved one at a time (forge_scf_run), from the same seeded starts, FP64.
+
+The batched reduction runs each matrix from its own first column inside a
+larger slot, so its panels start at other columns than in the single solve
+and the rounding differs at the 1e-15 level; the tolerance (1e-8 eV on every
+iteration's energy) allows for that growing along the soft modes of the SCF
+map over the trajectory.  The systems converge after different numbers of
+iterations, so the batch shrinks during the run.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from forge_amd.hf import Engine, Setup, Sweep
+from forge_amd.seed import seeded_rho
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [  # (ntheta, numk, nfilling): n = 76, 148, 244
+    (2, 3, 1.0),
+    (4, 3, -1.0),
+    (3, 3, 2.0),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _engines():
+    return [Engine(Setup(ntheta=t, numk=k, nfilling=f, xi_nm=40.0)) for t, k, f in SYSTEMS]
+
+
+def _start(e, nfill):
+    e.set_options(occ_mode=2, mix_mode=0, kT=0.001, etol=1e-8)
+    e.scf_init(seeded_rho(e.n, e.nnc, nfill))
+
+
+def test_sweep_matches_single_systems():
+    say("sweep vs single systems", SYSTEMS)
+    engs = _engines()
+    single = []
+    for e, (_, _, f) in zip(engs, SYSTEMS):
+        _start(e, f)
+        single.append(e.scf_run(80))
+        say(f"  single n={e.n}: {len(single[-1])} iterations, E {single[-1][-1, 1]:.9f}")
+    for e, (_, _, f) in zip(engs, SYSTEMS):
+        _start(e, f)
+    sw = Sweep(engs)
+    batched = sw.run(80)
+    sw.close()
+    for e, a, b in zip(engs, single, batched):
+        say(f"  sweep  n={e.n}: {len(b)} iterations, E {b[-1, 1]:.9f}")
+        assert abs(len(a) - len(b)) <= 1
+        m = min(len(a), len(b))
+        assert m >= 3
+        np.testing.assert_allclose(b[:m, 1], a[:m, 1], rtol=0, atol=1e-8)
+        np.testing.assert_allclose(b[:m, 9], a[:m, 9], rtol=0, atol=1e-8)  # Fermi level
+        np.testing.assert_array_equal(b[:m, 10:12], a[:m, 10:12])  # nOcc, nPart
+        assert abs(b[-1, 8]) < 1e-8
+    for e in engs:
+        e.close()
+
+
+def test_sweep_rejects_mixed_precision():
+    engs = _engines()[:2]
+    for e, (_, _, f) in zip(engs, SYSTEMS):
+        _start(e, f)
+    engs[1].set_options(occ_mode=2, mix_mode=0, kT=0.001, precision=1)
+    sw = Sweep(engs)
+    with pytest.raises(Exception, match="FP64"):
+        sw.run(2)
+    sw.close()
+    for e in engs:
+        e.close()
