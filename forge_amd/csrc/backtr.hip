@@ -55,32 +55,44 @@ __global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, siz
   }
 }
 
-// T (nb x nb, upper) from G = V^H V and tau, zlarft forward/columnwise:
-//   T(i,i) = tau_i,  T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).
-// The recurrence runs in FP64 on the stored (FP64 or FP32) entries.
+// T (nb x nb, upper) from G = V^H V and tau (zlarft 'F','C').  T^-1 = striu(G)
+// + diag(1/tau), so column i of T follows from the rows below it:
+//   T(i,i) = tau_i,   T(r,i) = -tau_r sum_{c=r+1..i} G(r,c) T(c,i)   (r < i),
+// which also holds for tau = 0 (an identity reflector: its row and column are
+// zero).  One wave per column (grid nb x K), the column in LDS, G(r, c) read
+// as conj(G(c, r)) (contiguous in c) one row ahead of its use.
 template <typename E>
-__global__ void k_bt_tbuild(int nb, i64 i0, i64 n, const E* __restrict__ G, const z2* __restrict__ tau,
-                            E* __restrict__ T) {
-  int k = blockIdx.x;
+__global__ void __launch_bounds__(64) k_bt_tbuild(int nb, i64 i0, i64 n, const E* __restrict__ G,
+                                                  const z2* __restrict__ tau, E* __restrict__ T) {
+  const int i = blockIdx.x, k = blockIdx.y, l = threadIdx.x;
   const E* g = G + (size_t)k * BT * BT;
   E* t = T + (size_t)k * BT * BT;
   const z2* tk = tau + (size_t)k * n + i0;
-  __shared__ z2 col[BT];
-  for (int q = threadIdx.x; q < BT * BT; q += blockDim.x) t[q] = narrow<E>(mk(0.0, 0.0));
+  __shared__ z2 x[BT];
+  for (int r = i + 1 + l; r < nb; r += 64) t[r + BT * i] = narrow<E>(mk(0.0, 0.0));
+  if (l == 0) x[i] = tk[i];
   __syncthreads();
-  for (int i = 0; i < nb; ++i) {
-    // col(0:i) = -tau_i G(0:i, i)
-    for (int r = threadIdx.x; r < i; r += blockDim.x) col[r] = (-1.0) * (tk[i] * widen(g[r + BT * i]));
-    __syncthreads();
-    // T(0:i, i) = T(0:i,0:i) col
-    for (int r = threadIdx.x; r < i; r += blockDim.x) {
-      z2 s = mk(0.0, 0.0);
-      for (int c = r; c < i; ++c) s = s + widen(t[r + BT * c]) * col[c];
-      t[r + BT * i] = narrow<E>(s);
+  // row r needs G(r, c) for c = r+1+l and r+65+l (c <= i)
+  auto load = [&](int r, int c) { return (r >= 0 && c <= i) ? conjz(widen(g[c + BT * r])) : mk(0.0, 0.0); };
+  z2 g0 = load(i - 1, i + l), g1 = load(i - 1, i + 64 + l);
+  for (int r = i - 1; r >= 0; --r) {
+    const z2 n0 = load(r - 1, r + l), n1 = load(r - 1, r + 64 + l);
+    const int c0 = r + 1 + l, c1 = c0 + 64;
+    z2 p = mk(0.0, 0.0);
+    if (c0 <= i) p = g0 * x[c0];
+    if (c1 <= i) p = p + g1 * x[c1];
+    double sx = p.x, sy = p.y;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sx += __shfl_xor(sx, off, 64);
+      sy += __shfl_xor(sy, off, 64);
     }
-    if (threadIdx.x == 0) t[i + BT * i] = narrow<E>(tk[i]);
+    if (l == 0) x[r] = (-1.0) * (tk[r] * mk(sx, sy));
     __syncthreads();
+    g0 = n0;
+    g1 = n1;
   }
+  for (int r = l; r <= i; r += 64) t[r + BT * i] = narrow<E>(x[r]);
 }
 
 // C(:, j) = C32(:, j) / |C32(:, j)|_2 in FP64.  grid (m, K), 256 threads.
@@ -171,7 +183,7 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
     FORGE_LAUNCH();
     // G = V^H V
     RB(gemm_sb(h, CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K));
-    hipLaunchKernelGGL(k_bt_tbuild<E>, dim3(K), dim3(128), 0, s, nb, i0, n, G, t.tau, T);
+    hipLaunchKernelGGL(k_bt_tbuild<E>, dim3(nb, K), dim3(64), 0, s, nb, i0, n, G, t.tau, T);
     FORGE_LAUNCH();
     // Y = V^H C(i0+1:n, :)
     RB(gemm_sb(h, CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K));

@@ -331,6 +331,18 @@ __global__ void __launch_bounds__(256) k_dc_prep(i64 n, const MergeDesc* __restr
 }
 
 // ---------------------------------------------------------------- merge: secular roots
+// Root i of f(t) = 1 + sum_j rho z_j^2 / (delta_j - t), delta_j = d_j - d_org,
+// in t = lambda - d_org, org the pole nearer to the root (chosen from the
+// sign of f at the interval's midpoint).  f increases from -inf to +inf
+// between the poles d_i and d_{i+1} (above d_{nk-1} for the last root).
+// Iteration: the two-pole rational model of LAPACK dlaed4's fixed-weight
+// scheme (the poles left of the root summed into psi, the rest into phi;
+// each part replaced by c + s / (pole - t) matching its value and slope at
+// the iterate), whose root is taken when it falls inside the bracket [a, b]
+// kept from the signs of f; otherwise (and after 20 steps) a bisection step.
+// Stops when |f| is below its rounding bound 8 eps (1 + sum_j |term_j|), the
+// step is below 4 eps |t| or the bracket has collapsed: about 5 passes over
+// the poles on random and clustered spectra where bisection needs about 55.
 __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __restrict__ md,
                                                     DCPtr P) {
   const int k = blockIdx.z;
@@ -342,8 +354,9 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
   const double* dl = P.dl + vo;
   const double* wz = P.wz + vo;
   const double rho = P.mrho[vo];
+  const bool last = i == nk - 1;
   double lo = dl[i], hi;
-  if (i < nk - 1) {
+  if (!last) {
     hi = dl[i + 1];
   } else {
     double s = 0.0;
@@ -353,7 +366,7 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
   double half = 0.5 * (hi - lo);
   int org;
   double a, b;
-  bool left = (i == nk - 1);
+  bool left = last;
   if (!left) {
     double f = 1.0;
     for (int j = 0; j < nk; ++j) f += rho * wz[j] * wz[j] / ((dl[j] - lo) - half);
@@ -362,21 +375,62 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
   if (left) {
     org = i;
     a = 0.0;
-    b = (i == nk - 1) ? (hi - lo) : half;
+    b = last ? (hi - lo) : half;
   } else {
     org = i + 1;
     a = -half;
     b = 0.0;
   }
   const double dorg = dl[org];
-  for (int it = 0; it < 200; ++it) {
-    double t = 0.5 * (a + b);
-    if (t == a || t == b) break;
-    double f = 1.0;
-    for (int j = 0; j < nk; ++j) f += rho * wz[j] * wz[j] / ((dl[j] - dorg) - t);
-    if (f > 0.0) b = t; else a = t;
-  }
+  // the poles bracketing the root, in t
+  const double pL = dl[i] - dorg, pR = last ? 0.0 : dl[i + 1] - dorg;
+  constexpr double eps = 1.1102230246251565e-16;
   double t = 0.5 * (a + b);
+  for (int it = 0; it < 120; ++it) {
+    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, absum = 0.0;
+    for (int j = 0; j < nk; ++j) {
+      const double del = (dl[j] - dorg) - t;
+      const double term = rho * wz[j] * wz[j] / del;
+      const double der = term / del;
+      if (j <= i) { psi += term; dpsi += der; }
+      else { phi += term; dphi += der; }
+      absum += fabs(term);
+    }
+    const double f = 1.0 + psi + phi;
+    if (fabs(f) <= 8.0 * eps * (1.0 + absum)) break;
+    if (f > 0.0) b = t; else a = t;
+    const double mid = 0.5 * (a + b);
+    if (mid == a || mid == b) break;
+    double tn = mid;
+    if (it < 20) {
+      const double D1 = pL - t;
+      double eta = 0.0;
+      bool ok = false;
+      if (last) {
+        const double c = f - dpsi * D1;
+        if (c > 0.0) { eta = D1 + dpsi * D1 * D1 / c; ok = true; }
+      } else {
+        const double D2 = pR - t;
+        const double c = f - dpsi * D1 - dphi * D2;
+        const double A = c * (D1 + D2) + dpsi * D1 * D1 + dphi * D2 * D2;
+        const double Bc = D1 * D2 * f;
+        if (c == 0.0) {
+          if (A != 0.0) { eta = Bc / A; ok = true; }
+        } else {
+          const double disc = sqrt(fabs(A * A - 4.0 * Bc * c));
+          if (A <= 0.0) eta = (A - disc) / (2.0 * c);
+          else eta = 2.0 * Bc / (A + disc);
+          ok = true;
+        }
+      }
+      const double tr = t + eta;
+      if (ok && tr > a && tr < b) {
+        tn = tr;
+        if (fabs(eta) <= 4.0 * eps * fabs(t)) { t = tn; break; }
+      }
+    }
+    t = tn;
+  }
   if (t == 0.0) t = (b != 0.0) ? b : a;
   P.org[vo + i] = org;
   P.taus[vo + i] = t;
