@@ -354,6 +354,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
                   c->S};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  tridiag_release(c->trd);
   for (i64 q = 0; q < c->trd.nev; ++q) hipEventDestroy(c->trd.evs[q]);
   delete[] c->trd.evs;
   for (int q = 0; q < 16; ++q) hipEventDestroy(c->ev[q]);
@@ -432,6 +433,7 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   c->trd.K = K;
   const bool mixed = mixed_phase(c);
   c->trd.S = mixed ? c->S : nullptr;
+  c->trd.use_graph = c->opt.graph;
   c->trd.sS = (size_t)c->ldH * n;
   double hemv_ms = 0, hemv_bytes = 0;
   long hemv_n = 0;
@@ -1158,6 +1160,7 @@ extern "C" int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sw
 extern "C" int forge_sweep_destroy(forge_sweep* w) {
   if (!w) return 0;
   if (w->s) hipStreamSynchronize(w->s);
+  tridiag_release(w->trd);
   if (w->H) hipFree(w->H);
   if (w->trd_ws) hipFree(w->trd_ws);
   if (w->s) hipStreamDestroy(w->s);
@@ -1200,6 +1203,8 @@ static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
   w->trd.K = slot;
   w->trd.first_col = w->first_col.data();
   w->trd.S = nullptr;
+  w->trd.use_graph = 0;
+  for (int i : act) w->trd.use_graph |= w->sys[i]->opt.graph;
   CK(tridiag(w->trd, w->s, nullptr, nullptr, nullptr));
   int fails = 0;
   for (auto& a : assign) {

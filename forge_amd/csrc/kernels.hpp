@@ -58,6 +58,11 @@ struct TridiagWork {
   // reduced from column first_col[k] on, so at column j the matrices with
   // first_col <= j are a prefix of the batch and share the trailing order.
   const i64* first_col;  // host array of K entries
+  // nonzero: tridiag() captures its launch sequence into a HIP graph on the
+  // first call for a shape (n, K, first_col, FP32 copy or not, buffers) and
+  // replays it while the shape is unchanged (untimed calls only)
+  int use_graph;
+  void* graph;  // the captured graph and its shape; tridiag_release frees it
 };
 constexpr int TRD_NB = 32;  // panel width
 constexpr int TRD_TS = 32;     // hemv column-strip width
@@ -67,6 +72,7 @@ size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes);
+void tridiag_release(TridiagWork& w);
 
 // dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
 struct DCWork;

@@ -35,6 +35,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -105,6 +106,8 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   w.evs = nullptr;
   w.nev = 0;
   w.first_col = nullptr;
+  w.use_graph = 0;
+  w.graph = nullptr;
   return 0;
 }
 
@@ -662,8 +665,10 @@ static void launch_hemv(dim3 g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, i64
                        w.scal, ntmax, msmax, nbmax);
 }
 
-int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
-            double* hemv_bytes) {
+// The launch sequence of one reduction.  capturing: inside a stream capture
+// (no host synchronisation).
+static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
+                             double* hemv_bytes, bool capturing) {
   const i64 n = w.n;
   const int K = w.K;
   const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = npart_stride(n);
@@ -742,7 +747,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     // 3 n launches ahead, which rocprofv3's dispatch tracing does not survive
     // (host fault in its record buffers beyond ~15000 pending dispatches);
     // the GPU idles only for the next launch's latency, ~100 times per call
-    if ((i0 / NB) % 4 == 3) FORGE_HIP(hipStreamSynchronize(s));
+    if (!capturing && (i0 / NB) % 4 == 3) FORGE_HIP(hipStreamSynchronize(s));
   }
   hipLaunchKernelGGL(k_trd_lastdiag, dim3(K), dim3(64), 0, s, n, w.A, w.sA, w.lda, w.d);
   FORGE_LAUNCH();
@@ -753,6 +758,69 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     *hemv_launches += nrec;
     *hemv_bytes += bytes;
   }
+  return 0;
+}
+
+// A captured reduction and the shape it was captured for: the kernel
+// arguments are fixed by n, K, the active prefix of every column
+// (first_col), the FP32 copy and the buffer addresses.
+struct TrdGraph {
+  hipGraphExec_t exec = nullptr;
+  i64 n = 0, lda = 0;
+  int K = 0;
+  const void *A = nullptr, *S = nullptr, *W = nullptr;
+  size_t sA = 0, sS = 0;
+  std::vector<i64> first;
+  bool same(const TridiagWork& w) const {
+    if (n != w.n || K != w.K || lda != w.lda || A != w.A || S != w.S || W != w.W || sA != w.sA ||
+        sS != w.sS || first.size() != (w.first_col ? (size_t)w.K : 0))
+      return false;
+    for (size_t q = 0; q < first.size(); ++q)
+      if (first[q] != w.first_col[q]) return false;
+    return true;
+  }
+};
+
+void tridiag_release(TridiagWork& w) {
+  auto* g = (TrdGraph*)w.graph;
+  if (!g) return;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  delete g;
+  w.graph = nullptr;
+}
+
+int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
+            double* hemv_bytes) {
+  if (!w.use_graph || hemv_ms) return enqueue_reduction(w, s, hemv_ms, hemv_launches, hemv_bytes, false);
+  auto* g = (TrdGraph*)w.graph;
+  if (!g || !g->same(w)) {
+    tridiag_release(w);
+    hipGraph_t graph = nullptr;
+    FORGE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_reduction(w, s, nullptr, nullptr, nullptr, true);
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (rc != 0) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ce != hipSuccess) {
+      set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      return -1;
+    }
+    g = new TrdGraph();
+    const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      delete g;
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      return -1;
+    }
+    g->n = w.n; g->K = w.K; g->lda = w.lda; g->A = w.A; g->S = w.S; g->W = w.W;
+    g->sA = w.sA; g->sS = w.sS;
+    if (w.first_col) g->first.assign(w.first_col, w.first_col + w.K);
+    w.graph = g;
+  }
+  FORGE_HIP(hipGraphLaunch(g->exec, s));
   return 0;
 }
 

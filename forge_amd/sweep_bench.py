@@ -5,7 +5,7 @@ uneven-size eigensolver batch of 63 matrices per iteration, forge_sweep_run)
 against the same iterations run system by system (forge_scf_run), and prints
 one JSON line.
 
-    python -m forge_amd.sweep_bench [--iters 3] [--warmup 1] [--mmin 8] [--mmax 14]
+    python -m forge_amd.sweep_bench [--iters 3] [--warmup 1] [--mmin 8] [--mmax 14] [--graph]
 """
 from __future__ import annotations
 
@@ -28,6 +28,8 @@ def main():
     ap.add_argument("--mmax", type=int, default=14)
     ap.add_argument("--numk", type=int, default=3)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the batched iterations with the reduction replayed from a HIP graph")
     a = ap.parse_args()
     rng = np.random.default_rng(a.seed)
     ms = list(range(a.mmin, a.mmax + 1))
@@ -35,9 +37,9 @@ def main():
     engs = [Engine(Setup(ntheta=m, numk=a.numk, nfilling=nu, xi_nm=40.0)) for m, nu in zip(ms, nus)]
     starts = [seeded_rho(e.n, e.nnc, nu) for e, nu in zip(engs, nus)]
 
-    def reset():
+    def reset(graph=0):
         for e, s in zip(engs, starts):
-            e.set_options(occ_mode=2, mix_mode=0, kT=0.001)
+            e.set_options(occ_mode=2, mix_mode=0, kT=0.001, graph=graph)
             e.scf_init(s)
 
     # system by system
@@ -57,6 +59,18 @@ def main():
     t_bat = (time.perf_counter() - t0) / a.iters
     sw.close()
     print(f"[sweep_bench] batched:    {t_bat * 1e3:.1f} ms per iteration of all systems", flush=True)
+    t_gr = None
+    if a.graph:
+        reset(graph=1)
+        sw = Sweep(engs)
+        sw.run(a.warmup)  # captures the reduction
+        t0 = time.perf_counter()
+        gr = sw.run(a.iters)
+        t_gr = (time.perf_counter() - t0) / a.iters
+        sw.close()
+        assert all(np.array_equal(x, y) for x, y in zip(gr, bat))
+        print(f"[sweep_bench] batched, reduction from a HIP graph: {t_gr * 1e3:.1f} ms per iteration",
+              flush=True)
     dE = max(float(np.abs(x[:, 1] - y[:, 1]).max()) for x, y in zip(seq, bat))
     out = {
         "metric": "SCF iteration of a twist-angle sweep (all systems)",
@@ -67,6 +81,8 @@ def main():
         "eigensolves_per_iteration": int(sum(e.Nk for e in engs)),
         "max_energy_difference_eV": dE, "dtype": "f64",
     }
+    if t_gr is not None:
+        out["graph_ms"] = round(t_gr * 1e3, 2)
     for e in engs:
         e.close()
     print(json.dumps(out), flush=True)

@@ -45,10 +45,15 @@ typedef struct {
   double beta;        /* linear mixing weight, mix_mode 1 */
   double etol;        /* > 0: forge_scf_run stops once |E_in - E_new| < etol (eV per cell) */
   int32_t precision;  /* 0: FP64 everywhere; 1: mixed -- the Householder hemv streams an FP32
-                         copy of the trailing matrix (FP64 accumulation, every other step
+                         copy of the trailing matrix (FP64 accumulation) and the
+                         back-transformation runs in complex FP32 (columns normalised in
                          FP64) until |E_in - E_new| < switch_de, then FP64 to convergence */
   int32_t timing;     /* nonzero: per-stage HIP-event timings (forge_scf_timings) */
   double switch_de;   /* eV, precision 1 (0: 1e-4) */
+  int32_t graph;      /* nonzero: the Householder reduction's launches (3 per column and one
+                         trailing update per panel) are captured into a HIP graph on first
+                         use and replayed while the batch shape is unchanged; not with timing */
+  int32_t reserved;   /* 0 */
 } forge_scf_options;
 
 typedef struct forge_ctx forge_ctx;
