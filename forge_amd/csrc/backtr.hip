@@ -13,17 +13,31 @@
 // output.  Eigenvector errors then enter the band energies only at second
 // order: for psi = psi* + d, <psi|H|psi> / <psi|psi> - eps = O(|d|^2), while
 // an unnormalised column would carry the first-order term 2 eps Re<psi*|d>.
+//
+// Block width autotuning: the width trades passes over C (n / BT reads and
+// writes of the n x m eigenvectors) against the triangular work of G and T
+// (growing with BT) and the efficiency of the matrix-core GEMMs at inner
+// dimension BT.  On the first call for a shape (n, m, K, FP64/FP32) every
+// candidate width applies the first block (the largest trailing order) to
+// the promoted eigenvectors twice; the second run is timed with HIP events
+// and the width with the least time per reflector is kept for the shape for
+// the rest of the process.
 #include <rocblas/rocblas.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "kernels.hpp"
 
 namespace forge {
 
-constexpr int BT = 128;
+constexpr int BT_MAX = 256;
+constexpr int BT_CAND[3] = {64, 128, 256};
 
 size_t backtr_workspace_bytes(i64 n, int K, i64 m) {
   // V (n x BT), G (BT x BT), T (BT x BT), Y (BT x m), Y2 (BT x m) per matrix
-  size_t per = (size_t)n * BT + 2 * BT * BT + 2 * (size_t)BT * m;
+  size_t per = (size_t)n * BT_MAX + 2 * BT_MAX * BT_MAX + 2 * (size_t)BT_MAX * m;
   return sizeof(z2) * per * K + 4096;
 }
 
@@ -61,9 +75,10 @@ __global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, siz
 // which also holds for tau = 0 (an identity reflector: its row and column are
 // zero).  One wave per column (grid nb x K), the column in LDS, G(r, c) read
 // as conj(G(c, r)) (contiguous in c) one row ahead of its use.
-template <typename E>
+template <typename E, int BT>
 __global__ void __launch_bounds__(64) k_bt_tbuild(int nb, i64 i0, i64 n, const E* __restrict__ G,
                                                   const z2* __restrict__ tau, E* __restrict__ T) {
+  constexpr int NL = BT / 64;  // entries of a row per lane
   const int i = blockIdx.x, k = blockIdx.y, l = threadIdx.x;
   const E* g = G + (size_t)k * BT * BT;
   E* t = T + (size_t)k * BT * BT;
@@ -72,15 +87,21 @@ __global__ void __launch_bounds__(64) k_bt_tbuild(int nb, i64 i0, i64 n, const E
   for (int r = i + 1 + l; r < nb; r += 64) t[r + BT * i] = narrow<E>(mk(0.0, 0.0));
   if (l == 0) x[i] = tk[i];
   __syncthreads();
-  // row r needs G(r, c) for c = r+1+l and r+65+l (c <= i)
+  // row r needs G(r, c) for c = r+1+l, r+65+l, ... (c <= i)
   auto load = [&](int r, int c) { return (r >= 0 && c <= i) ? conjz(widen(g[c + BT * r])) : mk(0.0, 0.0); };
-  z2 g0 = load(i - 1, i + l), g1 = load(i - 1, i + 64 + l);
+  z2 gc[NL];
+#pragma unroll
+  for (int q = 0; q < NL; ++q) gc[q] = load(i - 1, i + 64 * q + l);
   for (int r = i - 1; r >= 0; --r) {
-    const z2 n0 = load(r - 1, r + l), n1 = load(r - 1, r + 64 + l);
-    const int c0 = r + 1 + l, c1 = c0 + 64;
+    z2 gn[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) gn[q] = load(r - 1, r + 64 * q + l);
     z2 p = mk(0.0, 0.0);
-    if (c0 <= i) p = g0 * x[c0];
-    if (c1 <= i) p = p + g1 * x[c1];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int c = r + 1 + 64 * q + l;
+      if (c <= i) p = p + gc[q] * x[c];
+    }
     double sx = p.x, sy = p.y;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -89,8 +110,8 @@ __global__ void __launch_bounds__(64) k_bt_tbuild(int nb, i64 i0, i64 n, const E
     }
     if (l == 0) x[r] = (-1.0) * (tk[r] * mk(sx, sy));
     __syncthreads();
-    g0 = n0;
-    g1 = n1;
+#pragma unroll
+    for (int q = 0; q < NL; ++q) gc[q] = gn[q];
   }
   for (int r = l; r <= i; r += 64) t[r + BT * i] = narrow<E>(x[r]);
 }
@@ -130,6 +151,12 @@ __global__ void __launch_bounds__(256) k_bt_normalise(i64 n, const float2* __res
 
 rocblas_handle blas_handle(hipStream_t s);
 
+#define CK_BT(x)              \
+  do {                        \
+    int _r = (x);             \
+    if (_r != 0) return _r;   \
+  } while (0)
+
 // rocBLAS complex gemm for each storage type
 static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, i64 M,
                               i64 N, i64 Kd, double alpha, const z2* A, i64 lda, size_t sA,
@@ -154,10 +181,13 @@ static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_op
       (rocblas_float_complex*)C, (rocblas_int)ldc, (rocblas_stride)sC, batch);
 }
 
-// C (ldC, stride sC per matrix) <- Q Z, blocks last to first, in storage type E.
-template <typename E>
+// C (ldC, stride sC per matrix) <- Q Z, blocks of BT reflectors last to first,
+// in storage type E.  first_only: C <- (first block) Z, for timing, with the
+// event `promoted` recorded between the promotion of Z and the block.
+template <typename E, int BT>
 static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, E* C,
-                   size_t ldC, size_t sC, void* work, hipStream_t s) {
+                   size_t ldC, size_t sC, void* work, hipStream_t s, bool first_only,
+                   hipEvent_t promoted = nullptr) {
   const i64 n = t.n;
   const int K = t.K;
   E* V = (E*)work;
@@ -169,12 +199,13 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
   hipLaunchKernelGGL(k_bt_promote<E>, dim3(2048, K), dim3(256), 0, s, (size_t)n * m, Z, ldZ, sZ, n,
                      m, C, ldC, sC);
   FORGE_LAUNCH();
+  if (promoted) FORGE_HIP(hipEventRecord(promoted, s));
   if (n < 2) return 0;
   rocblas_handle h = blas_handle(s);
   if (!h) return -1;
   const auto CT = rocblas_operation_conjugate_transpose, NO = rocblas_operation_none;
   const i64 nblocks = (n - 1 + BT - 1) / BT;
-  for (i64 bq = nblocks - 1; bq >= 0; --bq) {
+  for (i64 bq = first_only ? 0 : nblocks - 1; bq >= 0; --bq) {
     const i64 i0 = bq * BT;
     const int nb = (int)((n - 1 - i0) < BT ? (n - 1 - i0) : BT);
     const i64 mt = n - i0 - 1;
@@ -183,7 +214,7 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
     FORGE_LAUNCH();
     // G = V^H V
     RB(gemm_sb(h, CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K));
-    hipLaunchKernelGGL(k_bt_tbuild<E>, dim3(nb, K), dim3(64), 0, s, nb, i0, n, G, t.tau, T);
+    hipLaunchKernelGGL((k_bt_tbuild<E, BT>), dim3(nb, K), dim3(64), 0, s, nb, i0, n, G, t.tau, T);
     FORGE_LAUNCH();
     // Y = V^H C(i0+1:n, :)
     RB(gemm_sb(h, CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K));
@@ -195,11 +226,74 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
   return 0;
 }
 
+template <typename E>
+static int apply_q_bt(int bt, const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m,
+                      E* C, size_t ldC, size_t sC, void* work, hipStream_t s, bool first_only,
+                      hipEvent_t promoted = nullptr) {
+  switch (bt) {
+    case 64: return apply_q<E, 64>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s, first_only, promoted);
+    case 128: return apply_q<E, 128>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s, first_only, promoted);
+    case 256: return apply_q<E, 256>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s, first_only, promoted);
+  }
+  set_error("backtransform: unsupported block width");
+  return -2;
+}
+
+static std::mutex g_bt_mu;
+static std::map<std::tuple<int, i64, i64, int, int>, int> g_bt_width;  // (device, n, m, K, fp32)
+
+// the tuned block width of a shape (measured on first use)
+template <typename E>
+static int tuned_width(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, E* C,
+                       size_t ldC, size_t sC, void* work, hipStream_t s, int* width) {
+  int dev = 0;
+  FORGE_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, t.n, m, t.K, (int)(sizeof(E) == sizeof(float2)));
+  {
+    std::lock_guard<std::mutex> g(g_bt_mu);
+    auto it = g_bt_width.find(key);
+    if (it != g_bt_width.end()) { *width = it->second; return 0; }
+  }
+  int best = 128;
+  if (t.n > 64 && m > 0) {
+    hipEvent_t e0, e1;
+    FORGE_HIP(hipEventCreate(&e0));
+    FORGE_HIP(hipEventCreate(&e1));
+    double best_per = 0.0;
+    for (int bt : BT_CAND) {
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {
+        int r = apply_q_bt<E>(bt, t, Z, ldZ, sZ, m, C, ldC, sC, work, s, true, e0);
+        if (r) return r;
+        FORGE_HIP(hipEventRecord(e1, s));
+        FORGE_HIP(hipEventSynchronize(e1));
+        FORGE_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      const double nb = (double)std::min<i64>(bt, t.n - 1);
+      const double per = ms / nb;
+      if (bt == BT_CAND[0] || per < best_per) { best_per = per; best = bt; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  std::lock_guard<std::mutex> g(g_bt_mu);
+  g_bt_width[key] = best;
+  *width = best;
+  return 0;
+}
+
 int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
-                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32) {
-  if (!C32) return apply_q<z2>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s);
+                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32, int* width_used) {
+  int bt = 128;
+  if (!C32) {
+    CK_BT(tuned_width<z2>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s, &bt));
+    if (width_used) *width_used = bt;
+    return apply_q_bt<z2>(bt, t, Z, ldZ, sZ, m, C, ldC, sC, work, s, false);
+  }
   const size_t ld32 = (size_t)t.n, s32 = (size_t)t.n * m;
-  int r = apply_q<float2>(t, Z, ldZ, sZ, m, C32, ld32, s32, work, s);
+  CK_BT(tuned_width<float2>(t, Z, ldZ, sZ, m, C32, ld32, s32, work, s, &bt));
+  if (width_used) *width_used = bt;
+  int r = apply_q_bt<float2>(bt, t, Z, ldZ, sZ, m, C32, ld32, s32, work, s, false);
   if (r) return r;
   if (m > 0) {
     hipLaunchKernelGGL(k_bt_normalise, dim3((unsigned)m, t.K), dim3(256), 0, s, t.n, C32, ld32, s32,

@@ -446,8 +446,10 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   if (timed) FORGE_HIP(hipEventRecord(c->ev[4], c->s));
   // mixed phase: the FP32 copy is free once the reduction is done and holds
   // the FP32 eigenvectors (n x mout x K <= ldH x n x K)
+  int width = 0;
   CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
-                   c->s, mixed ? c->S : nullptr));
+                   c->s, mixed ? c->S : nullptr, &width));
+  c->tm[13] = width;
   if (timed) {
     FORGE_HIP(hipEventRecord(c->ev[5], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[5]));

@@ -82,10 +82,12 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
 
 // backtr.hip: C(n x m, complex) = Q_house * Z(n x m, real), Q from tridiag.
 // C32 non-null: the products run in complex FP32 in C32 (n x m per matrix,
-// ld n), and C receives the columns normalised in FP64.
+// ld n), and C receives the columns normalised in FP64.  The block width of
+// the compact WY products is tuned per shape on first use (*width_used).
 size_t backtr_workspace_bytes(i64 n, int K, i64 m);
 int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
-                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32 = nullptr);
+                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32 = nullptr,
+                  int* width_used = nullptr);
 
 // density.hip
 int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,

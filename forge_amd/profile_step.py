@@ -47,7 +47,8 @@ def main():
             print("[profile_step]   stage ms: " + " ".join(
                 f"{name} {t[q]:.1f}" for q, name in enumerate(
                     ("assembly", "reduction", "tridiag_eig", "back_transform", "density",
-                     "energies_mixing"))), flush=True)
+                     "energies_mixing"))) + f" (back-transform block width {t[13]:.0f})",
+                  flush=True)
     e.close()
     return 0
 

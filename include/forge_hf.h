@@ -178,7 +178,8 @@ int forge_sweep_run(forge_sweep* sweep, int64_t max_iter, double* traj, int64_t*
  * assembly kernel (ms), [9] timed launches of the Householder hemv kernel
  * (every 17th column), [10] their summed time (ms), [11] their algorithmic
  * bytes (8 m^2 per matrix in FP64, 4 m^2 from the FP32 copy; m the trailing
- * order), [12] iterations run with precision 1 still in its mixed phase. */
+ * order), [12] iterations run with precision 1 still in its mixed phase,
+ * [13] block width of the last back-transformation (tuned per shape). */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
 
 /* ---- MLP of the learned initial guess (no reference counterpart) ----
