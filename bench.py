@@ -145,8 +145,9 @@ def roofline(stage, kernel, bytes_per_elem, workload):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the "
-              "k-points over GPUs is described in DESIGN.md, not measured here", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-points "
+              "would shard over GPUs (one rank per GPU, an all-reduce of rho per iteration), "
+              "which is not built or measured here", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
