@@ -14,6 +14,7 @@ module forge_c
   public :: forge_tb_hamiltonian, forge_hamiltonian_hf, forge_zdiagred, forge_solve
   public :: forge_getfock, forge_energies, forge_scf_set_options, forge_scf_init
   public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
+  public :: forge_sweep_create, forge_sweep_run, forge_sweep_destroy
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -149,6 +150,30 @@ module forge_c
       integer(c_int64_t), intent(out) :: iters_done
       integer(c_int) :: r
     end function forge_scf_run
+
+    ! twist-angle sweeps: contexts advanced together, one uneven-size batch
+    function forge_sweep_create(ctxs, nsys, sweep) bind(C, name="forge_sweep_create") result(r)
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int64_t), value :: nsys
+      type(c_ptr), intent(out) :: sweep
+      integer(c_int) :: r
+    end function forge_sweep_create
+
+    function forge_sweep_run(sweep, max_iter, traj, iters_done) bind(C, name="forge_sweep_run") result(r)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: sweep
+      integer(c_int64_t), value :: max_iter
+      real(c_double), intent(out) :: traj(*)
+      integer(c_int64_t), intent(out) :: iters_done(*)
+      integer(c_int) :: r
+    end function forge_sweep_run
+
+    function forge_sweep_destroy(sweep) bind(C, name="forge_sweep_destroy") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sweep
+      integer(c_int) :: r
+    end function forge_sweep_destroy
 
     function forge_scf_get_density(ctx, rho) bind(C, name="forge_scf_get_density") result(r)
       import :: c_int, c_ptr, c_double_complex
