@@ -64,7 +64,7 @@ struct TridiagWork {
   int use_graph;
   void* graph;  // the captured graph and its shape; tridiag_release frees it
 };
-constexpr int TRD_NB = 32;  // panel width
+constexpr int TRD_NB = 32;  // panel width (a multiple of 8, at most 128; 64 measured slower)
 constexpr int TRD_TS = 32;     // hemv column-strip width
 constexpr int TRD_SEGR = 1024; // hemv rows per block (more blocks for small trailing orders)
 constexpr int TRD_PB = 4;      // hemv panel-product blocks per row segment

@@ -283,11 +283,12 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
       scal_[k * 4 + 2] = mk(R.beta, 0.0);
     }
     const z2* W = W_ + (size_t)k * n * NB;
-    const int qb = wv * 8;  // 8 panel columns per wave, lanes over rows
+    constexpr int QW = NB / 4;  // panel columns per wave, lanes over rows
+    const int qb = wv * QW;
     // two panel columns per pass keep this branch's registers below the
     // strip branch's, which sets the kernel's occupancy
     z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
-    for (int q0 = 0; q0 < 8; q0 += 2) {
+    for (int q0 = 0; q0 < QW; q0 += 2) {
       z2 t1[2] = {mk(0.0, 0.0), mk(0.0, 0.0)}, t2[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
       if (qb + q0 < jj) {
         for (i64 rr = rs + lane; rr < re; rr += 64) {
@@ -545,17 +546,34 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
 #pragma unroll
     for (int v = 0; v < 2; ++v) re[u][v] = im[u][v] = f64x4{0.0, 0.0, 0.0, 0.0};
   const z2 zero = mk(0.0, 0.0);
+  // staging of 8 panel columns: this thread's two elements e = tid, tid + 256
+  // (panel column e / 64, tile row / column e % 64), loaded into registers
+  // one step ahead so the loads overlap the previous step's MFMAs
+  const int rr = threadIdx.x & 63, qq0 = threadIdx.x >> 6;  // e = tid: column qq0, e = tid + 256: qq0 + 4
+  const bool rin = r0 + rr < n, cin = c0 + rr < n;
+  const z2* Ar = A + (r0 + rr) + lda * i0;
+  const z2* Ac = A + (c0 + rr) + lda * i0;
+  const z2* Wr = W + (r0 + rr);
+  const z2* Wc = W + (c0 + rr);
+  z2 sv[2][4];
+#define HER2K_LOAD_STEP(Q0)                                                   \
+  _Pragma("unroll") for (int h = 0; h < 2; ++h) {                             \
+    const int q = (Q0) + qq0 + 4 * h;                                         \
+    const bool qv = q < nbp;                                                  \
+    sv[h][0] = (qv && rin) ? Ar[lda * q] : zero;                              \
+    sv[h][1] = (qv && cin) ? Ac[lda * q] : zero;                              \
+    sv[h][2] = (qv && rin) ? Wr[n * q] : zero;                                \
+    sv[h][3] = (qv && cin) ? Wc[n * q] : zero;                                \
+  }
+  HER2K_LOAD_STEP(0)
   for (int q0 = 0; q0 < nbp; q0 += 8) {
     __syncthreads();
-    for (int e = threadIdx.x; e < 8 * 64; e += 256) {
-      const int qq = e >> 6, rr = e & 63, q = q0 + qq;
-      const i64 r = r0 + rr, c = c0 + rr;
-      const bool qv = q < nbp, last = q == nbp - 1;
-      const z2 vr = (qv && r < n) ? A[r + lda * (i0 + q)] : zero;
-      const z2 vc = (qv && c < n) ? A[c + lda * (i0 + q)] : zero;
-      z2 wr = (qv && r < n) ? W[r + n * q] : zero;
-      z2 wc = (qv && c < n) ? W[c + n * q] : zero;
-      if (last) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qq = qq0 + 4 * h, q = q0 + qq;
+      const z2 vr = sv[h][0], vc = sv[h][1];
+      z2 wr = sv[h][2], wc = sv[h][3];
+      if (q == nbp - 1) {
         wr = wr + alpha * vr;
         wc = wc + alpha * vc;
       }
@@ -563,6 +581,7 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
       Xc[0][qq][rr] = vc.x; Xc[1][qq][rr] = vc.y; Xc[2][qq][rr] = wc.x; Xc[3][qq][rr] = wc.y;
     }
     __syncthreads();
+    if (q0 + 8 < nbp) { HER2K_LOAD_STEP(q0 + 8) }
     // component pairs: X comp -> (Yre comp, sign), (Yim comp, sign)
     //   Vr: (Wr, +) (Wi, -);  Vi: (Wi, +) (Wr, +);  Wr: (Vr, +) (Vi, -);  Wi: (Vi, +) (Vr, +)
 #pragma unroll
