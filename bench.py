@@ -15,8 +15,8 @@ mixed-precision + learned-guess path against the plain FP64 path.
                   the same path with linear mixing 0.3 is reported beside it
 
 Default mode (a few minutes): a step is one SCF iteration.  W warm-up and K
-timed plain-FP64 iterations from the seeded start, one timed mixed-phase
-iteration and one timed learned start; the value is the projected time to
+timed plain-FP64 iterations from the seeded start, W warm-up and one timed
+mixed-phase iteration, and one timed learned start; the value is the projected time to
 convergence of the mixed+learned path = t(learned start) + N_mixed t(mixed
 iteration) + N_fp64 t(FP64 iteration), with the iteration counts of complete
 recorded solves of the same workload (MIXED_LEARNED_ITERS, PLAIN_ITERS; logs
@@ -231,8 +231,10 @@ def main():
         t0 = time.perf_counter()
         traj, stage = iterate("plain timed", args.steps, False)
         t64 = (time.perf_counter() - t0) / max(len(traj), 1)
-        # one mixed-phase iteration (far from convergence: the switch is not reached)
+        # one mixed-phase iteration (far from convergence: the switch is not
+        # reached) after W untimed ones (the first tunes the FP32 back-transform)
         eng.set_options(**base, precision=1, switch_de=args.switch_de, etol=0.0)
+        iterate("mixed warmup", args.warmup, False)
         t0 = time.perf_counter()
         _, tmmx = iterate("mixed timed", 1, False)
         tmx = time.perf_counter() - t0
