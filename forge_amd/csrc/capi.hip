@@ -1099,6 +1099,7 @@ struct forge_sweep {
   hipStream_t s = nullptr;
   std::vector<i64> first_col;
   std::vector<std::pair<int, int>> assigned;  // (system, first slot) of the last iteration
+  float2* S = nullptr;  // FP32 copy of the slots (allocated when a system asks for precision 1)
 };
 
 // zero columns [0, c1) of K consecutive slots (all Nmax rows)
@@ -1164,6 +1165,7 @@ extern "C" int forge_sweep_destroy(forge_sweep* w) {
   if (w->s) hipStreamSynchronize(w->s);
   tridiag_release(w->trd);
   if (w->H) hipFree(w->H);
+  if (w->S) hipFree(w->S);
   if (w->trd_ws) hipFree(w->trd_ws);
   if (w->s) hipStreamDestroy(w->s);
   delete w;
@@ -1202,9 +1204,14 @@ static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
     CK(assemble(c->n, (int)c->nnc, (int)c->Nk, c->rho, c->V, c->T, c->pot, c->bias, c->P.alpha,
                 c->ph_asm, w->H + a.second * sH + s0 + w->ldH * s0, w->ldH, sH, w->s));
   }
+  // the shared reduction streams the FP32 copy while every system in the
+  // batch is in its mixed phase; each system's back-transform follows its own
+  bool mixed_all = w->S != nullptr;
+  for (int i : act) mixed_all = mixed_all && mixed_phase(w->sys[i]);
   w->trd.K = slot;
   w->trd.first_col = w->first_col.data();
-  w->trd.S = nullptr;
+  w->trd.S = mixed_all ? w->S : nullptr;
+  w->trd.sS = sH;
   w->trd.use_graph = 0;
   for (int i : act) w->trd.use_graph |= w->sys[i]->opt.graph;
   CK(tridiag(w->trd, w->s, nullptr, nullptr, nullptr));
@@ -1229,7 +1236,7 @@ static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
     v.sA = sH;
     v.lda = w->ldH;
     CK(backtransform(v, c->Zq, n, (size_t)n * c->nU, c->nU, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
-                     w->s));
+                     w->s, mixed_phase(c) ? c->S : nullptr));
   }
   return fails;
 }
@@ -1237,10 +1244,21 @@ static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
 extern "C" int forge_sweep_run(forge_sweep* w, int64_t max_iter, double* traj, int64_t* iters_done) {
   if (!w || max_iter < 0) { set_error("forge_sweep_run: invalid argument"); return -2; }
   const int ns = (int)w->sys.size();
+  bool any_mixed = false;
   for (forge_ctx* c : w->sys) {
     if (!c->scf_ready) { set_error("forge_sweep_run: call forge_scf_init on every system"); return -5; }
-    if (c->opt.precision != 0) { set_error("forge_sweep_run: sweeps run in FP64 (precision 0)"); return -2; }
+    any_mixed = any_mixed || c->opt.precision == 1;
     FORGE_HIP(hipStreamSynchronize(c->s));
+  }
+  if (any_mixed && !w->S) {
+    const size_t bytes = sizeof(float2) * (size_t)w->ldH * w->Nmax * w->Ktot;
+    size_t fr = 0, tot = 0;
+    FORGE_HIP(hipMemGetInfo(&fr, &tot));
+    if (bytes + ((size_t)256 << 20) > fr) {
+      set_error("forge_sweep_run: not enough device memory for the FP32 copy of precision 1");
+      return -4;
+    }
+    CK(dalloc((void**)&w->S, bytes));
   }
   // every system's work goes to the sweep's stream for the run
   std::vector<hipStream_t> own(ns);

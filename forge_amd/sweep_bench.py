@@ -5,7 +5,7 @@ uneven-size eigensolver batch of 63 matrices per iteration, forge_sweep_run)
 against the same iterations run system by system (forge_scf_run), and prints
 one JSON line.
 
-    python -m forge_amd.sweep_bench [--iters 3] [--warmup 1] [--mmin 8] [--mmax 14] [--graph]
+    python -m forge_amd.sweep_bench [--iters 3] [--warmup 1] [--mmin 8] [--mmax 14] [--precision 0|1] [--graph]
 """
 from __future__ import annotations
 
@@ -28,6 +28,8 @@ def main():
     ap.add_argument("--mmax", type=int, default=14)
     ap.add_argument("--numk", type=int, default=3)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--precision", type=int, default=0,
+                    help="1: mixed precision (FP32-copy hemv, FP32 back-transform) for every system")
     ap.add_argument("--graph", action="store_true",
                     help="also time the batched iterations with the reduction replayed from a HIP graph")
     a = ap.parse_args()
@@ -39,7 +41,8 @@ def main():
 
     def reset(graph=0):
         for e, s in zip(engs, starts):
-            e.set_options(occ_mode=2, mix_mode=0, kT=0.001, graph=graph)
+            e.set_options(occ_mode=2, mix_mode=0, kT=0.001, graph=graph, precision=a.precision,
+                          switch_de=1e-12)
             e.scf_init(s)
 
     # system by system
@@ -79,7 +82,8 @@ def main():
         "systems": [{"m": m, "n": e.n, "nu": round(nu, 4), "k_points": e.Nk}
                     for m, e, nu in zip(ms, engs, nus)],
         "eigensolves_per_iteration": int(sum(e.Nk for e in engs)),
-        "max_energy_difference_eV": dE, "dtype": "f64",
+        "max_energy_difference_eV": dE,
+        "dtype": "f64" if a.precision == 0 else "f64 (FP32-copy hemv, FP32 back-transform)",
     }
     if t_gr is not None:
         out["graph_ms"] = round(t_gr * 1e3, 2)

@@ -159,8 +159,11 @@ int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
  * H(k) are reduced to tridiagonal form in ONE batch (uneven sizes; see
  * TridiagWork::first_col), then every system finishes its iteration as
  * forge_scf_run would.  The contexts stay owned by the caller; each needs
- * forge_scf_set_options (precision 0) and forge_scf_init before
- * forge_sweep_run, and must hold all its k-points in one eigensolver batch. */
+ * forge_scf_set_options and forge_scf_init before forge_sweep_run, and must
+ * hold all its k-points in one eigensolver batch.  With precision 1 the shared
+ * reduction streams an FP32 copy while every system in the batch is in its
+ * mixed phase; each system's back-transformation and switch to FP64 follow
+ * its own options. */
 typedef struct forge_sweep forge_sweep;
 int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sweep** out);
 int forge_sweep_destroy(forge_sweep* sweep);
