@@ -343,14 +343,24 @@ __global__ void __launch_bounds__(256) k_dc_prep(i64 n, const MergeDesc* __restr
 // Stops when |f| is below its rounding bound 8 eps (1 + sum_j |term_j|), the
 // step is below 4 eps |t| or the bracket has collapsed: about 5 passes over
 // the poles on random and clustered spectra where bisection needs about 55.
+// One wave per root (4 roots per 256-thread block): the lanes split every
+// sum over the poles, a butterfly reduces it and lane 0's value is
+// broadcast, so all lanes take the same branches.
+__device__ inline double wave_total(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
 __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __restrict__ md,
                                                     DCPtr P) {
   const int k = blockIdx.z;
   const MergeDesc M = md[blockIdx.y];
   const size_t vo = (size_t)k * n + M.o;
   const int nk = P.mk[vo];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nk) return;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= nk) return;  // wave-uniform
   const double* dl = P.dl + vo;
   const double* wz = P.wz + vo;
   const double rho = P.mrho[vo];
@@ -360,17 +370,17 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
     hi = dl[i + 1];
   } else {
     double s = 0.0;
-    for (int j = 0; j < nk; ++j) s += wz[j] * wz[j];
-    hi = dl[nk - 1] + rho * s;
+    for (int j = lane; j < nk; j += 64) s += wz[j] * wz[j];
+    hi = dl[nk - 1] + rho * wave_total(s);
   }
   double half = 0.5 * (hi - lo);
   int org;
   double a, b;
   bool left = last;
   if (!left) {
-    double f = 1.0;
-    for (int j = 0; j < nk; ++j) f += rho * wz[j] * wz[j] / ((dl[j] - lo) - half);
-    left = f > 0.0;
+    double f = 0.0;
+    for (int j = lane; j < nk; j += 64) f += rho * wz[j] * wz[j] / ((dl[j] - lo) - half);
+    left = 1.0 + wave_total(f) > 0.0;
   }
   if (left) {
     org = i;
@@ -388,7 +398,7 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
   double t = 0.5 * (a + b);
   for (int it = 0; it < 120; ++it) {
     double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, absum = 0.0;
-    for (int j = 0; j < nk; ++j) {
+    for (int j = lane; j < nk; j += 64) {
       const double del = (dl[j] - dorg) - t;
       const double term = rho * wz[j] * wz[j] / del;
       const double der = term / del;
@@ -396,6 +406,9 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
       else { phi += term; dphi += der; }
       absum += fabs(term);
     }
+    psi = wave_total(psi); dpsi = wave_total(dpsi);
+    phi = wave_total(phi); dphi = wave_total(dphi);
+    absum = wave_total(absum);
     const double f = 1.0 + psi + phi;
     if (fabs(f) <= 8.0 * eps * (1.0 + absum)) break;
     if (f > 0.0) b = t; else a = t;
@@ -432,8 +445,10 @@ __global__ void __launch_bounds__(256) k_dc_secular(i64 n, const MergeDesc* __re
     t = tn;
   }
   if (t == 0.0) t = (b != 0.0) ? b : a;
-  P.org[vo + i] = org;
-  P.taus[vo + i] = t;
+  if (lane == 0) {
+    P.org[vo + i] = org;
+    P.taus[vo + i] = t;
+  }
 }
 
 // zhat (Gu-Eisenstat) and then the column norms of the secular eigenvectors.
@@ -712,7 +727,7 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
     hipLaunchKernelGGL(k_dc_prep, dim3(nm, K), dim3(256), 0, s, n, md, e, P);
     FORGE_LAUNCH();
     dim3 gr(cdiv(lm, 256), nm, K);
-    hipLaunchKernelGGL(k_dc_secular, gr, dim3(256), 0, s, n, md, P);
+    hipLaunchKernelGGL(k_dc_secular, dim3(cdiv(lm, 4), nm, K), dim3(256), 0, s, n, md, P);
     FORGE_LAUNCH();
     hipLaunchKernelGGL(k_dc_zhat, gr, dim3(256), 0, s, n, md, P);
     FORGE_LAUNCH();
