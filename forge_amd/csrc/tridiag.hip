@@ -333,9 +333,19 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
   const int ncol = (int)((n - c0) < TS ? (n - c0) : TS);
   __shared__ z2 vJ[TS];
   __shared__ z2 red[4][TS];
+  // v of this block's rows (relative rows ra0 + q; zero outside [rs, re)),
+  // staged once so that the row steps read it from LDS instead of waiting
+  // on a load per step
+  __shared__ z2 vrow[SEGR + 32];
+  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
   if (threadIdx.x < TS) {
     const i64 c = c0 + threadIdx.x;
     vJ[threadIdx.x] = c >= n ? mk(0.0, 0.0) : (c == base ? mk(1.0, 0.0) : A[c + lda * j] * scale);
+  }
+  for (int q = threadIdx.x; q < SEGR + 32; q += 256) {
+    const i64 rr = ra0 + q;
+    vrow[q] = (rr < rs || rr >= re) ? mk(0.0, 0.0)
+                                    : (rr == 0 ? mk(1.0, 0.0) : A[(base + rr) + lda * j] * scale);
   }
   __syncthreads();
   const int rsub = lane & 15, cg = lane >> 4;  // row in the step, column group
@@ -350,9 +360,8 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
   // scalar base, which saves a VGPR pair per column address
   const TM* strip = M_ + k * sM + (size_t)lda * c0;
   const unsigned lda32 = (unsigned)lda, cofs = (unsigned)cb * lda32;
-  // row steps start at an absolute multiple of 16 rows, so with lda a multiple
-  // of 16 every 16-row column chunk is one aligned 256-byte segment
-  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+  // row steps start at an absolute multiple of 16 rows (ra0), so with lda a
+  // multiple of 16 every 16-row column chunk is one aligned 256-byte segment
   // The FP32 copy is read one step ahead (8-byte loads alone leave too few
   // bytes in flight per wave); the FP64 loads gain nothing from it.
   constexpr bool PF = sizeof(TM) == 8;
@@ -367,7 +376,7 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     const i64 rr = rr0 + rsub;  // row relative to base
     const bool valid = rr >= rs && rr < re;
     const i64 r = base + rr;
-    const z2 vr = !valid ? mk(0.0, 0.0) : (rr == 0 ? mk(1.0, 0.0) : A[r + lda * j] * scale);
+    const z2 vr = vrow[rr - ra0];  // zero unless valid
     const i64 dr = rr - TS * J;  // < 32 inside the diagonal tile
     z2 a[8];
     if (PF) {
