@@ -53,7 +53,7 @@ MIXED_LEARNED_ITERS = {"oda": {"mixed": 28, "fp64": 4}, "linear": {"mixed": 21, 
 # rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated on a copy kernel of known size),
 # measured on tbg_1.05deg_n11908_k3 for the FP64 matrix (16 B per element) and
 # the FP32 copy (8 B per element)
-HEMV_TRAFFIC_RATIO = {("tbg_1.05deg_n11908_k3", 16): 1.040, ("tbg_1.05deg_n11908_k3", 8): 1.089}
+HEMV_TRAFFIC_RATIO = {("tbg_1.05deg_n11908_k3", 16): 1.034, ("tbg_1.05deg_n11908_k3", 8): 1.087}
 MODEL = os.path.join(REPO, "forge_amd", "data", "guess_mlp.npz")
 
 WORKLOADS = {
