@@ -83,7 +83,7 @@ def parse():
                          "mixing); oda = the reference's OptimalStep")
     ap.add_argument("--kT", type=float, default=0.001)
     ap.add_argument("--model", default=MODEL)
-    ap.add_argument("--judge", type=int, default=0,
+    ap.add_argument("--fp64-check", type=int, default=0,
                     help="--full: N plain FP64 iterations from the mixed+learned solve's final "
                          "state; reports how far they move its energy")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -202,15 +202,16 @@ def main():
             t_init = timed_learned_start()
             traj, tm = iterate("mixed+learned", args.max_iter, True)
             results["mixed_learned"] = (time.perf_counter() - t0, traj, tm, t_init)
-            if args.judge > 0:
-                # the plain FP64 iteration as the judge of the converged state
+            if args.fp64_check > 0:
+                # the plain FP64 iteration as the check of the converged state
                 Efin = float(traj[-1, 1] - traj[-1, 8])
                 rho = eng.density()
                 eng.set_options(**base, precision=0, etol=0.0)
                 eng.scf_init(rho)
-                jt, _ = iterate("FP64 judge", args.judge, False)
-                judge = float(np.abs(jt[:, 1] - jt[:, 8] - Efin).max())
-                log(f"FP64 judge: {args.judge} plain iterations move the energy by at most {judge:.2e} eV")
+                jt, _ = iterate("FP64 check", args.fp64_check, False)
+                fp64_move = float(np.abs(jt[:, 1] - jt[:, 8] - Efin).max())
+                log(f"FP64 check: {args.fp64_check} plain iterations move the energy by at most "
+                    f"{fp64_move:.2e} eV")
         if args.path in ("both", "plain"):
             eng.set_options(**base, precision=0, etol=0.0)
             t0 = time.perf_counter()
@@ -231,9 +232,9 @@ def main():
             if k == "mixed_learned":
                 extra["mixed_learned_mixed_phase_iterations"] = int(tm[12])
                 extra["mixed_learned_start_s"] = round(ti, 2)
-                if args.judge > 0:
-                    extra["fp64_judge_iterations"] = args.judge
-                    extra["fp64_judge_max_energy_change_eV"] = judge
+                if args.fp64_check > 0:
+                    extra["fp64_check_iterations"] = args.fp64_check
+                    extra["fp64_check_max_energy_change_eV"] = fp64_move
         if len(results) == 2:
             extra["speedup_vs_plain"] = round(results["plain"][0] / results["mixed_learned"][0], 3)
         rf = roofline(stage, f"k_trd_hemv ({key} solve, FP32-copy and FP64 launches)", 0,

@@ -76,7 +76,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--cells", default="6-15")
     ap.add_argument("--fillings", default="-2,-1,0,1,2")
-    ap.add_argument("--data", default="gpurun_out/guess_data.npz")
+    ap.add_argument("--data", default="guess_data.npz")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "guess_mlp.npz"))
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=4096)
