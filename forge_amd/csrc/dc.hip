@@ -31,6 +31,9 @@ namespace forge {
 rocblas_handle blas_handle(hipStream_t s);
 
 constexpr int LEAF = 32;
+// merges of at least this order multiply on rocBLAS dgemm (matrix cores);
+// smaller ones on k_dc_gemm, where one launch covers all merges of a level
+constexpr int DC_BLAS_MIN = 2048;
 constexpr double EPSU = 1.1102230246251565e-16;  // unit roundoff, 2^-53
 
 struct MergeDesc {
@@ -778,10 +781,36 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
                          evals);
       FORGE_LAUNCH();
     } else {
-      unsigned tiles = (unsigned)(((lhalf + 63) / 64) * ((lmout + 63) / 64));
-      hipLaunchKernelGGL(k_dc_gemm, dim3(tiles, 2 * nm, K), dim3(256), 0, s, n, md, P.Qa, P.Wm,
-                         dst, ldo);
-      FORGE_LAUNCH();
+      if (lm >= DC_BLAS_MIN) {
+        // large lower merges: their half products on rocBLAS dgemm as well,
+        // one call per merge and half, batched over the K matrices
+        rocblas_handle h = blas_handle(s);
+        if (!h) return -1;
+        const double one = 1.0, zero = 0.0;
+        const rocblas_stride snn = (rocblas_stride)((size_t)n * n);
+        for (size_t q = 0; q < nm; ++q) {
+          const MergeDesc& M = all[levels[lv].first + q];
+          for (int half = 0; half < 2; ++half) {
+            const int rb = half ? M.n1 : 0, rn = half ? M.n2 : M.n1;
+            if (rn == 0 || M.mout == 0) continue;
+            const size_t a0 = (size_t)(M.o + rb) + (size_t)n * (M.o + rb);
+            const size_t b0 = (size_t)(M.o + rb) + (size_t)n * M.o;
+            const rocblas_status st = rocblas_dgemm_strided_batched(
+                h, rocblas_operation_none, rocblas_operation_none, rn, M.mout, rn, &one,
+                P.Qa + a0, (rocblas_int)n, snn, P.Wm + b0, (rocblas_int)n, snn, &zero, dst + b0,
+                (rocblas_int)ldo, snn, K);
+            if (st != rocblas_status_success) {
+              set_error(std::string("rocblas_dgemm_strided_batched: ") + rocblas_status_to_string(st));
+              return -1;
+            }
+          }
+        }
+      } else {
+        unsigned tiles = (unsigned)(((lhalf + 63) / 64) * ((lmout + 63) / 64));
+        hipLaunchKernelGGL(k_dc_gemm, dim3(tiles, 2 * nm, K), dim3(256), 0, s, n, md, P.Qa, P.Wm,
+                           dst, ldo);
+        FORGE_LAUNCH();
+      }
       hipLaunchKernelGGL(k_dc_evcopy, dim3(cdiv(lmout, 256), nm, K), dim3(256), 0, s, n, md, P,
                          P.Db);
       FORGE_LAUNCH();

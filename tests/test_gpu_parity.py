@@ -80,6 +80,25 @@ def test_zdiagred_random_hermitian(n):
     assert np.abs(Z.conj().T @ Z - np.eye(iu)).max() < 1e-12 * n
 
 
+def test_zdiagred_large_merges():
+    """n = 4200: the divide-and-conquer merges of order >= 2048 (the top two
+    levels) multiply on the matrix-core GEMM path; eigenvalues against
+    LAPACK, residual and orthogonality of the eigenvectors."""
+    n = 4200
+    say("zdiagred large merges", n)
+    rng = np.random.default_rng(4200)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    iu = n // 2
+    w, Z, info = zdiagred(np.tril(A), 1, iu)
+    assert info == 0
+    wr = np.linalg.eigvalsh(A)[:iu]
+    scale = abs(wr).max()
+    np.testing.assert_allclose(w, wr, rtol=0, atol=1e-13 * scale * np.sqrt(n))
+    assert np.abs(A @ Z - Z * w).max() < 1e-14 * scale * n
+    assert np.abs(Z.conj().T @ Z - np.eye(iu)).max() < 1e-13 * n
+
+
 @pytest.mark.parametrize("case", ["t1k3", "t2k3", "t3k6"])
 def test_zdiagred_graphene_matches_reference(case):
     say("zdiagred graphene", case)
