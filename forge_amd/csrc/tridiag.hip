@@ -449,12 +449,25 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
   const double beta = scal_[k * 4 + 2].x;
   __shared__ z2 t1[NB], t2[NB];
   __shared__ double sh[256];
-  if (threadIdx.x < 2 * NB) {
+  // W^H v and V^H v: the ns0 * PB chunk partials of each of the 2 NB entries,
+  // summed by 256 / (2 NB) threads per entry (strided chunks, a short chain
+  // each) and then in a fixed order in LDS
+  {
+    constexpr int NE = 2 * NB, NL = 256 / NE;
+    __shared__ z2 tsum[NL][NE];
+    const int e = threadIdx.x % NE, c0 = threadIdx.x / NE;
     const z2* tp = tpart_ + (size_t)k * msmax * TRD_PB * 2 * NB;
     z2 a = mk(0.0, 0.0);
-    for (i64 s = 0; s < ns0 * TRD_PB; ++s) a = a + tp[s * 2 * NB + threadIdx.x];
-    if (threadIdx.x < NB) t1[threadIdx.x] = a;
-    else t2[threadIdx.x - NB] = a;
+    for (i64 s = c0; s < ns0 * TRD_PB; s += NL) a = a + tp[s * 2 * NB + e];
+    tsum[c0][e] = a;
+    __syncthreads();
+    if (threadIdx.x < NE) {
+      z2 t = tsum[0][threadIdx.x];
+#pragma unroll
+      for (int q = 1; q < NL; ++q) t = t + tsum[q][threadIdx.x];
+      if (threadIdx.x < NB) t1[threadIdx.x] = t;
+      else t2[threadIdx.x - NB] = t;
+    }
   }
   __syncthreads();
   // FIN_ROWS rows per block, 4 lanes per row: lane g sums the strip partials
