@@ -46,9 +46,9 @@ HBM_PEAK_GBS = 8000.0
 # per iteration), so 85 is a strict lower bound
 PLAIN_ITERS = 85
 # complete mixed+learned solves of the default workload: mixed-phase and FP64
-# iterations (bench.py --full --mixer oda --switch-de 2e-6: 28 + 4; the linear
+# iterations (bench.py --full --mixer oda --switch-de 2e-6: 29 + 3; the linear
 # solve was recorded with --switch-de 1e-4: 21 + 52)
-MIXED_LEARNED_ITERS = {"oda": {"mixed": 28, "fp64": 4}, "linear": {"mixed": 21, "fp64": 52}}
+MIXED_LEARNED_ITERS = {"oda": {"mixed": 29, "fp64": 3}, "linear": {"mixed": 21, "fp64": 52}}
 # HBM traffic of k_trd_hemv per launch over algorithmic bytes (read + write,
 # rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated on a copy kernel of known size),
 # measured on tbg_1.05deg_n11908_k3 for the FP64 matrix (16 B per element) and
