@@ -1144,18 +1144,29 @@ extern "C" int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sw
   }
   w->ldH = pad_ld(w->Nmax);
   const size_t sH = (size_t)w->ldH * w->Nmax;
-  FORGE_HIP(hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking));
+  const hipError_t se = hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking);
+  if (se != hipSuccess) {
+    set_error(std::string("forge_sweep_create: hipStreamCreateWithFlags: ") + hipGetErrorString(se));
+    w->s = nullptr;
+    forge_sweep_destroy(w);
+    return -1;
+  }
   if (dalloc((void**)&w->H, sizeof(z2) * sH * w->Ktot) ||
       dalloc(&w->trd_ws, tridiag_workspace_bytes(w->Nmax, w->Ktot))) {
     forge_sweep_destroy(w);
     return -4;
   }
-  FORGE_HIP(hipMemsetAsync(w->H, 0, sizeof(z2) * sH * w->Ktot, w->s));
-  CK(tridiag_bind(w->trd, w->trd_ws, w->Nmax, w->Ktot));
+  tridiag_bind(w->trd, w->trd_ws, w->Nmax, w->Ktot);
   w->trd.A = w->H;
   w->trd.sA = sH;
   w->trd.lda = w->ldH;
-  FORGE_HIP(hipStreamSynchronize(w->s));
+  const hipError_t me = hipMemsetAsync(w->H, 0, sizeof(z2) * sH * w->Ktot, w->s);
+  const hipError_t ye = me == hipSuccess ? hipStreamSynchronize(w->s) : me;
+  if (ye != hipSuccess) {
+    set_error(std::string("forge_sweep_create: clearing the slots: ") + hipGetErrorString(ye));
+    forge_sweep_destroy(w);
+    return -1;
+  }
   *out = w;
   return 0;
 }
