@@ -86,6 +86,9 @@ def parse():
     ap.add_argument("--fp64-check", type=int, default=0,
                     help="--full: N plain FP64 iterations from the mixed+learned solve's final "
                          "state; reports how far they move its energy")
+    ap.add_argument("--guess-diag", type=int, default=1,
+                    help="1: the learned start ends with a diagonalisation of H[guess]; 0: the "
+                         "guess itself is the start state")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-ntheta", type=int, default=11)
     return ap.parse_args()
@@ -188,7 +191,7 @@ def main():
 
     def timed_learned_start():
         t0 = time.perf_counter()
-        learned_start(eng, model, nfill, bf16=True)
+        learned_start(eng, model, nfill, bf16=True, diagonalise=bool(args.guess_diag))
         dt = time.perf_counter() - t0
         log(f"learned start (first diagonalisation + MLP + guess diagonalisation): {dt:.2f}s")
         return dt

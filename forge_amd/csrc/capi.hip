@@ -897,18 +897,20 @@ static int guess_diagonalisation(forge_ctx* c) {
   return fails;
 }
 
-extern "C" int forge_scf_init_guess_update(forge_ctx* c, const double* diag, int64_t nupd,
-                                           const int64_t* row, const int64_t* col,
-                                           const int64_t* cell, const forge_z* delta) {
+// Applies the learned guess to the current state on the device: diagonal
+// values and distinct (row, col, cell) increments.
+static int apply_guess(forge_ctx* c, const char* who, const double* diag, int64_t nupd,
+                       const int64_t* row, const int64_t* col, const int64_t* cell,
+                       const forge_z* delta) {
   if (!c || !diag || nupd < 0 || (nupd > 0 && (!row || !col || !cell || !delta))) {
-    set_error("forge_scf_init_guess_update: invalid argument");
+    set_error(std::string(who) + ": invalid argument");
     return -2;
   }
-  if (!c->scf_ready) { set_error("forge_scf_init_guess_update: call forge_scf_init first"); return -5; }
+  if (!c->scf_ready) { set_error(std::string(who) + ": call forge_scf_init first"); return -5; }
   const i64 n = c->n;
   for (i64 q = 0; q < nupd; ++q)
     if (row[q] < 0 || row[q] >= n || col[q] < 0 || col[q] >= n || cell[q] < 0 || cell[q] >= c->nnc) {
-      set_error("forge_scf_init_guess_update: entry out of range");
+      set_error(std::string(who) + ": entry out of range");
       return -2;
     }
   std::fill(c->tm, c->tm + 16, 0.0);
@@ -937,8 +939,23 @@ extern "C" int forge_scf_init_guess_update(forge_ctx* c, const double* diag, int
   if (dd) (void)hipFree(dd);
   if (ix) (void)hipFree(ix);
   if (dv) (void)hipFree(dv);
-  if (rc != 0) return rc;
+  return rc;
+}
+
+extern "C" int forge_scf_init_guess_update(forge_ctx* c, const double* diag, int64_t nupd,
+                                           const int64_t* row, const int64_t* col,
+                                           const int64_t* cell, const forge_z* delta) {
+  CK(apply_guess(c, "forge_scf_init_guess_update", diag, nupd, row, col, cell, delta));
   return guess_diagonalisation(c);
+}
+
+// The guess itself as the start state (its potential and energies; no
+// diagonalisation): the first SCF iteration then solves H[guess] and mixes.
+extern "C" int forge_scf_init_update(forge_ctx* c, const double* diag, int64_t nupd,
+                                     const int64_t* row, const int64_t* col, const int64_t* cell,
+                                     const forge_z* delta) {
+  CK(apply_guess(c, "forge_scf_init_update", diag, nupd, row, col, cell, delta));
+  return init_state_of_rho(c);
 }
 
 extern "C" int forge_scf_init_guess(forge_ctx* c, const forge_z* rho_guess) {

@@ -173,6 +173,16 @@ class Engine:
         rho = _f(rho_guess, np.complex128)
         return L.check(self.lib.forge_scf_init_guess(self.ctx, _p(rho)), "forge_scf_init_guess")
 
+    def scf_init_update(self, diag, row, col, cell, delta):
+        """The current state with the guess applied (diagonal values and
+        distinct (row, col, cell) increments, 0-based) becomes the start state,
+        without a diagonalisation."""
+        d = _f(diag, np.float64)
+        r, c, k = (np.ascontiguousarray(a, np.int64) for a in (row, col, cell))
+        v = np.ascontiguousarray(delta, np.complex128)
+        return L.check(self.lib.forge_scf_init_update(self.ctx, _p(d), int(r.size), _p(r), _p(c),
+                                                      _p(k), _p(v)), "forge_scf_init_update")
+
     def scf_init_guess_update(self, diag, row, col, cell, delta):
         """As scf_init_guess, the guess given as changes to the current state
         (diagonal values and distinct (row, col, cell) increments, 0-based)."""

@@ -171,14 +171,19 @@ def guess_updates(g: Geometry, Yp: np.ndarray, bonds: Bonds | None = None):
             (uk // (n * n)).astype(np.int64), acc / cnt)
 
 
-def learned_start(engine, model, nfilling: float, bf16: bool = True):
+def learned_start(engine, model, nfilling: float, bf16: bool = True, diagonalise: bool = True):
     """Start state of the engine's SCF from the network: the first
     diagonalisation (rho_TB, on the device), the network's site charges and
-    bond shifts (bf16 matrix-core inference), one diagonalisation of
-    H[rho_TB + proposal].  Returns the network outputs."""
+    bond shifts (bf16 matrix-core inference), then either one diagonalisation
+    of H[rho_TB + proposal] (diagonalise) or the proposal itself as the start
+    state, whose first SCF iteration solves H[proposal] and mixes.  Returns
+    the network outputs."""
     engine.scf_init(None)
     Yp = model.predict(descriptors(engine.geom, nfilling), bf16=bf16)
-    engine.scf_init_guess_update(*guess_updates(engine.geom, Yp))
+    if diagonalise:
+        engine.scf_init_guess_update(*guess_updates(engine.geom, Yp))
+    else:
+        engine.scf_init_update(*guess_updates(engine.geom, Yp))
     return Yp
 
 

@@ -53,6 +53,7 @@ _SIGS = {
     "forge_scf_init": (C.c_int, [vp, vp]),
     "forge_scf_init_guess": (C.c_int, [vp, vp]),
     "forge_scf_init_guess_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),
+    "forge_scf_init_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),
     "forge_scf_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),
     "forge_scf_get_density": (C.c_int, [vp, vp]),
     "forge_scf_timings": (C.c_int, [vp, vp]),

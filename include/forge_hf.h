@@ -143,6 +143,13 @@ int forge_scf_init_guess_update(forge_ctx* ctx, const double* diag, int64_t nupd
                                 const int64_t* row, const int64_t* col, const int64_t* cell,
                                 const forge_z* delta);
 
+/* As forge_scf_init_guess_update, but the updated density itself becomes the
+ * start state (its potential and energies are formed; no diagonalisation):
+ * the first forge_scf_run iteration solves H[guess] and mixes it with the
+ * guess.  No reference counterpart (learned initial guess). */
+int forge_scf_init_update(forge_ctx* ctx, const double* diag, int64_t nupd, const int64_t* row,
+                          const int64_t* col, const int64_t* cell, const forge_z* delta);
+
 /* Runs up to max_iter SCF iterations (Main.f90:365-679).  traj receives one
  * row of 12 doubles per iteration: it, E_in, 2Ekin_in, 2EF_in, EH_in, EU_in,
  * max|dDensity|, Step, E_in - E_new, E_Fermi, nOcc, nPart.  *iters_done is
