@@ -66,7 +66,9 @@ struct TridiagWork {
 };
 constexpr int TRD_NB = 32;  // panel width (a multiple of 8, at most 128; 64 measured slower)
 constexpr int TRD_TS = 32;     // hemv column-strip width
-constexpr int TRD_SEGR = 1024; // hemv rows per block (more blocks for small trailing orders)
+constexpr int TRD_SEGR = 1024;   // hemv rows per block streaming the FP64 matrix
+constexpr int TRD_SEGR32 = 2048; // ... streaming the FP32 copy (measured: 1024 best for FP64,
+                                 // 2048 for the FP32 copy, whose blocks carry half the bytes)
 constexpr int TRD_PB = 4;      // hemv panel-product blocks per row segment
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);

@@ -43,12 +43,18 @@ namespace forge {
 
 constexpr int NB = TRD_NB;
 constexpr int TS = TRD_TS;
-constexpr int SEGR = TRD_SEGR;
+constexpr int SEGR = TRD_SEGR;  // the smaller segment height: sizes the partial arrays
 
 // column strips of width TS over the trailing order m
 __host__ __device__ static inline i64 nt_of(i64 m) { return (m + TS - 1) / TS; }
-// row segments of strip J (rows 32J .. m-1)
-__host__ __device__ static inline i64 nseg_of(i64 m, i64 J) { return (m - TS * J + SEGR - 1) / SEGR; }
+// row segments of strip J (rows 32J .. m-1) of height segr
+__host__ __device__ static inline i64 nseg_of(i64 m, i64 J, i64 segr = SEGR) {
+  return (m - TS * J + segr - 1) / segr;
+}
+// segment height of the hemv streaming elements of type TM
+template <typename TM> __host__ __device__ constexpr int segr_of() {
+  return sizeof(TM) == 8 ? TRD_SEGR32 : TRD_SEGR;
+}
 __host__ __device__ static inline i64 nblk_of(i64 m) { return (m + 255) / 256; }
 // rows per k_trd_fin block and its block count
 constexpr int FIN_ROWS = 64;
@@ -244,7 +250,7 @@ __global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z
 
 // ---------------------------------------------------------------- K2: hemv
 // grid (PB + nt, nseg(m, 0), K).  Strip blocks (x = PB + J) stream column strip J
-// (32 columns, c0 = base + 32 J) over rows [32 J + s SEGR, min(m, 32 J + (s+1) SEGR))
+// (32 columns, c0 = base + 32 J) over rows [32 J + s SG, min(m, 32 J + (s+1) SG)), SG = 1024 (FP64) or 2048 (FP32 copy)
 // of the trailing matrix.  Lane l of a wave covers row (l mod 16) and the 8
 // columns 8 (l / 16) .. + 7 of a 16-row step, so one load instruction reads
 // four 256-byte column segments, a thread holds only 8 column partials, and
@@ -261,6 +267,7 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
                                                   const z2* W_, const double* npart_, z2* P_,
                                                   z2* P2_, z2* tpart_, z2* scal_, i64 ntmax,
                                                   i64 msmax, i64 nbmax) {
+  constexpr int SG = segr_of<TM>();  // rows per segment
   const int k = blockIdx.z;
   const i64 s = blockIdx.y;
   const i64 base = j + 1, m = n - base, nt = nt_of(m);
@@ -271,8 +278,8 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     // of segment s.  These blocks come first in the grid and are small, so
     // they finish under the strip blocks instead of forming the tail.
     const int pb = blockIdx.x;
-    const i64 rs = s * SEGR + pb * (SEGR / TRD_PB);
-    const i64 re = rs + SEGR / TRD_PB < m ? rs + SEGR / TRD_PB : m;
+    const i64 rs = s * SG + pb * (SG / TRD_PB);
+    const i64 re = rs + SG / TRD_PB < m ? rs + SG / TRD_PB : m;
     const bool publish = s == 0 && pb == 0;
     Reflector R{};
     if (publish || (jj > 0 && rs < m))  // uniform over the block
@@ -324,9 +331,9 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     return;
   }
   const i64 J = (i64)blockIdx.x - TRD_PB;  // column strip
-  const i64 rs = TS * J + s * SEGR;  // first row (relative to base) of this block
+  const i64 rs = TS * J + s * SG;  // first row (relative to base) of this block
   if (J >= nt || rs >= m) return;
-  const i64 re = rs + SEGR < m ? rs + SEGR : m;
+  const i64 re = rs + SG < m ? rs + SG : m;
   const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
   const z2 scale = R.scale;
   const i64 c0 = base + J * TS;
@@ -336,13 +343,13 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
   // v of this block's rows (relative rows ra0 + q; zero outside [rs, re)),
   // staged once so that the row steps read it from LDS instead of waiting
   // on a load per step
-  __shared__ z2 vrow[SEGR + 32];
+  __shared__ z2 vrow[SG + 32];
   const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
   if (threadIdx.x < TS) {
     const i64 c = c0 + threadIdx.x;
     vJ[threadIdx.x] = c >= n ? mk(0.0, 0.0) : (c == base ? mk(1.0, 0.0) : A[c + lda * j] * scale);
   }
-  for (int q = threadIdx.x; q < SEGR + 32; q += 256) {
+  for (int q = threadIdx.x; q < SG + 32; q += 256) {
     const i64 rr = ra0 + q;
     vrow[q] = (rr < rs || rr >= re) ? mk(0.0, 0.0)
                                     : (rr == 0 ? mk(1.0, 0.0) : A[(base + rr) + lda * j] * scale);
@@ -439,9 +446,9 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
                                                  i64 lda, z2* W_, double* e_, z2* tau_,
                                                  const z2* scal_, const z2* P_, const z2* P2_,
                                                  const z2* tpart_, z2* dpart_, i64 ntmax, i64 msmax,
-                                                 i64 nbmax) {
+                                                 i64 nbmax, int segr) {
   const int k = blockIdx.y;
-  const i64 base = j + 1, m = n - base, nt = nt_of(m), ns0 = nseg_of(m, 0);
+  const i64 base = j + 1, m = n - base, nt = nt_of(m), ns0 = nseg_of(m, 0, segr);
   (void)nt;
   z2* A = A_ + k * sA;
   z2* W = W_ + (size_t)k * n * NB;
@@ -504,7 +511,7 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
   y.x += __shfl_xor(y.x, 2, 64);
   y.y += __shfl_xor(y.y, 2, 64);
   if (live && g == 0) {
-    const i64 nseg = nseg_of(m, R);
+    const i64 nseg = nseg_of(m, R, segr);
     const z2* P2 = P2_ + ((size_t)k * ntmax + R) * msmax * TS;
     for (i64 s = 0; s < nseg; ++s) y = y + P2[s * TS + rr];
     const z2 v = (r == base) ? mk(1.0, 0.0) : A[r + lda * j] * scale;
@@ -754,7 +761,8 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
       hipLaunchKernelGGL(k_trd_col, dim3(cdiv(n - j, 256), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
-      const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0);
+      const int segr = w.S ? segr_of<float2>() : segr_of<z2>();
+      const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0, segr);
       const dim3 hgrid((unsigned)(nt + TRD_PB), (unsigned)ms, Kj);
       const bool tj = timed && (j % TIME_STRIDE == 0);
       hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -776,7 +784,7 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
       FORGE_LAUNCH();
       hipLaunchKernelGGL(k_trd_fin, dim3((unsigned)nfin_of(m), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
-                         msmax, nbmax);
+                         msmax, nbmax, segr);
       FORGE_LAUNCH();
     }
     const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
