@@ -652,6 +652,7 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
         }
       }
 }
+#undef HER2K_LOAD_STEP
 
 // S = fp32(A) on the lower triangle (column c, rows c..n-1); grid
 // (cdiv(n, 256), ny, K), columns blockIdx.y, blockIdx.y + ny, ...
