@@ -202,6 +202,16 @@ class Engine:
             print("zheev failed", rc)
         return traj[:done.value]
 
+    def write_reference_outputs(self, outdir, traj, statename="SSS", append=False):
+        """The reference's output/Energy* and output/Convergence* files
+        (Main.f90:352-358, 616-631) for trajectory rows of scf_run."""
+        from .checkpoint import run_parameters
+        from .outputs import write_outputs
+        st = self.setup
+        params = run_parameters(statename, st.nfilling, 1, st.ntheta, st.nlayers, 0, st.epsilon,
+                                st.U, st.nscreen, st.xi_nm, st.Delta, st.numI, st.numC, st.numk)
+        return write_outputs(outdir, params, traj, append=append)
+
     def density(self):
         rho = np.zeros((self.n, self.n, self.nnc), np.complex128, order="F")
         L.check(self.lib.forge_scf_get_density(self.ctx, _p(rho)), "forge_scf_get_density")
