@@ -682,7 +682,8 @@ double orc_optimal_step(const orc_params* p, double* e_inout, const zc* rhoNew, 
 i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double* MomVal,
             const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
             const zc* rho_start, i64 itmax, int occ_mode, double kT, int mix_mode, double beta,
-            double etol, double* traj, zc* rho_out, double* timing, int nthreads) {
+            double etol, double* traj, zc* rho_out, double* timing, int nthreads,
+            double* bands_last) {
   i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
   i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
   size_t nn = (size_t)n * n, tot = nn * nnc;
@@ -774,6 +775,8 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
   }
 #undef WINDOW_DENSITY
   if (rho_out) memcpy(rho_out, rho, sizeof(zc) * tot);
+  /* window bands of the last iteration's Solve, Bands(numb, Nk) as in Main.f90 */
+  if (bands_last) memcpy(bands_last, Bands, sizeof(double) * N);
   if (timing) { timing[0] = omp_get_wtime() - t0; timing[1] = tsolve; }
   free(bulk); free(rho); free(rhoIn); free(zEig); free(zSorted); free(nSM); free(Bands);
   free(S); free(wts); free(V); free(D); free(Din); free(dt);

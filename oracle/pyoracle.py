@@ -236,14 +236,21 @@ class System:
         return step, e
 
     def scf(self, itmax, rho_start=None, occ_mode=0, kT=0.0, mix_mode=0, beta=0.3, etol=0.0,
-            nthreads=0):
+            nthreads=0, keep_bands=False):
+        """keep_bands: the last iteration's window bands (numb x Nk) are left
+        in self.last_bands."""
         traj = np.zeros((itmax, 12))
         rho_out = fz((self.n, self.n, self.nnc), np.complex128)
         timing = np.zeros(2)
+        nb = int(self.p.nUpper - self.p.nLower + 1)
+        bands = fz((nb, self.numk * self.numk), np.float64) if keep_bands else None
         A = Args()
         its = lib().orc_scf(self.pp, ptr(self.Coords), ptr(self.tn), ptr(self.MomentaValues),
                             ptr(self.nMomentaComponents), ptr(self.u1), ptr(self.u2),
                             ptr(self.longrange()), A(rho_start, np.complex128),
                             i64(itmax), C.c_int(occ_mode), dbl(kT), C.c_int(mix_mode), dbl(beta),
-                            dbl(etol), ptr(traj), ptr(rho_out), ptr(timing), C.c_int(nthreads))
+                            dbl(etol), ptr(traj), ptr(rho_out), ptr(timing), C.c_int(nthreads),
+                            ptr(bands) if keep_bands else None)
+        if keep_bands:
+            self.last_bands = bands
         return int(its), traj[:max(int(its), 0)], rho_out, timing

@@ -17,12 +17,14 @@
 # compile.sh:1's file order is also wrong (Geometry.f90:5 uses TightBinding),
 # so the modules are compiled in dependency order here.
 #
-# Usage: [WRITEFOCK=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
+# Usage: [WRITEFOCK=1] [PRINTOUT=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
+#   PRINTOUT=1 keeps the reference's nPrintOutputs = 1 (the Bands file, Main.f90:601-608).
 #   XI_NM is xi*0.246 (gate parameter in nm).  Toolchain: amdflang (ROCm) and
 #   MKL LP64 sequential from /opt/conda/lib.
 set -euo pipefail
 CASE=$1; NTHETA=$2; NUMK=$3; NFILL=$4; ITMAX=$5; NREAD=$6; XINM=$7; NLAYERS=${8:-2}
 WRITEFOCK=${WRITEFOCK:-0}
+PRINTOUT=${PRINTOUT:-0}
 REF=/root/reference
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref/$CASE
@@ -45,7 +47,7 @@ sed -i \
   -e "s/^real(dp), parameter :: nfillingIn = .*/real(dp), parameter :: nfillingIn = ${NFILL}_dp/" \
   -e "s/^integer(dp), parameter :: itmax = .*/integer(dp), parameter :: itmax = $ITMAX/" \
   -e "s/^integer(dp), parameter :: nRead = .*/integer(dp), parameter :: nRead = $NREAD/" \
-  -e "s/^integer(dp), parameter :: nPrintOutputs = .*/integer(dp), parameter :: nPrintOutputs = 0/" \
+  -e "s/^integer(dp), parameter :: nPrintOutputs = .*/integer(dp), parameter :: nPrintOutputs = $PRINTOUT/" \
   -e "s/^integer(dp), parameter :: nWriteFock = .*/integer(dp), parameter :: nWriteFock = $WRITEFOCK/" \
   -e "s/^real(dp), parameter:: xi = .*/real(dp), parameter:: xi = ${XINM}_dp\/0.246_dp/" \
   -e "s/^real(dp), parameter:: xiIn = .*/real(dp), parameter:: xiIn = ${XINM}_dp\/0.246_dp/" \

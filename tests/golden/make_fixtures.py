@@ -134,7 +134,30 @@ def checkpoint_fixture():
     shutil.copy(os.path.join(rdir, "dataFock", fname), os.path.join(HERE, "t1k3_it1.ckpt"))
 
 
+def bands_fixture():
+    """The Bands file the reference program writes with nPrintOutputs = 1
+    (PlotBands, TightBinding.f90:275-369, after the last iteration) for the
+    28-orbital cell on a 6 x 6 k grid, filling +1, 3 iterations from the
+    reference's own start: tests/golden/t1k6.bands.txt."""
+    e = dict(os.environ, PRINTOUT="1")
+    subprocess.run([REF_BUILD, "t1k6p", "1", "6", "1.0", "3", "0", "40.0", "2"], check=True,
+                   stdout=subprocess.DEVNULL, env=e)
+    rdir = os.path.join(REF_OUT, "t1k6p", "run")
+    shutil.rmtree(rdir, ignore_errors=True)
+    os.makedirs(os.path.join(rdir, "output"))
+    os.makedirs(os.path.join(rdir, "dataFock"))
+    run("../forge.x > forge.log 2>&1", rdir)
+    (fname,) = [f for f in os.listdir(os.path.join(rdir, "output")) if f.startswith("Bands")]
+    shutil.copy(os.path.join(rdir, "output", fname), os.path.join(HERE, "t1k6.bands.txt"))
+    print("wrote t1k6.bands.txt from", fname, flush=True)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1:])
-    if not sys.argv[1:] or "ckpt" in sys.argv[1:]:
+    args = sys.argv[1:]
+    cases = [a for a in args if a not in ("ckpt", "bands")]
+    if not args or cases:
+        main(cases)
+    if not args or "ckpt" in args:
         checkpoint_fixture()
+    if not args or "bands" in args:
+        bands_fixture()

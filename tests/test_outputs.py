@@ -72,3 +72,28 @@ def test_fortran_f_edge_cases():
     assert fortran_f(123.456, 5, 3) == "*****"   # does not fit
     assert fortran_f(1.0, 5, 1) == "  1.0"
     assert fortran_f(-337.235297886, 24, 9) == "          -337.235297886"
+
+
+def test_bands_file_matches_the_reference():
+    """The Bands file the reference program wrote (tests/golden/t1k6.bands.txt:
+    28-orbital cell, 6 x 6 k, filling +1, 3 iterations from the reference's
+    own start) against plot_bands_lines fed with the oracle's bands and Fermi
+    energy of the same run: same lines, same columns, the path coordinates
+    character for character, band energies within the F16.8 resolution."""
+    from forge_amd.geometry import build_geometry
+    from forge_amd.outputs import flattened_momenta, plot_bands_lines
+    from oracle.pyoracle import System
+
+    ref = [l.rstrip("\n") for l in open(os.path.join(GOLDEN, "t1k6.bands.txt")) if l.strip()]
+    s = System(1, 6, 1.0, nlayers=2, xi_nm=40.0)
+    _, traj, _, _ = s.scf(3, rho_start=None, occ_mode=0, mix_mode=0, keep_bands=True)
+    g = build_geometry(1, 2, 6, 1)
+    lines = plot_bands_lines(s.last_bands, traj[-1, 9], flattened_momenta(g.nMomentaComponents, 6),
+                             6, g.g1, g.g12)
+    assert len(lines) == len(ref) == 11
+    for a, b in zip(lines, ref):
+        va, vb = a.split(), b.split()
+        assert len(va) == len(vb) == 21
+        assert va[0] == vb[0]
+        np.testing.assert_allclose([float(x) for x in va[1:]], [float(x) for x in vb[1:]],
+                                   rtol=0, atol=1.5e-8)
