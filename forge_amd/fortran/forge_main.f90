@@ -16,7 +16,10 @@
 !
 ! Arguments (key=value): ntheta nlayers numk nfilling itmax xi_nm epsilon U
 ! Delta mode start (first | path of a checkpoint in the reference's Fock
-! format) occ mix kT beta etol precision switch_de out (trajectory file).
+! format) occ mix kT beta etol precision switch_de out (trajectory file)
+! refout (optional: a directory prefix such as output/, which must exist,
+! receiving the reference's own Energy<parameters> and
+! Convergence<parameters> files, Main.f90:352-358, 616-631).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -26,7 +29,7 @@ program forge_main
   implicit none
 
   character(len=16) :: mode = 'reference'
-  character(len=512) :: start = 'first', outfile = 'forge_traj.txt'
+  character(len=512) :: start = 'first', outfile = 'forge_traj.txt', refout = ''
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
   real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp, switch_de = 0.0_dp
   integer(dp) :: nnc, it, n, nFermiLevel, nOcc, nPart, icount, i, nU
@@ -108,6 +111,11 @@ contains
       do i = 1, done
         write(fout, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') traj(1, i), traj(7, i), traj(8, i), traj(9, i)
       end do
+      if (len_trim(refout) > 0) &
+        call write_reference_files(traj(1:6, 1:done), &
+                                   reshape([traj(1, 1:done), traj(7, 1:done), traj(8, 1:done), &
+                                            traj(9, 1:done)], [int(done), 4], order=[1, 2]), &
+                                   int(done, dp))
     else
       call reference_sequence()
     end if
@@ -216,7 +224,44 @@ contains
     do i = 1, min(it, itmax)
       write(fout, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') crow(:, i)
     end do
+    if (len_trim(refout) > 0) &
+      call write_reference_files(erow(:, 1:min(it, itmax)), transpose(crow(:, 1:min(it, itmax))), &
+                                 min(it, itmax))
   end subroutine reference_sequence
+
+  ! The run-parameter string of the reference's file names (Main.f90:180-183;
+  ! one spin, no relaxation, state label SSS, dp = 8).
+  function run_parameters() result(par)
+    character(len=210) :: par
+    character(len=1) :: dop
+    dop = '+'
+    if (nfilling < 0.0_dp) dop = '-'
+    write(par, '(A1,A3,A8,A1,F3.1,A5,I0,A2,I0,A8,I0,A6,I0,A4,F0.1,A2,F4.2,A7,I0,A3,F0.1,A6,F0.3,&
+               &A5,I0,A5,I0,A5,I0,A3,I0,A4)') &
+        '-', 'SSS', '-filling', dop, abs(nfilling), '-numS', 1, '-i', ntheta, '-nlayers', nlayers, &
+        '-relax', 0, '-eps', epsilon, '-U', U, '-screen', nscreen, '-xi', xi * 0.246_dp, &
+        '-delta', Delta, '-numI', numI, '-numC', numC, '-numk', numk, '-dp', 8, '.dat'
+  end function run_parameters
+
+  ! refout//Energy<parameters> and refout//Convergence<parameters> in the
+  ! reference's record formats (Main.f90:616-631).  e(6, nr): it, E_tot,
+  ! 2 Ekin, 2 EF, EH, EU; c(nr, 4): it, max|dDensity|, Step, E_in - E_new.
+  subroutine write_reference_files(e, c, nr)
+    real(dp), intent(in) :: e(:, :), c(:, :)
+    integer(dp), intent(in) :: nr
+    integer :: ue, uc
+    integer(dp) :: r
+    character(len=210) :: par
+    par = run_parameters()
+    open(newunit=ue, file=trim(refout)//'Energy'//trim(par), status='replace')
+    open(newunit=uc, file=trim(refout)//'Convergence'//trim(par), status='replace')
+    do r = 1, nr
+      write(ue, '(F5.1,3X,5(F24.9,3X))') e(:, r)
+      write(uc, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') c(r, :)
+    end do
+    close(ue)
+    close(uc)
+  end subroutine write_reference_files
 
   ! Fermi level and degenerate shell (Main.f90:396-414): states within
   ! EnergyTolerance of E_F share the remaining electrons equally.
@@ -305,6 +350,7 @@ contains
     write(fout, '(2I8)') (nMomentaComponents(i, :), i = 1, Nk)
     write(fout, '(A)') '# nUnitCell'
     write(fout, '(2I8)') (nUnitCell_1(i), nUnitCell_2(i), i = 1, nnc)
+    write(fout, '(A,A)') '# parameters ', trim(run_parameters())
     close(fout)
   end subroutine write_geometry
 
@@ -340,6 +386,7 @@ contains
       case ('precision'); read(val, *) prec
       case ('switch_de'); read(val, *) switch_de
       case ('out'); outfile = val
+      case ('refout'); refout = val
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)
         error stop 1

@@ -1,11 +1,13 @@
 """The Fortran host (forge_amd/fortran, built into forge_amd/_lib/forge_main).
 
-CPU: the executable is built and its geometry (its own Fortran restatement of
-Geometry.f90) equals the reference's bit for bit on the fixture cases.
+CPU: the executable is built, its geometry (its own Fortran restatement of
+Geometry.f90) equals the reference's bit for bit on the fixture cases, and
+its run-parameter string reproduces the names of the reference's files.
 GPU: both host modes reproduce the reference's Energy/Convergence rows --
 mode=reference calls the drop-in shims where Main.f90 calls the reference
 routines, mode=device runs the loop device-resident; the seeded case goes
-through the reference's checkpoint format (nRead = 1 path).
+through the reference's checkpoint format (nRead = 1 path); both also write
+the reference's own Energy/Convergence files (refout=).
 """
 import os
 import subprocess
@@ -60,6 +62,28 @@ def test_fortran_geometry_matches_reference_bitwise(case, tmp_path):
     np.testing.assert_array_equal(g["nUnitCell"].reshape(-1, 2)[:, 1], f["nUnitCell_2"])
 
 
+def _reference_file_name(case):
+    """The reference's Energy file name recorded with the fixture run."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", case + ".traj.txt")
+    for line in open(path):
+        if line.startswith("# file: "):
+            return line[len("# file: "):].strip()
+    raise AssertionError("no file name in " + path)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fortran_run_parameters_match_reference_file_names(case, tmp_path):
+    """run_parameters() of the Fortran host (Main.f90:180-183) reproduces the
+    names of the reference's output files (CPU: geometry mode)."""
+    c = case_params(load_case(case))
+    out = tmp_path / "geom.txt"
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", *_args(c), f"out={out}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    par = [l.split()[2] for l in open(out) if l.startswith("# parameters ")][0]
+    assert "Energy" + par == _reference_file_name(case)
+
+
 def _traj(path):
     eng, conv, cur = [], [], None
     for line in open(path):
@@ -84,8 +108,10 @@ def test_fortran_host_trajectory_matches_reference(case, mode, tmp_path):
     print("fortran host", case, mode, flush=True)
     f = load_case(case)
     c = case_params(f)
+    refdir = tmp_path / "output"
+    refdir.mkdir()
     args = [FORTRAN_EXE, f"mode={mode}", *_args(c), f"itmax={c['itmax']}",
-            f"out={tmp_path / 'traj.txt'}"]
+            f"out={tmp_path / 'traj.txt'}", f"refout={refdir}/"]
     if c["seeded"]:
         g = build_geometry(c["ntheta"], c["nlayers"], c["numk"])
         ck = tmp_path / "start.ckpt"
@@ -101,3 +127,11 @@ def test_fortran_host_trajectory_matches_reference(case, mode, tmp_path):
     np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
     np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
     np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
+    # the reference's own output files (refout=): its names, the same rows
+    name = _reference_file_name(case)
+    fe = refdir / name
+    fc = refdir / ("Convergence" + name[len("Energy"):])
+    Ef = np.array([[float(x) for x in l.split()] for l in open(fe) if l.strip()])
+    Cf = np.array([[float(x) for x in l.split()] for l in open(fc) if l.strip()])
+    np.testing.assert_array_equal(Ef, E)
+    np.testing.assert_array_equal(Cf, Cv)
