@@ -14,6 +14,10 @@
  * reference declares it; complex values are C99 double complex
  * (= Fortran complex(8)); indices are int64 (= integer(dp), Setup.f90:6).
  *
+ * The C3-reduced k-set (nenforceC3 = 1: orc_c3_pairs, orc_solve_c3,
+ * orc_getfock_c3) is pinned against the reference program built with that
+ * setting (the tests/golden/ files named <case>_c3.traj.txt).
+ *
  * Extensions with no reference counterpart (used by the configurations that
  * ask for them): Fermi-Dirac window occupations (occ_mode 2), physical
  * degenerate-shell weight (occ_mode 1), linear mixing (mix_mode 1) and an
@@ -538,6 +542,229 @@ void orc_getfock(const orc_params* p, zc* zFock, const zc* zSorted, const i64* n
   free(zP);
 }
 
+/* ------------------------------------------------------------------ C3-reduced k-set */
+
+/* C3_RelatedPoints, Geometry.f90:1172-1223: perm[i] (0-based) is the site j of
+ * the same layer and sublattice half that sits at C3 r_i (rotation by +120
+ * degrees about the origin) within 1e-3; the last site of each half (the two
+ * cell corners of each layer) maps to itself.  Returns the number of sites
+ * assigned (ndim for a C3-symmetric cell; the reference prints an error
+ * otherwise and leaves unmatched entries unset, here they are -1). */
+i64 orc_c3_pairs(const double* C, i64 ndim, i64 nlayers, i64* perm) {
+  const double h3 = 0.5 * sqrt(3.0);
+  i64 per = ndim / nlayers, half = per / 2, cnt = 0;
+  for (i64 i = 0; i < ndim; ++i) perm[i] = -1;
+  for (i64 l = 0; l < nlayers; ++l)
+    for (int hf = 0; hf < 2; ++hf) {
+      i64 lo = l * per + hf * half, hi = lo + half;
+      for (i64 i = lo; i < hi; ++i) {
+        double x = C[i], y = C[i + ndim];
+        double x3 = -0.5 * x - h3 * y, y3 = h3 * x - 0.5 * y;
+        for (i64 j = lo; j < hi; ++j) {
+          double dx = C[j] - x3, dy = C[j + ndim] - y3;
+          if (sqrt(dx * dx + dy * dy) < 1e-3) { perm[i] = j; ++cnt; }
+        }
+      }
+    }
+  for (i64 l = 0; l < nlayers; ++l) {
+    perm[l * per + half - 1] = l * per + half - 1;
+    perm[l * per + per - 1] = l * per + per - 1;
+    cnt += 2;
+  }
+  return cnt;
+}
+
+/* Corner-site factors of the C3 images.  For layer rotation sign rot, the
+ * first corner (last site of the first half) carries e^{i rot a} and the second
+ * (last site of the layer) e^{-i rot a}; every other site carries 1.  The
+ * reference writes these as row/column scalings of the rotated matrices
+ * (HartreeFock.f90:300-330, 344-372) and as zphase of the rotated eigenvectors
+ * (HartreeFock.f90:225-275). */
+static void c3_corner(const orc_params* p, const i64* rot, double a, zc* f) {
+  i64 n = p->ndim, per = n / p->nlayers, half = per / 2;
+  for (i64 i = 0; i < n; ++i) f[i] = 1.0;
+  for (i64 l = 0; l < p->nlayers; ++l) {
+    double r = (double)rot[l];
+    if (rot[l] != 1 && rot[l] != -1) continue;
+    f[l * per + half - 1] = cexp(I * r * a);
+    f[l * per + per - 1] = cexp(-I * r * a);
+  }
+}
+
+/* Flat momentum index ivk1*numk + ivk2 (0-based) of the two C3 partners of k,
+ * HartreeFock.f90:152-165: rel[0] = (-i1-i2, i1), rel[1] = (i2, -i1-i2) mod numk. */
+static void c3_related(i64 numk, i64 i1, i64 i2, i64* rel) {
+  i64 m = ((-i1 - i2) % numk + numk) % numk;
+  rel[0] = m * numk + (i1 % numk);
+  rel[1] = (i2 % numk) * numk + m;
+}
+
+/* The C3-invariant momenta Gamma, K, K' (HartreeFock.f90:218, 377, 594):
+ * flat indices 0, numk/3 (numk+1), 2 numk/3 (numk+1); numk % 3 == 0. */
+static int c3_invariant(i64 numk, i64 k) {
+  return k == 0 || k == numk / 3 * (numk + 1) || k == 2 * numk / 3 * (numk + 1);
+}
+
+/* T1(i,j) = T(perm i, perm j) f_i conj(f_j): the C3 image of a density block
+ * (the reference's row/column scalings, with f the row factor). */
+static void c3_image(i64 n, const i64* perm, const zc* f, const zc* T, zc* out) {
+  for (i64 j = 0; j < n; ++j)
+    for (i64 i = 0; i < n; ++i)
+      out[i + n * j] = T[perm[i] + n * perm[j]] * f[i] * conj(f[j]);
+}
+
+/* Solve_C3, HartreeFock.f90:131-398.  Diagonalises H(k) only at one momentum
+ * of each C3 orbit (the first in flat order), copies its window bands to the
+ * two partners, writes the partners' window eigenvectors as permuted and
+ * corner-phased copies, and adds the bulk density of the orbit as
+ * T phi_c(k) + C3[T] phi_c(k'') + C3^2[T] phi_c(k'), divided by 3 at the
+ * invariant momenta.  Returns -2 when numk % 3 != 0 (the reference's
+ * invariant-point test is only right then), else zheevr's info. */
+int orc_solve_c3(const orc_params* p, const i64* rot, const i64* perm, zc* zFockBulk,
+                 const zc* zFock, const double* Potential, double* Bands, zc* zEig,
+                 const double* C, const double* MomVal, const i64* nMomComp, const i64* u1,
+                 const i64* u2, const double* tn, int nthreads) {
+  i64 n = p->ndim, nnc = p->nnc, numk = p->numk, Nk = numk * numk;
+  i64 nL = p->nLower, nU = p->nUpper, nb = nU - nL + 1;
+  size_t nn = (size_t)n * n;
+  if (numk % 3 != 0) return -2;
+  for (i64 k = 0; k < Nk; ++k)
+    if (nMomComp[k] * numk + nMomComp[k + Nk] != k) return -2;
+  /* representatives, HartreeFock.f90:168-189 */
+  i64* reps = (i64*)malloc(sizeof(i64) * Nk);
+  char* incl = (char*)calloc(Nk, 1);
+  i64 nrep = 0;
+  for (i64 k = 0; k < Nk; ++k) {
+    if (incl[k]) continue;
+    i64 rel[2];
+    c3_related(numk, nMomComp[k], nMomComp[k + Nk], rel);
+    reps[nrep++] = k;
+    incl[k] = incl[rel[0]] = incl[rel[1]] = 1;
+  }
+  memset(zFockBulk, 0, sizeof(zc) * nn * nnc);
+  int status = 0;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+  {
+    zc* H = (zc*)malloc(sizeof(zc) * nn);
+    zc* T = (zc*)malloc(sizeof(zc) * nn);
+    zc* R1 = (zc*)malloc(sizeof(zc) * nn);
+    zc* R2 = (zc*)malloc(sizeof(zc) * nn);
+    zc* zP = (zc*)malloc(sizeof(zc) * n);
+    zc* f1 = (zc*)malloc(sizeof(zc) * n);
+    zc* f2 = (zc*)malloc(sizeof(zc) * n);
+    double* E = (double*)malloc(sizeof(double) * n);
+    zc* acc = (zc*)calloc(nn * nnc, sizeof(zc));
+#pragma omp for schedule(dynamic, 1)
+    for (i64 r = 0; r < nrep; ++r) {
+      i64 k = reps[r], i1 = nMomComp[k], i2 = nMomComp[k + Nk];
+      i64 rel[2];
+      c3_related(numk, i1, i2, rel);
+      const int inv = c3_invariant(numk, k);
+      double vk[2] = {MomVal[i1 + numk * i2], MomVal[i1 + numk * i2 + Nk]};
+      orc_hamiltonian_hf(p, H, C, Potential, zFock, u1, u2, vk, tn);
+      int info = orc_zdiagred(H, n, E, 1, nU);
+      if (info) {
+#pragma omp atomic write
+        status = info;
+      }
+      for (i64 b = nL; b <= nU; ++b)
+        for (int q = 0; q < 3; ++q) Bands[(b - nL) + nb * (q == 0 ? k : rel[q - 1])] = E[b - 1];
+      /* window eigenvectors: C3^2 image at rel[0], C3 image at rel[1]
+       * (HartreeFock.f90:212-277, fphase = 0) */
+      i64 k2 = rel[0], k3 = rel[1];
+      c3_corner(p, rot, 2.0 * PI * (double)(nMomComp[k2] + nMomComp[k2 + Nk]) / (double)numk, f2);
+      c3_corner(p, rot, 2.0 * PI * (double)nMomComp[k3 + Nk] / (double)numk, f1);
+      for (i64 b = nL; b <= nU; ++b) {
+        const zc* v = H + n * (b - 1);
+        memcpy(zEig + n * ((b - nL) + nb * k), v, sizeof(zc) * n);
+        if (inv) continue;
+        zc* w2 = zEig + n * ((b - nL) + nb * k2);
+        zc* w1 = zEig + n * ((b - nL) + nb * k3);
+        for (i64 i = 0; i < n; ++i) {
+          w2[i] = v[perm[perm[i]]] * f2[i];
+          w1[i] = v[perm[i]] * f1[i];
+        }
+      }
+      /* bulk block of the representative, HartreeFock.f90:280-295 */
+      memset(T, 0, sizeof(zc) * nn);
+      for (i64 b = 1; b <= nL - 1; ++b) {
+        for (i64 i = 0; i < n; ++i) {
+          double ph = (vk[0] * C[i] + vk[1] * C[i + n]) * (double)p->fphase;
+          zP[i] = conj(H[i + n * (b - 1)] * cexp(I * ph));
+        }
+        vector_mul(zP, T, (int)n, 1.0);
+      }
+      /* C3 image (first-corner rows e^{i rot t1}, t1 = 2 pi (i1+i2)/numk) and
+       * its C3 image (first-corner rows e^{-i rot t2}, t2 = 2 pi i1/numk),
+       * HartreeFock.f90:297-375 */
+      c3_corner(p, rot, 2.0 * PI * (double)(i1 + i2) / (double)numk, f1);
+      c3_image(n, perm, f1, T, R1);
+      c3_corner(p, rot, -2.0 * PI * (double)i1 / (double)numk, f2);
+      c3_image(n, perm, f2, R1, R2);
+      const double w = inv ? 1.0 / 3.0 : 1.0;
+      for (i64 c = 0; c < nnc; ++c) {
+        zc ph0 = c == 0 ? 1.0 : cexp(I * cell_phase_arg(p, i1, i2, u1[c], u2[c]));
+        zc ph1 = c == 0 ? 1.0 : cexp(I * cell_phase_arg(p, nMomComp[k3], nMomComp[k3 + Nk], u1[c], u2[c]));
+        zc ph2 = c == 0 ? 1.0 : cexp(I * cell_phase_arg(p, nMomComp[k2], nMomComp[k2 + Nk], u1[c], u2[c]));
+        zc* dst = acc + nn * c;
+        for (size_t q = 0; q < nn; ++q) dst[q] += w * (T[q] * ph0 + R1[q] * ph1 + R2[q] * ph2);
+      }
+    }
+#pragma omp critical
+    for (size_t q = 0; q < nn * nnc; ++q) zFockBulk[q] += acc[q];
+    free(H); free(T); free(R1); free(R2); free(zP); free(f1); free(f2); free(E); free(acc);
+  }
+  free(reps); free(incl);
+  return status;
+}
+
+/* GetFock_C3, HartreeFock.f90:564-881.  States at C3-non-invariant momenta
+ * add as in GetFock (their eigenvectors are already the C3 images written by
+ * Solve_C3); a state at an invariant momentum adds (T + C3[T] + C3^2[T]) / 3
+ * to every cell with its own cell phase.  Here the images are formed on the
+ * state vector (zP1 = f1 zP(perm), zP2 = f2 zP1(perm)), which is the same
+ * product as the reference's row/column scaling of the permuted block.
+ * Weights as in orc_getfock. */
+void orc_getfock_c3(const orc_params* p, const i64* rot, const i64* perm, zc* zFock,
+                    const zc* zSorted, const i64* nSortedMom, double Deg, i64 nOcc, i64 nPart,
+                    const double* C, const double* MomVal, const i64* u1, const i64* u2,
+                    int occ_mode, const double* weights, i64 count) {
+  i64 n = p->ndim, nnc = p->nnc, numk = p->numk, Nk = numk * numk;
+  i64 N = (p->nUpper - p->nLower + 1) * Nk;
+  size_t nn = (size_t)n * n;
+  zc zdeg = occ_mode == 0 ? ((double)(float)Deg + 8.0 * I) : (zc)Deg;
+  i64 last = occ_mode == 2 ? count : nOcc;
+  zc* zP = (zc*)malloc(sizeof(zc) * 3 * n);
+  zc* f = (zc*)malloc(sizeof(zc) * n);
+  for (i64 s = 0; s < last && s < N; ++s) {
+    i64 i1 = nSortedMom[s], i2 = nSortedMom[s + N];
+    double vk[2] = {MomVal[i1 + numk * i2], MomVal[i1 + numk * i2 + Nk]};
+    for (i64 i = 0; i < n; ++i) {
+      double ph = (vk[0] * C[i] + vk[1] * C[i + n]) * (double)p->fphase;
+      zP[i] = conj(zSorted[i + n * s] * cexp(I * ph));
+    }
+    zc wgt;
+    if (occ_mode == 2) wgt = weights[s];
+    else wgt = (s < nOcc - nPart) ? (zc)1.0 : zdeg;
+    if (occ_mode == 2 && weights[s] == 0.0) continue;
+    int nvec = 1;
+    if (c3_invariant(numk, i1 * numk + i2)) {
+      nvec = 3;
+      wgt /= 3.0;
+      c3_corner(p, rot, 2.0 * PI * (double)(i1 + i2) / (double)numk, f);
+      for (i64 i = 0; i < n; ++i) zP[n + i] = zP[perm[i]] * f[i];
+      c3_corner(p, rot, -2.0 * PI * (double)i1 / (double)numk, f);
+      for (i64 i = 0; i < n; ++i) zP[2 * n + i] = zP[n + perm[i]] * f[i];
+    }
+    for (i64 c = 0; c < nnc; ++c) {
+      zc ph = cexp(I * cell_phase_arg(p, i1, i2, u1[c], u2[c])) * wgt;
+      for (int q = 0; q < nvec; ++q) vector_mul(zP + n * q, zFock + nn * c, (int)n, ph);
+    }
+  }
+  free(zP); free(f);
+}
+
 /* ------------------------------------------------------------------ energies */
 
 /* GetKineticEnergy, HartreeFock.f90:1012-1059. */
@@ -678,12 +905,16 @@ double orc_optimal_step(const orc_params* p, double* e_inout, const zc* rhoNew, 
  *   etol        >0: stop after the first iteration with |E_in - E_new| < etol
  *   traj        itmax x 12 row-major: it, E_in, 2Ekin_in, 2EF_in, EH_in, EU_in,
  *               maxdDensity, Step, E_in - E_new, E_Fermi, nOcc, nPart
- * Returns the number of iterations run (negative on an eigensolver error). */
+ *   rot, c3perm NULL: Solve / GetFock; else nenforceC3 = 1 (Main.f90:244-250,
+ *               273-276, 375-378, 500-505): Solve_C3 / GetFock_C3 with the
+ *               layer rotation signs and the orc_c3_pairs permutation
+ * Returns the number of iterations run (negative on an eigensolver error or,
+ * with c3perm, numk % 3 != 0). */
 i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double* MomVal,
             const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
             const zc* rho_start, i64 itmax, int occ_mode, double kT, int mix_mode, double beta,
             double etol, double* traj, zc* rho_out, double* timing, int nthreads,
-            double* bands_last) {
+            double* bands_last, const i64* rot, const i64* c3perm) {
   i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
   i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
   size_t nn = (size_t)n * n, tot = nn * nnc;
@@ -705,6 +936,18 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
   double t0 = omp_get_wtime(), tsolve = 0;
   int info;
 
+#define GETFOCK(mode, w, cnt)                                                            \
+  do {                                                                                   \
+    if (c3perm)                                                                          \
+      orc_getfock_c3(p, rot, c3perm, rho, zSorted, nSM, Deg, nOcc, nPart, C, MomVal, u1, u2, \
+                     mode, w, cnt);                                                      \
+    else                                                                                 \
+      orc_getfock(p, rho, zSorted, nSM, Deg, nOcc, nPart, C, MomVal, u1, u2, mode, w, cnt); \
+  } while (0)
+#define SOLVE(rho_in)                                                                    \
+  (c3perm ? orc_solve_c3(p, rot, c3perm, bulk, rho_in, V, Bands, zEig, C, MomVal, nMomComp, \
+                         u1, u2, tn, nthreads)                                           \
+          : orc_solve(p, bulk, rho_in, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads))
 #define WINDOW_DENSITY()                                                             \
   do {                                                                               \
     orc_sort_energies(p, Bands, zEig, nMomComp, S, zSorted, nSM, NULL);              \
@@ -712,9 +955,9 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
     memset(rho, 0, sizeof(zc) * tot);                                                \
     if (occ_mode == 2) {                                                             \
       EF = orc_fermi_dirac(S, N, nF, kT, wts);                                       \
-      orc_getfock(p, rho, zSorted, nSM, Deg, nOcc, nPart, C, MomVal, u1, u2, 2, wts, N); \
+      GETFOCK(2, wts, N);                                                            \
     } else {                                                                         \
-      orc_getfock(p, rho, zSorted, nSM, Deg, nOcc, nPart, C, MomVal, u1, u2, occ_mode, NULL, 0); \
+      GETFOCK(occ_mode, NULL, 0);                                                    \
     }                                                                                \
     for (size_t q = 0; q < tot; ++q) rho[q] = (rho[q] + bulk[q]) / (zc)(double)Nk;  \
   } while (0)
@@ -722,7 +965,7 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
   if (rho_start == NULL) {
     memset(rhoIn, 0, sizeof(zc) * tot);
     double t = omp_get_wtime();
-    info = orc_solve(p, bulk, rhoIn, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads);
+    info = SOLVE(rhoIn);
     tsolve += omp_get_wtime() - t;
     if (info) return -1;
     WINDOW_DENSITY();
@@ -739,7 +982,7 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
   while (it < itmax) {
     ++it;
     double t = omp_get_wtime();
-    info = orc_solve(p, bulk, rho, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads);
+    info = SOLVE(rho);
     tsolve += omp_get_wtime() - t;
     if (info) return -it;
     WINDOW_DENSITY();
@@ -774,6 +1017,8 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
     if (etol > 0 && fabs(EtotIn - Etot) < etol) break;
   }
 #undef WINDOW_DENSITY
+#undef SOLVE
+#undef GETFOCK
   if (rho_out) memcpy(rho_out, rho, sizeof(zc) * tot);
   /* window bands of the last iteration's Solve, Bands(numb, Nk) as in Main.f90 */
   if (bands_last) memcpy(bands_last, Bands, sizeof(double) * N);

@@ -53,6 +53,7 @@ def lib():
         L.orc_optimal_step.restype = dbl
         L.orc_fermi_dirac.restype = dbl
         L.orc_scf.restype = i64
+        L.orc_c3_pairs.restype = i64
         _lib = L
     return _lib
 
@@ -112,6 +113,7 @@ class System:
         if rotate is None:
             rotate = [-1, 1] if nlayers == 2 else ([-1, 1, -1] if nlayers == 3 else [-1])
         rot = np.array(rotate, dtype=np.int64)
+        self.rot = rot
         Nk = numk * numk
         self.Coords = fz((n, 3))
         self.tn = fz((2, 3))
@@ -235,10 +237,23 @@ class System:
                                       ptr(self.longrange()), ptr(self.u1), ptr(self.u2), ptr(self.tn))
         return step, e
 
+    def c3_pairs(self):
+        """nC3pairs (C3_RelatedPoints, Geometry.f90:1172-1223), 0-based, and
+        the number of sites matched."""
+        perm = np.zeros(self.n, np.int64)
+        cnt = int(lib().orc_c3_pairs(ptr(self.Coords), i64(self.n), i64(self.nlayers), ptr(perm)))
+        return perm, cnt
+
     def scf(self, itmax, rho_start=None, occ_mode=0, kT=0.0, mix_mode=0, beta=0.3, etol=0.0,
-            nthreads=0, keep_bands=False):
+            nthreads=0, keep_bands=False, c3=False):
         """keep_bands: the last iteration's window bands (numb x Nk) are left
-        in self.last_bands."""
+        in self.last_bands.  c3: the reference's nenforceC3 = 1 path
+        (Solve_C3 / GetFock_C3)."""
+        perm = None
+        if c3:
+            perm, cnt = self.c3_pairs()
+            if cnt != self.n:
+                raise ValueError(f"cell is not C3-symmetric ({cnt} of {self.n} sites matched)")
         traj = np.zeros((itmax, 12))
         rho_out = fz((self.n, self.n, self.nnc), np.complex128)
         timing = np.zeros(2)
@@ -250,7 +265,8 @@ class System:
                             ptr(self.longrange()), A(rho_start, np.complex128),
                             i64(itmax), C.c_int(occ_mode), dbl(kT), C.c_int(mix_mode), dbl(beta),
                             dbl(etol), ptr(traj), ptr(rho_out), ptr(timing), C.c_int(nthreads),
-                            ptr(bands) if keep_bands else None)
+                            ptr(bands) if keep_bands else None,
+                            ptr(self.rot) if c3 else None, ptr(perm) if c3 else None)
         if keep_bands:
             self.last_bands = bands
         return int(its), traj[:max(int(its), 0)], rho_out, timing

@@ -103,20 +103,58 @@ def main(selected):
                                     xi_nm=xinm, delta=0.0, numI=1, numC=3, numk=numk)
             write_checkpoint(os.path.join(rdir, "dataFock", fname), rho)
         run("../forge.x > forge.log 2>&1", rdir)
-        outs = sorted(os.listdir(os.path.join(rdir, "output")))
-        eng = [f for f in outs if f.startswith("Energy")][0]
-        conv = [f for f in outs if f.startswith("Convergence")][0]
-        with open(os.path.join(HERE, name + ".traj.txt"), "w") as f:
-            f.write(f"# reference FORGE run: ntheta={ntheta} nlayers={nlayers} numk={numk} "
-                    f"nfilling={nfill} xi={xinm}nm itmax={itmax} start={'seeded' if seeded else 'reference'}\n")
-            f.write(f"# file: {eng}\n")
-            f.write("# Energy: it Etot 2Ekin 2EF EH EU\n")
-            f.write(open(os.path.join(rdir, "output", eng)).read())
-            f.write("# Convergence: it maxdDensity Step dE\n")
-            f.write(open(os.path.join(rdir, "output", conv)).read())
+        write_traj(os.path.join(HERE, name + ".traj.txt"), rdir,
+                   f"ntheta={ntheta} nlayers={nlayers} numk={numk} nfilling={nfill} xi={xinm}nm "
+                   f"itmax={itmax} start={'seeded' if seeded else 'reference'}")
         arrays["case"] = np.array([ntheta, numk, nfill, itmax, int(seeded), xinm, nlayers])
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
         print("wrote", name, flush=True)
+
+
+def write_traj(path, rdir, header):
+    """The reference's Energy and Convergence rows of one run, under a header
+    naming the configuration."""
+    outs = sorted(os.listdir(os.path.join(rdir, "output")))
+    eng = [f for f in outs if f.startswith("Energy")][0]
+    conv = [f for f in outs if f.startswith("Convergence")][0]
+    with open(path, "w") as f:
+        f.write(f"# reference FORGE run: {header}\n")
+        f.write(f"# file: {eng}\n")
+        f.write("# Energy: it Etot 2Ekin 2EF EH EU\n")
+        f.write(open(os.path.join(rdir, "output", eng)).read())
+        f.write("# Convergence: it maxdDensity Step dE\n")
+        f.write(open(os.path.join(rdir, "output", conv)).read())
+
+
+# C3-reduced k-set (nenforceC3 = 1): name, ntheta, numk, nfilling, itmax, nlayers
+C3_CASES = [
+    ("t1k3", 1, 3, 1.0, 12, 2),
+    ("tri1k3", 1, 3, 2.0, 6, 3),
+    ("t2k3", 2, 3, 1.0, 6, 2),
+    ("t1k6", 1, 6, 1.0, 4, 2),
+]
+
+
+def c3_fixtures():
+    """Trajectories of the reference program built with nenforceC3 = 1
+    (Solve_C3 / GetFock_C3, HartreeFock.f90:131-398, 564-881) from its own
+    first diagonalisation: tests/golden/<case>_c3.traj.txt."""
+    e = dict(os.environ, C3="1")
+    for (name, ntheta, numk, nfill, itmax, nlayers) in C3_CASES:
+        cname = name + "_c3"
+        subprocess.run([REF_BUILD, cname, str(ntheta), str(numk), f"{nfill:.1f}", str(itmax), "0",
+                        "40.0", str(nlayers)], check=True, stdout=subprocess.DEVNULL, env=e)
+        rdir = os.path.join(REF_OUT, cname, "run")
+        shutil.rmtree(rdir, ignore_errors=True)
+        os.makedirs(os.path.join(rdir, "output"))
+        os.makedirs(os.path.join(rdir, "dataFock"))
+        run("../forge.x > forge.log 2>&1", rdir)
+        if "ERROR C3" in open(os.path.join(rdir, "forge.log")).read():
+            raise RuntimeError(f"{cname}: the reference reports a cell that is not C3-symmetric")
+        write_traj(os.path.join(HERE, cname + ".traj.txt"), rdir,
+                   f"ntheta={ntheta} nlayers={nlayers} numk={numk} nfilling={nfill} xi=40.0nm "
+                   f"itmax={itmax} start=reference nenforceC3=1")
+        print("wrote", cname, flush=True)
 
 
 def checkpoint_fixture():
@@ -154,10 +192,12 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    cases = [a for a in args if a not in ("ckpt", "bands")]
+    cases = [a for a in args if a not in ("ckpt", "bands", "c3")]
     if not args or cases:
         main(cases)
     if not args or "ckpt" in args:
         checkpoint_fixture()
     if not args or "bands" in args:
         bands_fixture()
+    if not args or "c3" in args:
+        c3_fixtures()

@@ -32,3 +32,11 @@ def load_traj(name):
             else:
                 cur.append([float(x) for x in line.split()])
     return np.array(eng), np.array(conv)
+
+
+def traj_meta(name):
+    """The key=value pairs of a trajectory fixture's first header line
+    (ntheta, nlayers, numk, nfilling, xi, itmax, start, ...)."""
+    with open(os.path.join(GOLDEN, name + ".traj.txt")) as f:
+        head = f.readline()
+    return dict(kv.split("=") for kv in head.split(": ", 1)[1].split())

@@ -12,13 +12,14 @@ import numpy as np
 import pytest
 from scipy.special import k0
 
-from golden_io import GOLDEN, case_params, load_case, load_traj
+from golden_io import GOLDEN, case_params, load_case, load_traj, traj_meta
 from forge_amd.checkpoint import checkpoint_name, read_checkpoint, write_checkpoint
 from forge_amd.seed import seeded_rho, seeded_site_density
 from oracle.pyoracle import System
 
 HARNESS_CASES = ["t1k3", "t2k3", "t2k3xi10", "t3k6", "tri1k3"]
 TRAJ_CASES = ["t1k3", "t1k3s", "t2k3", "t2k3s", "t2k3xi10", "t3k6", "tri1k3", "t5k6s"]
+C3_CASES = ["t1k3_c3", "tri1k3_c3", "t2k3_c3", "t1k6_c3"]
 
 _systems = {}
 
@@ -147,6 +148,57 @@ def test_scf_trajectory_matches_reference(case):
     np.testing.assert_allclose(traj[:, 6], Cv[:, 1], rtol=0, atol=1e-6)
     np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
     np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
+
+
+def c3_system(case):
+    m = traj_meta(case)
+    return System(int(m["ntheta"]), int(m["numk"]), float(m["nfilling"]), nlayers=int(m["nlayers"]),
+                  xi_nm=float(m["xi"].rstrip("nm"))), int(m["itmax"])
+
+
+@pytest.mark.parametrize("case", C3_CASES)
+def test_c3_trajectory_matches_reference(case):
+    """nenforceC3 = 1 (Solve_C3 / GetFock_C3): the reference program built with
+    that setting, from its own first diagonalisation.  A flipped corner phase
+    or the inverse site permutation moves these energies by tens of eV."""
+    s, itmax = c3_system(case)
+    its, traj, _, _ = s.scf(itmax, c3=True)
+    E, Cv = load_traj(case)
+    assert its == itmax == len(E)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=2e-9)
+    np.testing.assert_allclose(traj[:, 6], Cv[:, 1], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
+
+
+@pytest.mark.parametrize("case", ["t1k3_c3", "tri1k3_c3", "t2k3_c3"])
+def test_c3_pairs_is_the_rotation(case):
+    """nC3pairs is a permutation of order 3 within each layer and sublattice
+    half, maps every non-corner site onto its +120 degree image, and fixes the
+    two corner sites of each layer."""
+    s, _ = c3_system(case)
+    perm, cnt = s.c3_pairs()
+    n, per = s.n, s.n // s.nlayers
+    assert cnt == n and sorted(perm) == list(range(n))
+    assert (perm[perm[perm]] == np.arange(n)).all()
+    assert (perm // (per // 2) == np.arange(n) // (per // 2)).all()
+    c, h = -0.5, np.sqrt(3) / 2
+    xy = s.Coords[:, :2]
+    img = np.stack([c * xy[:, 0] - h * xy[:, 1], h * xy[:, 0] + c * xy[:, 1]], 1)
+    corners = {l * per + per // 2 - 1 for l in range(s.nlayers)} | {l * per + per - 1 for l in range(s.nlayers)}
+    for i in range(n):
+        if i in corners:
+            assert perm[i] == i
+        else:
+            assert np.hypot(*(xy[perm[i]] - img[i])) < 1e-9
+
+
+def test_c3_rejects_a_grid_without_the_invariant_points():
+    """numk % 3 != 0: the reference's invariant-momentum test (fixed flat
+    indices of Gamma, K, K') no longer names the C3-invariant momenta."""
+    s = System(1, 4, 1.0)
+    its, _, _, _ = s.scf(1, c3=True)
+    assert its < 0
 
 
 def test_ft_closed_forms():
