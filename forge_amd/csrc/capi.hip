@@ -174,6 +174,23 @@ struct forge_ctx {
   int it = 0;
   double tm[16] = {};
   hipEvent_t ev[16];
+  // C3-reduced k-set (nenforceC3 = 1): tables built by c3_configure from
+  // RotateLayers and nC3pairs; c3 = the SCF loop uses Solve_C3 / GetFock_C3
+  bool c3 = false, c3_ready = false;
+  std::vector<int64_t> c3_key;
+  std::vector<i64> c3_reps, c3_k2, c3_k3;  // representatives; per flat k its partners
+  std::vector<char> c3_inv;                // per flat k: C3-invariant
+  std::vector<C3Col> c3_expand, c3_psi;
+  std::vector<C3Slot> c3_slots;            // per representative
+  std::vector<double> c3_wph_h;
+  C3Geom c3g{};
+  int* c3_perm = nullptr;
+  z2* c3_tab = nullptr;
+  double* c3_ph_asm = nullptr;
+  C3Col *c3_dexpand = nullptr, *c3_dpsi = nullptr;
+  z2* c3_Psi = nullptr;
+  double* c3_wph = nullptr;
+  i64 c3_cap = 0;  // columns of c3_Psi
 };
 
 static double total_energy(const forge_params& P, const double* e) {
@@ -351,7 +368,8 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
   void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
-                  c->S};
+                  c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
+                  c->c3_Psi, c->c3_wph};
   for (void* q : ptrs)
     if (q) hipFree(q);
   tridiag_release(c->trd);
@@ -508,6 +526,206 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   if (info) *info = fails;
   hipFree(H); hipFree(tw); hipFree(dw); hipFree(Zq); hipFree(ev); hipFree(Cv); hipFree(bw);
   hipStreamDestroy(s);
+  return fails;
+}
+
+// ---------------------------------------------------------------- C3-reduced k-set
+// Tables of Solve_C3 / GetFock_C3 (HartreeFock.f90:131-398, 564-881) for one
+// RotateLayers / nC3pairs (1-based, as C3_RelatedPoints returns it), rebuilt
+// only when those change.  Corner factor sets per flat momentum k (6 each):
+//   6k+0  f: first corner e^{i rot t1}, t1 = 2 pi (i1+i2)/numk (C3 block image)
+//   6k+1  u = f g, g: first corner e^{-i rot t2}, t2 = 2 pi i1/numk (C3^2 image)
+//   6k+2, 6k+3  conj(f), conj(u): the images of a state vector
+//   6k+4  zphase of the C3^2 partner's eigenvectors, e^{i rot 2 pi (j1+j2)/numk}
+//   6k+5  zphase of the C3 partner's eigenvectors, e^{i rot 2 pi j2/numk}
+// ((j1, j2) the partner's components, fphase = 0); second corners conjugate.
+static int c3_configure(forge_ctx* c, const int64_t* rot, const int64_t* pairs, const char* who) {
+  const i64 n = c->n, Nk = c->Nk, numk = c->numk, nnc = c->nnc, nb = c->nb;
+  const i64 nl = c->P.nlayers;
+  if (!rot || !pairs) { set_error(std::string(who) + ": null RotateLayers or nC3pairs"); return -2; }
+  if (numk % 3 != 0) {
+    set_error(std::string(who) + ": numk must be a multiple of 3 (the C3-invariant momenta are "
+              "the flat indices of Gamma, K and K')");
+    return -2;
+  }
+  if (nl < 1 || nl > 64 || n % nl != 0 || (n / nl) % 2 != 0) {
+    set_error(std::string(who) + ": layers must split ndim into two equal sublattice halves");
+    return -2;
+  }
+  for (i64 k = 0; k < Nk; ++k)
+    if (c->nMomComp[k] * numk + c->nMomComp[k + Nk] != k) {
+      set_error(std::string(who) + ": momenta must be in SampleBZ order (flat index i1*numk + i2)");
+      return -2;
+    }
+  std::vector<int64_t> key(rot, rot + nl);
+  key.insert(key.end(), pairs, pairs + n);
+  if (c->c3_ready && key == c->c3_key) return 0;
+  const i64 per = n / nl, half = per / 2;
+  std::vector<int> p(2 * n);
+  std::vector<char> seen(n, 0);
+  for (i64 i = 0; i < n; ++i) {
+    const int64_t j = pairs[i] - 1;
+    if (j < 0 || j >= n || seen[j]) {
+      set_error(std::string(who) + ": nC3pairs is not a permutation of 1..ndim");
+      return -2;
+    }
+    seen[j] = 1;
+    p[i] = (int)j;
+  }
+  for (i64 l = 0; l < nl; ++l)
+    if (p[l * per + half - 1] != l * per + half - 1 || p[l * per + per - 1] != l * per + per - 1) {
+      set_error(std::string(who) + ": nC3pairs must fix the two corner sites of each layer");
+      return -2;
+    }
+  for (i64 i = 0; i < n; ++i) p[n + i] = p[p[i]];
+  // orbits, HartreeFock.f90:152-189
+  c->c3_k2.assign(Nk, 0);
+  c->c3_k3.assign(Nk, 0);
+  c->c3_inv.assign(Nk, 0);
+  c->c3_reps.clear();
+  std::vector<char> incl(Nk, 0);
+  for (i64 k = 0; k < Nk; ++k) {
+    const i64 i1 = c->nMomComp[k], i2 = c->nMomComp[k + Nk];
+    const i64 m = ((-i1 - i2) % numk + numk) % numk;
+    c->c3_k2[k] = m * numk + i1 % numk;
+    c->c3_k3[k] = (i2 % numk) * numk + m;
+    c->c3_inv[k] = k == 0 || k == numk / 3 * (numk + 1) || k == 2 * numk / 3 * (numk + 1);
+  }
+  for (i64 k = 0; k < Nk; ++k) {
+    if (incl[k]) continue;
+    c->c3_reps.push_back(k);
+    incl[k] = incl[c->c3_k2[k]] = incl[c->c3_k3[k]] = 1;
+  }
+  const i64 R = (i64)c->c3_reps.size();
+  // corner factors
+  std::vector<z2> tab((size_t)6 * Nk * nl * 2);
+  auto put = [&](i64 set, double a) {
+    for (i64 l = 0; l < nl; ++l) {
+      const double r = (rot[l] == 1 || rot[l] == -1) ? (double)rot[l] : 0.0;
+      tab[(set * nl + l) * 2] = r != 0.0 ? mk(std::cos(r * a), std::sin(r * a)) : mk(1.0, 0.0);
+      tab[(set * nl + l) * 2 + 1] = r != 0.0 ? mk(std::cos(r * a), -std::sin(r * a)) : mk(1.0, 0.0);
+    }
+  };
+  for (i64 k = 0; k < Nk; ++k) {
+    const i64 i1 = c->nMomComp[k], i2 = c->nMomComp[k + Nk];
+    const i64 k2 = c->c3_k2[k], k3 = c->c3_k3[k];
+    const double t1 = 2.0 * PI * (double)(i1 + i2) / (double)numk;
+    const double t2 = 2.0 * PI * (double)i1 / (double)numk;
+    put(6 * k + 0, t1);
+    put(6 * k + 1, t1 - t2);
+    put(6 * k + 2, -t1);
+    put(6 * k + 3, -(t1 - t2));
+    put(6 * k + 4, 2.0 * PI * (double)(c->nMomComp[k2] + c->nMomComp[k2 + Nk]) / (double)numk);
+    put(6 * k + 5, 2.0 * PI * (double)c->nMomComp[k3 + Nk] / (double)numk);
+  }
+  // partners' window columns and the per-representative bulk slots
+  c->c3_expand.clear();
+  c->c3_slots.clear();
+  for (i64 r = 0; r < R; ++r) {
+    const i64 k = c->c3_reps[r];
+    const i64 k2 = c->c3_k2[k], k3 = c->c3_k3[k];
+    C3Slot sl;
+    sl.ph0 = nullptr;  // device pointers set below
+    sl.fset = (int)(6 * k + 0);
+    sl.uset = (int)(6 * k + 1);
+    sl.w = c->c3_inv[k] ? 1.0 / 3.0 : 1.0;
+    c->c3_slots.push_back(sl);
+    if (c->c3_inv[k]) continue;
+    for (i64 b = 0; b < nb; ++b) {
+      c->c3_expand.push_back(C3Col{b + nb * k, b + nb * k2, 2, (int)(6 * k + 4)});
+      c->c3_expand.push_back(C3Col{b + nb * k, b + nb * k3, 1, (int)(6 * k + 5)});
+    }
+  }
+  // device tables (allocated once per context; sizes depend only on n, Nk)
+  if (!c->c3_perm) {
+    c->c3_cap = nb * (Nk + 6);
+    CK(dalloc((void**)&c->c3_perm, sizeof(int) * 2 * n));
+    CK(dalloc((void**)&c->c3_tab, sizeof(z2) * tab.size()));
+    CK(dalloc((void**)&c->c3_ph_asm, sizeof(double) * Nk * nnc * 2));
+    CK(dalloc((void**)&c->c3_dexpand, sizeof(C3Col) * std::max<i64>(2 * nb * Nk, 1)));
+    CK(dalloc((void**)&c->c3_dpsi, sizeof(C3Col) * c->c3_cap));
+    CK(dalloc((void**)&c->c3_Psi, sizeof(z2) * n * c->c3_cap));
+    CK(dalloc((void**)&c->c3_wph, sizeof(double) * c->c3_cap * nnc * 2));
+  }
+  std::vector<double> pa((size_t)R * nnc * 2);
+  for (i64 r = 0; r < R; ++r)
+    std::memcpy(&pa[(size_t)r * nnc * 2], &c->h_ph_asm[(size_t)c->c3_reps[r] * nnc * 2],
+                sizeof(double) * nnc * 2);
+  for (i64 r = 0; r < R; ++r) {
+    const i64 k = c->c3_reps[r];
+    c->c3_slots[r].ph0 = c->ph_cell + k * nnc * 2;
+    c->c3_slots[r].ph1 = c->ph_cell + c->c3_k3[k] * nnc * 2;
+    c->c3_slots[r].ph2 = c->ph_cell + c->c3_k2[k] * nnc * 2;
+  }
+  FORGE_HIP(hipMemcpy(c->c3_perm, p.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice));
+  FORGE_HIP(hipMemcpy(c->c3_tab, tab.data(), sizeof(z2) * tab.size(), hipMemcpyHostToDevice));
+  FORGE_HIP(hipMemcpy(c->c3_ph_asm, pa.data(), sizeof(double) * pa.size(), hipMemcpyHostToDevice));
+  if (!c->c3_expand.empty())
+    FORGE_HIP(hipMemcpy(c->c3_dexpand, c->c3_expand.data(), sizeof(C3Col) * c->c3_expand.size(),
+                        hipMemcpyHostToDevice));
+  c->c3g = C3Geom{n, per, (int)nl, c->c3_perm, c->c3_tab};
+  c->c3_key = key;
+  c->c3_ready = true;
+  return 0;
+}
+
+// Solve_C3 on the device: assembly, eigensolver and bulk density for the
+// orbit representatives only, then the partners' window columns as images.
+static int solve_device_c3(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
+                           std::vector<double>& hband, bool timed) {
+  const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nU = c->nU, nb = c->nb;
+  const i64 R = (i64)c->c3_reps.size();
+  hband.assign(nb * Nk, 0.0);
+  std::vector<double> hev(n * c->Kb);
+  int fails = 0;
+  for (i64 r0 = 0; r0 < R; r0 += c->Kb) {
+    const int K = (int)std::min<i64>(c->Kb, R - r0);
+    if (timed) FORGE_HIP(hipEventRecord(c->ev[0], c->s));
+    CK(assemble(n, (int)nnc, K, rho_dev, c->V, c->T, pot_dev, c->bias, c->P.alpha,
+                c->c3_ph_asm + r0 * nnc * 2, c->H, c->ldH, (size_t)c->ldH * n, c->s));
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[1], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[1]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      c->tm[0] += a;
+      c->tm[7] += 1;
+      c->tm[8] += a;
+    }
+    int f = eig_batch(c, K, nU, timed);
+    if (f < 0) return f;
+    fails += f;
+    if (timed) FORGE_HIP(hipEventRecord(c->ev[6], c->s));
+    CK(bulk_density_c3(c->c3g, K, c->Cv, n, (size_t)n * nU, c->nL - 1, (int)nnc,
+                       c->c3_slots.data() + r0, c->rho_new, r0 > 0, c->s));
+    for (int kb = 0; kb < K; ++kb) {
+      hipLaunchKernelGGL(k_copy_window, dim3(cdiv(n * nb, 256), 1), dim3(256), 0, c->s, n, nb,
+                         c->nL - 1, 1, c->c3_reps[r0 + kb], c->Cv + (size_t)kb * n * nU, (size_t)n,
+                         (size_t)n * nU, c->Cwin);
+      FORGE_LAUNCH();
+    }
+    if (timed) {
+      FORGE_HIP(hipEventRecord(c->ev[7], c->s));
+      FORGE_HIP(hipEventSynchronize(c->ev[7]));
+      float a;
+      hipEventElapsedTime(&a, c->ev[6], c->ev[7]);
+      c->tm[4] += a;
+    }
+    FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    for (int kb = 0; kb < K; ++kb) {
+      const i64 k = c->c3_reps[r0 + kb];
+      for (i64 b = 0; b < nb; ++b) {
+        const double e = hev[kb * n + (c->nL - 1) + b];
+        hband[b + nb * k] = e;
+        hband[b + nb * c->c3_k2[k]] = e;
+        hband[b + nb * c->c3_k3[k]] = e;
+      }
+    }
+  }
+  // partners' eigenvectors (HartreeFock.f90:212-277): sources are the
+  // representatives' columns, destinations the partners' (disjoint)
+  CK(c3_columns(c->c3g, (i64)c->c3_expand.size(), c->c3_dexpand, c->Cwin, c->Cwin, c->s));
   return fails;
 }
 
@@ -715,6 +933,104 @@ extern "C" int forge_getfock(forge_ctx* c, forge_z* zFock, const forge_z* zSorte
   return 0;
 }
 
+// ---------------------------------------------------------------- C3 entry points
+
+static int c3_window(forge_ctx* c, i64 S, const std::vector<i64>& cols,
+                     const std::vector<i64>& kof, const double* wph, const z2* src,
+                     double scale);
+
+extern "C" int64_t forge_c3_pairs(const double* Coords, int64_t ndim, int64_t nlayers,
+                                  int64_t* nC3pairs) {
+  if (!Coords || !nC3pairs || ndim < 2 || nlayers < 1 || ndim % nlayers != 0 ||
+      (ndim / nlayers) % 2 != 0) {
+    set_error("forge_c3_pairs: invalid argument");
+    return -2;
+  }
+  const i64 cnt = c3_pairs(Coords, ndim, nlayers, nC3pairs);
+  for (i64 i = 0; i < ndim; ++i)
+    if (nC3pairs[i] >= 0) nC3pairs[i] += 1;  // Fortran numbering; unmatched sites stay -1
+  return cnt;
+}
+
+extern "C" int forge_solve_c3(forge_ctx* c, forge_z* zFockBulk, const forge_z* zFock,
+                              const double* Potential, double* Bands, forge_z* zEigenvectors,
+                              const int64_t* RotateLayers, const int64_t* nC3pairs) {
+  if (!c || !zFockBulk || !zFock || !Potential || !Bands || !zEigenvectors) {
+    set_error("forge_solve_c3: null argument");
+    return -2;
+  }
+  CK(c3_configure(c, RotateLayers, nC3pairs, "forge_solve_c3"));
+  const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nb = c->nb;
+  const size_t nn = (size_t)n * n;
+  FORGE_HIP(hipMemcpyAsync(c->rho, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  FORGE_HIP(hipMemcpyAsync(c->db, Potential, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  std::vector<double> hb;
+  int f = solve_device_c3(c, c->rho, c->db, hb, false);
+  if (f < 0) return f;
+  FORGE_HIP(hipMemcpyAsync(zFockBulk, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(zEigenvectors, c->Cwin, sizeof(z2) * n * nb * Nk, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  std::memcpy(Bands, hb.data(), sizeof(double) * nb * Nk);
+  c->scf_ready = false;
+  return f;
+}
+
+extern "C" int forge_getfock_c3(forge_ctx* c, forge_z* zFock, const forge_z* zSorted,
+                                const int64_t* nSortedMomenta, double DegFactor, int64_t OccStates,
+                                int64_t PartOccStates, const int64_t* RotateLayers,
+                                const int64_t* nC3pairs) {
+  if (!c || !zFock || !zSorted || !nSortedMomenta) { set_error("forge_getfock_c3: null argument"); return -2; }
+  const i64 n = c->n, nnc = c->nnc, numk = c->numk, N = c->nb * c->Nk;
+  if (OccStates < 0 || OccStates > N || PartOccStates < 0 || PartOccStates > OccStates) {
+    set_error("forge_getfock_c3: invalid state counts");
+    return -2;
+  }
+  for (i64 s = 0; s < OccStates; ++s)
+    if (nSortedMomenta[s] < 0 || nSortedMomenta[s] >= numk || nSortedMomenta[s + N] < 0 ||
+        nSortedMomenta[s + N] >= numk) {
+      set_error("forge_getfock_c3: momentum component out of range");
+      return -2;
+    }
+  CK(c3_configure(c, RotateLayers, nC3pairs, "forge_getfock_c3"));
+  const size_t nn = (size_t)n * n;
+  std::vector<double> wph;
+  std::vector<i64> cols(OccStates), kof(OccStates);
+  for (i64 s = 0; s < OccStates; ++s) {
+    std::complex<double> w = 1.0;
+    if (s >= OccStates - PartOccStates)
+      w = c->opt.occ_mode == 0 ? std::complex<double>((double)(float)DegFactor, 8.0)
+                               : std::complex<double>(DegFactor, 0.0);
+    i64 i1 = nSortedMomenta[s], i2 = nSortedMomenta[s + N];
+    cols[s] = s;
+    kof[s] = i1 * numk + i2;
+    for (i64 cc = 0; cc < nnc; ++cc) {
+      double a = -2.0 * PI * (double)(i1 * c->u1[cc] + i2 * c->u1[cc] + i2 * c->u2[cc]) / (double)numk;
+      std::complex<double> z = std::complex<double>(std::cos(a), std::sin(a)) * w;
+      wph.push_back(z.real());
+      wph.push_back(z.imag());
+    }
+  }
+  FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  if (OccStates > 0)
+    FORGE_HIP(hipMemcpyAsync(c->Psi, zSorted, sizeof(z2) * n * OccStates, hipMemcpyHostToDevice, c->s));
+  CK(c3_window(c, OccStates, cols, kof, wph.data(), c->Psi, 1.0));
+  FORGE_HIP(hipMemcpyAsync(zFock, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_scf_set_c3(forge_ctx* c, const int64_t* RotateLayers, const int64_t* nC3pairs) {
+  if (!c) { set_error("forge_scf_set_c3: null context"); return -2; }
+  if (!RotateLayers && !nC3pairs) {
+    c->c3 = false;
+    return 0;
+  }
+  CK(c3_configure(c, RotateLayers, nC3pairs, "forge_scf_set_c3"));
+  c->c3 = true;
+  return 0;
+}
+
 // Energy pieces of a = rho (dens da) and optionally b = rho_new (dens db):
 // pieces[0..4] = K_a, K_b, F_aa, F_ab, F_bb; [5..7] = H_aa, H_ab, H_bb
 // (d.L.d), [8..10] = U_aa, U_ab, U_bb; [11..12] bias terms of a, b.
@@ -814,9 +1130,51 @@ extern "C" int forge_scf_set_options(forge_ctx* c, const forge_scf_options* opt)
   return 0;
 }
 
+// GetFock_C3's window columns (HartreeFock.f90:564-881): a state at a
+// C3-invariant momentum becomes three columns, itself and its two images,
+// each with a third of its weight; every other state is copied as is.
+// cols[s] is the source column of state s, wph its nnc weights.
+static int c3_window(forge_ctx* c, i64 S, const std::vector<i64>& cols,
+                     const std::vector<i64>& kof, const double* wph, const z2* src,
+                     double scale) {
+  const i64 n = c->n, nnc = c->nnc;
+  c->c3_psi.clear();
+  c->c3_wph_h.clear();
+  for (i64 s = 0; s < S; ++s) {
+    const i64 k = kof[s];
+    const int copies = c->c3_inv[k] ? 3 : 1;
+    for (int r = 0; r < copies; ++r) {
+      const int set = r == 0 ? -1 : (int)(6 * k + 1 + r);  // conj(f), conj(u)
+      c->c3_psi.push_back(C3Col{cols[s], (i64)c->c3_psi.size(), r, set});
+      for (i64 q = 0; q < 2 * nnc; ++q) c->c3_wph_h.push_back(wph[s * nnc * 2 + q] / (double)copies);
+    }
+  }
+  const i64 S3 = (i64)c->c3_psi.size();
+  if (S3 > c->c3_cap) { set_error("c3_window: more window columns than allocated"); return -2; }
+  if (S3 > 0) {
+    FORGE_HIP(hipMemcpyAsync(c->c3_dpsi, c->c3_psi.data(), sizeof(C3Col) * S3, hipMemcpyHostToDevice, c->s));
+    FORGE_HIP(hipMemcpyAsync(c->c3_wph, c->c3_wph_h.data(), sizeof(double) * c->c3_wph_h.size(),
+                             hipMemcpyHostToDevice, c->s));
+    CK(c3_columns(c->c3g, S3, c->c3_dpsi, src, c->c3_Psi, c->s));
+  }
+  CK(window_density(n, (int)nnc, S3, c->c3_Psi, c->c3_wph, c->rho_new, scale, c->s));
+  // the descriptor vectors are reused by the next call: wait for the copies
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
 // window density into c->rho_new (which holds the bulk): rho_new = (bulk + win) / Nk
 static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation& occ) {
   occupations(c, band, occ);
+  if (c->c3) {
+    const i64 S = (i64)occ.idx.size();
+    std::vector<i64> cols(S), kof(S);
+    for (i64 s = 0; s < S; ++s) {
+      cols[s] = occ.idx[s];
+      kof[s] = occ.idx[s] / c->nb;
+    }
+    return c3_window(c, S, cols, kof, occ.wph.data(), c->Cwin, 1.0 / (double)c->Nk);
+  }
   const i64 n = c->n, S = (i64)occ.idx.size();
   if (S > 0) {
     FORGE_HIP(hipMemcpyAsync(c->gidx, occ.idx.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->s));
@@ -844,7 +1202,8 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
     // first diagonalisation: rho = 0, Potential = 0 (Main.f90:238-282)
     FORGE_HIP(hipMemsetAsync(c->pot, 0, sizeof(double) * n, c->s));
     std::vector<double> band;
-    fails = solve_device(c, nullptr, c->pot, band, false);
+    fails = c->c3 ? solve_device_c3(c, nullptr, c->pot, band, false)
+                  : solve_device(c, nullptr, c->pot, band, false);
     if (fails < 0) return fails;
     Occupation occ;
     CK(window_step(c, band, occ));
@@ -888,7 +1247,8 @@ static int guess_diagonalisation(forge_ctx* c) {
   const size_t tot = (size_t)c->nnc * c->n * c->n;
   CK(init_state_of_rho(c));
   std::vector<double> band;
-  const int fails = solve_device(c, c->rho, c->pot, band, false);
+  const int fails = c->c3 ? solve_device_c3(c, c->rho, c->pot, band, false)
+                          : solve_device(c, c->rho, c->pot, band, false);
   if (fails < 0) return fails;
   Occupation occ;
   CK(window_step(c, band, occ));
@@ -981,7 +1341,9 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   std::vector<double> band;
   const bool mixed = c->opt.precision == 1 && !c->mixed_done;
   if (mixed) c->tm[12] += 1;
-  int fails = solve_device(c, c->rho, c->pot, band, timed, eig_ready);
+  if (c->c3 && eig_ready) { set_error("scf_step: the C3-reduced k-set is not batched in sweeps"); return -2; }
+  int fails = c->c3 ? solve_device_c3(c, c->rho, c->pot, band, timed)
+                    : solve_device(c, c->rho, c->pot, band, timed, eig_ready);
   if (fails < 0) return fails;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
   Occupation occ;
@@ -1150,6 +1512,11 @@ extern "C" int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sw
   for (i64 i = 0; i < nsys; ++i) {
     forge_ctx* c = ctxs[i];
     if (!c) { delete w; set_error("forge_sweep_create: null context"); return -2; }
+    if (c->c3) {
+      delete w;
+      set_error("forge_sweep_create: the C3-reduced k-set is not batched in sweeps");
+      return -2;
+    }
     if (c->Kb < c->Nk) {
       delete w;
       set_error("forge_sweep_create: every system must solve all its k-points in one batch");

@@ -97,4 +97,25 @@ int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int
 int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
                    double scale, hipStream_t s);
 
+// c3.hip: the C3-reduced k-set (nenforceC3 = 1).
+struct C3Geom {
+  i64 n, per;        // sites, sites per layer
+  int nl;            // layers
+  const int* perm;   // device [2][n]: p, then p o p (0-based)
+  const z2* tab;     // device corner factors [set][nl][2]
+};
+struct C3Col {       // one image column: dst(:, dst) = src(p^r(i), src) * corner(set, i)
+  i64 src, dst;
+  int r, set;
+};
+struct C3Slot {      // one orbit representative of a batch
+  const double *ph0, *ph1, *ph2;  // device cell phases [nnc][2] of k, its C3 and C3^2 partners
+  int fset, uset;                 // corner factor sets of the C3 and C3^2 block images
+  double w;                       // 1, or 1/3 at a C3-invariant momentum
+};
+int c3_columns(const C3Geom& g, i64 ncol, const C3Col* d, const z2* src, z2* dst, hipStream_t s);
+int bulk_density_c3(const C3Geom& g, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
+                    const C3Slot* slot, z2* rho, bool accumulate, hipStream_t s);
+i64 c3_pairs(const double* C, i64 n, i64 nl, i64* perm);
+
 }  // namespace forge

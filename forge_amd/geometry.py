@@ -109,6 +109,7 @@ class Geometry:
     nMomentaComponents: np.ndarray  # (Nk, 2)
     nUnitCell_1: np.ndarray
     nUnitCell_2: np.ndarray
+    RotateLayers: np.ndarray = None  # (nlayers,) +-1 per layer (Setup.f90:8)
 
     @property
     def ndim(self):
@@ -166,4 +167,4 @@ def build_geometry(ntheta: int, nlayers: int = 2, numk: int = 3, numI: int = 1,
     t2r = rot_apply(R, t2)
     tn = np.asfortranarray(np.stack([t1r, t2r, t2r - t1r], 1))
     return Geometry(ntheta, nlayers, numk, numI, np.asfortranarray(C), tn, g1, g12, mom,
-                    np.asfortranarray(comp), u1, u2)
+                    np.asfortranarray(comp), u1, u2, np.array(rotate, dtype=np.int64))

@@ -139,6 +139,59 @@ class Engine:
                                        int(OccStates), int(PartOccStates)), "forge_getfock")
         return out
 
+    # -- C3-reduced k-set (nenforceC3 = 1) -----------------------------------
+    def c3_pairs(self):
+        """nC3pairs of C3_RelatedPoints (Geometry.f90:1172-1223), 1-based."""
+        g = self.geom
+        out = np.zeros(self.n, np.int64)
+        cnt = self.lib.forge_c3_pairs(_p(g.Coords), self.n, g.nlayers, _p(out))
+        if cnt < 0:
+            raise RuntimeError("forge_c3_pairs: " + L.last_error())
+        if cnt != self.n:
+            raise ValueError(f"cell is not C3-symmetric ({cnt} of {self.n} sites matched)")
+        return out
+
+    def _c3_tables(self):
+        if getattr(self, "_c3", None) is None:
+            rot = np.ascontiguousarray(self.geom.RotateLayers, dtype=np.int64)
+            self._c3 = (rot, self.c3_pairs())
+        return self._c3
+
+    def solve_c3(self, zFock, Potential):
+        """Solve_C3 (HartreeFock.f90:131-398): one diagonalisation per C3 orbit
+        of momenta; (zFockBulk unnormalised, Bands, zEigenvectors) for all Nk."""
+        rot, pairs = self._c3_tables()
+        bulk = np.zeros((self.n, self.n, self.nnc), np.complex128, order="F")
+        bands = np.zeros((self.nb, self.Nk), order="F")
+        vecs = np.zeros((self.n, self.nb, self.Nk), np.complex128, order="F")
+        rho, pot = _f(zFock, np.complex128), _f(Potential, np.float64)
+        rc = L.check(self.lib.forge_solve_c3(self.ctx, _p(bulk), _p(rho), _p(pot), _p(bands), _p(vecs),
+                                             _p(rot), _p(pairs)), "forge_solve_c3")
+        if rc > 0:
+            print("zheev failed", rc)
+        return bulk, bands, vecs
+
+    def getfock_c3(self, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, OccStates,
+                   PartOccStates):
+        """GetFock_C3 (HartreeFock.f90:564-881): returns zFock + window density,
+        states at Gamma, K, K' symmetrised over their C3 images."""
+        rot, pairs = self._c3_tables()
+        out = np.array(zFock, dtype=np.complex128, order="F", copy=True)
+        vecs = _f(zSortedEigenvectors, np.complex128)
+        mom = _f(nSortedMomenta, np.int64)
+        L.check(self.lib.forge_getfock_c3(self.ctx, _p(out), _p(vecs), _p(mom), float(DegFactor),
+                                          int(OccStates), int(PartOccStates), _p(rot), _p(pairs)),
+                "forge_getfock_c3")
+        return out
+
+    def set_c3(self, on=True):
+        """nenforceC3 for the device-resident SCF loop (Main.f90:244-276, 375-505)."""
+        if on:
+            rot, pairs = self._c3_tables()
+            L.check(self.lib.forge_scf_set_c3(self.ctx, _p(rot), _p(pairs)), "forge_scf_set_c3")
+        else:
+            L.check(self.lib.forge_scf_set_c3(self.ctx, None, None), "forge_scf_set_c3")
+
     def energies(self, zFock, dens):
         """(Ekin, EF, EH, EU) of GetKineticEnergy/GetFockEnergy/GetHartreeHubbardEnergy."""
         out = np.zeros(4)

@@ -50,6 +50,10 @@ _SIGS = {
     "forge_energies": (C.c_int, [vp, vp, vp, vp]),
     "forge_hartree_hubbard": (C.c_int, [vp, vp, vp, vp]),
     "forge_scf_set_options": (C.c_int, [vp, vp]),
+    "forge_c3_pairs": (i64, [vp, i64, i64, vp]),
+    "forge_solve_c3": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "forge_getfock_c3": (C.c_int, [vp, vp, vp, vp, dbl, i64, i64, vp, vp]),
+    "forge_scf_set_c3": (C.c_int, [vp, vp, vp]),
     "forge_scf_init": (C.c_int, [vp, vp]),
     "forge_scf_init_guess": (C.c_int, [vp, vp]),
     "forge_scf_init_guess_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),

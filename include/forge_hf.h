@@ -113,6 +113,38 @@ int forge_getfock(forge_ctx* ctx, forge_z* zFock, const forge_z* zSortedEigenvec
  * density matrix zFock and site densities dens (Density - DensitySub). */
 int forge_energies(forge_ctx* ctx, const forge_z* zFock, const double* dens, double* out4);
 
+/* ---- C3-reduced k-set (nenforceC3 = 1, Setup.f90:27) ----
+ * RotateLayers: nlayers signs (Setup.f90:8); nC3pairs: ndim site indices,
+ * 1-based, as C3_RelatedPoints returns them.  numk must be a multiple of 3
+ * (the reference's invariant-momentum test names Gamma, K and K' by their
+ * flat indices 1, numk/3 (numk+1) + 1, 2 numk/3 (numk+1) + 1). */
+
+/* C3_RelatedPoints(nC3pairs, Coords, ndim), Geometry.f90:1172-1223, on the
+ * host (no device work): nC3pairs(i) = the site of the same layer and
+ * sublattice half within 1e-3 of the +120 degree image of site i; the two
+ * corner sites of each layer map to themselves.  Returns the number of sites
+ * matched (ndim for a C3-symmetric cell; unmatched entries are -1). */
+int64_t forge_c3_pairs(const double* Coords, int64_t ndim, int64_t nlayers, int64_t* nC3pairs);
+
+/* Solve_C3(zFockBulk, zFock, Potential, alpha, Bands, zEigenvectors, ...,
+ * RotateLayers, ..., nC3pairs), HartreeFock.f90:131-398: H(k) is assembled
+ * and diagonalised only at the first momentum of each C3 orbit; the partners
+ * receive its window bands and permuted, corner-phased eigenvectors, and the
+ * bulk density adds the orbit's block and its two C3 images (a third of each
+ * at Gamma, K, K').  Outputs as forge_solve, for all Nk momenta. */
+int forge_solve_c3(forge_ctx* ctx, forge_z* zFockBulk, const forge_z* zFock,
+                   const double* Potential, double* Bands, forge_z* zEigenvectors,
+                   const int64_t* RotateLayers, const int64_t* nC3pairs);
+
+/* GetFock_C3(zFock, zSortedEigenvectors, ndim, RotateLayers, numb, numk,
+ * DegFactor, OccStates, PartOccStates, ..., nSortedMomenta, nC3pairs),
+ * HartreeFock.f90:564-881: as forge_getfock, except that a state at a
+ * C3-invariant momentum adds (T + C3[T] + C3^2[T]) / 3 of its density T. */
+int forge_getfock_c3(forge_ctx* ctx, forge_z* zFock, const forge_z* zSortedEigenvectors,
+                     const int64_t* nSortedMomenta, double DegFactor, int64_t OccStates,
+                     int64_t PartOccStates, const int64_t* RotateLayers,
+                     const int64_t* nC3pairs);
+
 /* GetHartreeHubbardEnergy(fsumH, fsumU, LongRange, dens1, dens2, ndim),
  * HartreeFock.f90:1066-1092, with the context's device copy of LongRange:
  * out2 = {1/2 (d1+d2).L.(d1+d2), d1.d2}. */
@@ -121,6 +153,12 @@ int forge_hartree_hubbard(forge_ctx* ctx, const double* dens1, const double* den
 /* ---- device-resident SCF (the loop of Main.f90:364-679 on the device) ---- */
 
 int forge_scf_set_options(forge_ctx* ctx, const forge_scf_options* opt);
+
+/* nenforceC3 = 1 for the device-resident loop: forge_scf_init and
+ * forge_scf_run call Solve_C3 / GetFock_C3 where the reference does
+ * (Main.f90:244-250, 273-276, 375-378, 500-505).  Both pointers null: back to
+ * Solve / GetFock.  Not available in sweeps. */
+int forge_scf_set_c3(forge_ctx* ctx, const int64_t* RotateLayers, const int64_t* nC3pairs);
 
 /* Initial state: rho_start (ndim x ndim x nnc) as read from a checkpoint
  * (Main.f90:187-235), or NULL for the first diagonalisation

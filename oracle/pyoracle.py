@@ -47,6 +47,7 @@ def lib():
         L.orc_num_neighbor_cells.restype = i64
         L.orc_zdiagred.restype = C.c_int
         L.orc_solve.restype = C.c_int
+        L.orc_solve_c3.restype = C.c_int
         L.orc_kinetic_energy.restype = dbl
         L.orc_fock_energy.restype = dbl
         L.orc_total_energy.restype = dbl
@@ -179,6 +180,34 @@ class System:
                                ptr(self.nMomentaComponents), ptr(self.u1), ptr(self.u2),
                                ptr(self.tn), C.c_int(nthreads))
         return bulk, Bands, zEig, info
+
+    def solve_c3(self, rho, Potential, nthreads=0):
+        """Solve_C3: (bulk, Bands, zEig, info); info -2 when numk % 3 != 0."""
+        perm, _ = self.c3_pairs()
+        p = self.p
+        Nk = self.numk ** 2
+        nb = p.nUpper - p.nLower + 1
+        bulk = fz((self.n, self.n, self.nnc), np.complex128)
+        Bands = fz((nb, Nk))
+        zEig = fz((self.n, nb, Nk), np.complex128)
+        A = Args()
+        info = lib().orc_solve_c3(self.pp, ptr(self.rot), ptr(perm), ptr(bulk),
+                                  A(rho, np.complex128), A(Potential, float), ptr(Bands), ptr(zEig),
+                                  ptr(self.Coords), ptr(self.MomentaValues),
+                                  ptr(self.nMomentaComponents), ptr(self.u1), ptr(self.u2),
+                                  ptr(self.tn), C.c_int(nthreads))
+        return bulk, Bands, zEig, info
+
+    def getfock_c3(self, zS, nSM, Deg, nOcc, nPart, occ_mode=0, weights=None):
+        perm, _ = self.c3_pairs()
+        rho = fz((self.n, self.n, self.nnc), np.complex128)
+        cnt = 0 if weights is None else weights.size
+        A = Args()
+        lib().orc_getfock_c3(self.pp, ptr(self.rot), ptr(perm), ptr(rho), A(zS, np.complex128),
+                             A(nSM, np.int64), dbl(Deg), i64(nOcc), i64(nPart), ptr(self.Coords),
+                             ptr(self.MomentaValues), ptr(self.u1), ptr(self.u2),
+                             C.c_int(occ_mode), A(weights, float), i64(cnt))
+        return rho
 
     def sort(self, Bands, zEig):
         nb, Nk = Bands.shape

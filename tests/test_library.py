@@ -81,3 +81,24 @@ def test_setup_band_window_matches_reference_params():
     p = Setup(ntheta=1, numk=3, nfilling=1.0).params(4)
     assert (p.ndim, p.nLower, p.nUpper) == (int(prm[0]), int(prm[4]), int(prm[5]))
     assert p.xi == pytest.approx(prm[3], rel=1e-15)
+
+
+@pytest.mark.parametrize("ntheta,nlayers", [(1, 2), (2, 2), (1, 3), (3, 2)])
+def test_c3_pairs_host_matches_oracle(ntheta, nlayers):
+    """forge_c3_pairs (C3_RelatedPoints, host code of the library) against the
+    oracle's restatement, which the reference's C3 trajectories pin."""
+    from oracle.pyoracle import System
+    s = System(ntheta, 3, 1.0, nlayers=nlayers)
+    perm, cnt = s.c3_pairs()
+    g = build_geometry(ntheta, nlayers, 3)
+    out = np.zeros(g.ndim, np.int64)
+    got = L.load().forge_c3_pairs(g.Coords.ctypes.data, g.ndim, nlayers, out.ctypes.data)
+    assert got == cnt == g.ndim
+    assert np.array_equal(out - 1, perm)
+    assert list(g.RotateLayers) == list(s.rot)
+
+
+def test_c3_pairs_rejects_an_odd_layer_split():
+    out = np.zeros(27, np.int64)
+    z = np.zeros((27, 3), order="F")
+    assert L.load().forge_c3_pairs(z.ctypes.data, 27, 2, out.ctypes.data) < 0
