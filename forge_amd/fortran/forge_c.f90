@@ -15,6 +15,7 @@ module forge_c
   public :: forge_getfock, forge_energies, forge_scf_set_options, forge_scf_init
   public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
   public :: forge_sweep_create, forge_sweep_run, forge_sweep_destroy
+  public :: forge_c3_pairs, forge_solve_c3, forge_getfock_c3, forge_scf_set_c3
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -127,6 +128,44 @@ module forge_c
       integer(c_int) :: r
     end function forge_energies
 
+    function forge_c3_pairs(Coords, ndim, nlayers, nC3pairs) bind(C, name="forge_c3_pairs") &
+        result(r)
+      import :: c_double, c_int64_t
+      real(c_double), intent(in) :: Coords(*)
+      integer(c_int64_t), value :: ndim, nlayers
+      integer(c_int64_t), intent(out) :: nC3pairs(*)
+      integer(c_int64_t) :: r
+    end function forge_c3_pairs
+    function forge_solve_c3(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors, RotateLayers, &
+                            nC3pairs) bind(C, name="forge_solve_c3") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: zFockBulk(*), zEigenvectors(*)
+      complex(c_double_complex), intent(in) :: zFock(*)
+      real(c_double), intent(in) :: Potential(*)
+      real(c_double), intent(out) :: Bands(*)
+      integer(c_int64_t), intent(in) :: RotateLayers(*), nC3pairs(*)
+      integer(c_int) :: r
+    end function forge_solve_c3
+    function forge_getfock_c3(ctx, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, &
+                              OccStates, PartOccStates, RotateLayers, nC3pairs) &
+        bind(C, name="forge_getfock_c3") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: zFock(*)
+      complex(c_double_complex), intent(in) :: zSortedEigenvectors(*)
+      integer(c_int64_t), intent(in) :: nSortedMomenta(*), RotateLayers(*), nC3pairs(*)
+      real(c_double), value :: DegFactor
+      integer(c_int64_t), value :: OccStates, PartOccStates
+      integer(c_int) :: r
+    end function forge_getfock_c3
+    function forge_scf_set_c3(ctx, RotateLayers, nC3pairs) bind(C, name="forge_scf_set_c3") &
+        result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: RotateLayers, nC3pairs
+      integer(c_int) :: r
+    end function forge_scf_set_c3
     function forge_scf_set_options(ctx, opt) bind(C, name="forge_scf_set_options") result(r)
       import :: c_int, c_ptr, forge_scf_options_t
       type(c_ptr), value :: ctx

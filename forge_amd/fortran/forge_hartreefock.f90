@@ -24,6 +24,7 @@ module forge_hartreefock
   implicit none
   private
   public :: forge_attach, forge_check, LongRangeInteraction, HamiltonianHartreeFock, Solve, GetFock
+  public :: C3_RelatedPoints, Solve_C3, GetFock_C3
   public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
   public :: OptimalStep, TightBindingHamiltonian, zdiagRed
 
@@ -152,6 +153,55 @@ contains
     call forge_check(forge_getfock(ctx, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, &
                                    OccStates, PartOccStates), 'GetFock')
   end subroutine GetFock
+
+  ! C3_RelatedPoints(nC3pairs, Coords, ndim), Geometry.f90:1172-1223 (host code
+  ! of the library; prints the reference's message when the cell is not
+  ! C3-symmetric).
+  subroutine C3_RelatedPoints(nC3pairs, Coords, ndim_)
+    integer(dp), intent(in) :: ndim_
+    real(dp), intent(in) :: Coords(ndim_, 3)
+    integer(dp), intent(inout) :: nC3pairs(ndim_)
+    integer(dp) :: cnt
+    cnt = forge_c3_pairs(Coords, ndim_, int(nlayers, c_int64_t), nC3pairs)
+    if (cnt < ndim_) write(*, *) 'ERROR C3. cnt=', cnt
+  end subroutine C3_RelatedPoints
+
+  subroutine Solve_C3(zFockBulk, zFock, Potential, alpha_, Bands, zEigenvectors, Coords, &
+                      MomentaValues, nMomentaComponents, nLower_, nUpper_, ndim_, RotateLayers_, &
+                      numk_, Nk_, numNeighborCells, nUnitCell_1, nUnitCell_2, tn, g1, g12, nC3pairs)
+    integer(dp), intent(in) :: ndim_, numk_, Nk_, nLower_, nUpper_, numNeighborCells
+    integer(dp), intent(in) :: nC3pairs(ndim_), RotateLayers_(*)
+    integer(dp), intent(in) :: nMomentaComponents(Nk_, 2)
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: tn(2, 3), MomentaValues(numk_, numk_, 2), Coords(ndim_, 3)
+    real(dp), intent(in) :: Potential(ndim_), g1(2), g12(2), alpha_
+    complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
+    real(dp), intent(out) :: Bands(nUpper_ - nLower_ + 1, Nk_)
+    complex(dp), intent(out) :: zEigenvectors(ndim_, nUpper_ - nLower_ + 1, Nk_)
+    complex(dp), intent(out) :: zFockBulk(ndim_, ndim_, numNeighborCells)
+    integer(c_int) :: rc
+    call need(ndim_, numNeighborCells, 'Solve_C3')
+    rc = forge_solve_c3(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors, RotateLayers_, &
+                        nC3pairs)
+    call forge_check(rc, 'Solve_C3')
+    if (rc > 0) write(*, *) 'zheev failed', rc
+  end subroutine Solve_C3
+
+  subroutine GetFock_C3(zFock, zSortedEigenvectors, ndim_, RotateLayers_, numb_, numk_, DegFactor, &
+                        OccStates, PartOccStates, Coords, MomentaValues, numNeighborCells, &
+                        nUnitCell_1, nUnitCell_2, nSortedMomenta, nC3pairs)
+    integer(dp), intent(in) :: ndim_, numb_, numk_, OccStates, PartOccStates, numNeighborCells
+    integer(dp), intent(in) :: RotateLayers_(*), nC3pairs(ndim_)
+    integer(dp), intent(in) :: nUnitCell_1(numNeighborCells), nUnitCell_2(numNeighborCells)
+    integer(dp), intent(in) :: nSortedMomenta(numb_ * numk_ * numk_, 2)
+    real(dp), intent(in) :: Coords(ndim_, 3), MomentaValues(numk_, numk_, 2), DegFactor
+    complex(dp), intent(in) :: zSortedEigenvectors(ndim_, numb_ * numk_ * numk_)
+    complex(dp), intent(inout) :: zFock(ndim_, ndim_, numNeighborCells)
+    call need(ndim_, numNeighborCells, 'GetFock_C3')
+    call forge_check(forge_getfock_c3(ctx, zFock, zSortedEigenvectors, nSortedMomenta, DegFactor, &
+                                      OccStates, PartOccStates, RotateLayers_, nC3pairs), &
+                     'GetFock_C3')
+  end subroutine GetFock_C3
 
   ! Window states of all k flattened band-major (index (b-1) Nk + k) and
   ! sorted by energy (heap sort); vectors and momenta follow their energies.

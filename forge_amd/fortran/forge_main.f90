@@ -19,7 +19,8 @@
 ! format) occ mix kT beta etol precision switch_de out (trajectory file)
 ! refout (optional: a directory prefix such as output/, which must exist,
 ! receiving the reference's own Energy<parameters> and
-! Convergence<parameters> files, Main.f90:352-358, 616-631).
+! Convergence<parameters> files, Main.f90:352-358, 616-631) c3 (nenforceC3,
+! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -31,6 +32,8 @@ program forge_main
   character(len=16) :: mode = 'reference'
   character(len=512) :: start = 'first', outfile = 'forge_traj.txt', refout = ''
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
+  integer(dp) :: nenforceC3 = 0
+  integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:)
   real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp, switch_de = 0.0_dp
   integer(dp) :: nnc, it, n, nFermiLevel, nOcc, nPart, icount, i, nU
   integer(dp), allocatable :: nUnitCell_1(:), nUnitCell_2(:), nMomentaComponents(:, :)
@@ -63,6 +66,12 @@ program forge_main
   call build_cell(Coords, t1, t2, t3, g1, g12)
   call SampleBZ(nMomentaComponents, MomentaValues, numk, g1, g12)
   tn = reshape([t1, t2, t3], [2, 3])
+  if (nenforceC3 == 1) then
+    ! C3_RelatedPoints (Main.f90:163)
+    allocate(nC3pairs(n), rotC3(nlayers))
+    rotC3 = RotateLayers
+    call C3_RelatedPoints(nC3pairs, Coords, n)
+  end if
   if (trim(mode) == 'geometry') then
     call write_geometry()
   else
@@ -96,6 +105,8 @@ contains
       opt%occ_mode = occ; opt%mix_mode = mix; opt%kT = kT; opt%beta = beta; opt%etol = etol
       opt%precision = prec; opt%switch_de = switch_de
       call forge_check(forge_scf_set_options(ctx, opt), 'forge_scf_set_options')
+      if (nenforceC3 == 1) &
+        call forge_check(forge_scf_set_c3(ctx, c_loc(rotC3), c_loc(nC3pairs)), 'forge_scf_set_c3')
       if (trim(start) == 'first') then
         call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
       else
@@ -133,16 +144,13 @@ contains
 
     ! initial state: first diagonalisation (Main.f90:236-288) unless read
     if (trim(start) == 'first') then
-      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
-                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
-                 nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
+      call solve_k()
       call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
                               SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
                               zEigenvectors(:, :, :, 1), Bands(:, :, 1))
       call fermi_level(SortedEnergies(:, 1), 1_dp)
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
-      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
-                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+      call getfock_k()
       zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
     end if
     do i = 1, n
@@ -162,16 +170,13 @@ contains
     it = 0
     do while (it < itmax)
       it = it + 1
-      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
-                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
-                 nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
+      call solve_k()
       call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
                               SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
                               zEigenvectors(:, :, :, 1), Bands(:, :, 1))
       call fermi_level(SortedEnergies(:, 1), numS)
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
-      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
-                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+      call getfock_k()
       zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
 
       HartreeEnergyIn = HartreeEnergy
@@ -354,6 +359,31 @@ contains
     close(fout)
   end subroutine write_geometry
 
+  ! Solve / Solve_C3 and GetFock / GetFock_C3 as Main.f90:244-250, 273-276,
+  ! 375-378 and 500-505 choose between them.
+  subroutine solve_k()
+    if (nenforceC3 == 1) then
+      call Solve_C3(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
+                    zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+                    nUpper, n, rotC3, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn, g1, g12, nC3pairs)
+    else
+      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
+                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+                 nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
+    end if
+  end subroutine solve_k
+
+  subroutine getfock_k()
+    if (nenforceC3 == 1) then
+      call GetFock_C3(zFock(:, :, :, 1), zSorted(:, :, 1), n, rotC3, numb, numk, DegFactor, nOcc, &
+                      nPart, Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, &
+                      nSortedMomenta(:, :, 1), nC3pairs)
+    else
+      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
+                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+    end if
+  end subroutine getfock_k
+
   subroutine parse_args()
     integer :: k, eq
     character(len=512) :: arg, key, val
@@ -387,6 +417,7 @@ contains
       case ('switch_de'); read(val, *) switch_de
       case ('out'); outfile = val
       case ('refout'); refout = val
+      case ('c3'); read(val, *) nenforceC3
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)
         error stop 1

@@ -7,7 +7,8 @@ GPU: both host modes reproduce the reference's Energy/Convergence rows --
 mode=reference calls the drop-in shims where Main.f90 calls the reference
 routines, mode=device runs the loop device-resident; the seeded case goes
 through the reference's checkpoint format (nRead = 1 path); both also write
-the reference's own Energy/Convergence files (refout=).
+the reference's own Energy/Convergence files (refout=); c3=1 (nenforceC3)
+reproduces the reference built with Solve_C3 / GetFock_C3.
 """
 import os
 import subprocess
@@ -16,7 +17,7 @@ import sys
 import numpy as np
 import pytest
 
-from golden_io import case_params, load_case, load_traj
+from golden_io import case_params, load_case, load_traj, traj_meta
 from forge_amd.build import FORTRAN_EXE
 
 CASES = ["t1k3", "t2k3", "t3k6", "tri1k3", "t2k3xi10"]
@@ -135,3 +136,40 @@ def test_fortran_host_trajectory_matches_reference(case, mode, tmp_path):
     Cf = np.array([[float(x) for x in l.split()] for l in open(fc) if l.strip()])
     np.testing.assert_array_equal(Ef, E)
     np.testing.assert_array_equal(Cf, Cv)
+
+
+def _c3_args(case):
+    m = traj_meta(case)
+    return [f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}", f"numk={m['numk']}",
+            f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}", "c3=1"], int(m["itmax"])
+
+
+@pytest.mark.parametrize("case", ["t1k3_c3", "tri1k3_c3", "t1k6_c3"])
+def test_fortran_c3_pairs_at_setup(case, tmp_path):
+    """c3=1 runs C3_RelatedPoints at set-up (Main.f90:163) through the
+    library's host code; every site is matched, so the reference's
+    'ERROR C3' message does not appear (CPU: geometry mode)."""
+    args, _ = _c3_args(case)
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", *args, f"out={tmp_path / 'g.txt'}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ERROR C3" not in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,mode", [("t1k3_c3", "reference"), ("tri1k3_c3", "device"),
+                                       ("t1k6_c3", "reference")])
+def test_fortran_host_c3_trajectory_matches_reference(case, mode, tmp_path):
+    print("fortran host c3", case, mode, flush=True)
+    args, itmax = _c3_args(case)
+    r = subprocess.run([FORTRAN_EXE, f"mode={mode}", *args, f"itmax={itmax}",
+                        f"out={tmp_path / 'traj.txt'}"], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
+    np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
