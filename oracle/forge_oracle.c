@@ -1027,3 +1027,209 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
   free(S); free(wts); free(V); free(D); free(Din); free(dt);
   return it;
 }
+
+/* ------------------------------------------------------------------ two spins (numS = 2) */
+
+/* SortEnergies_2Spins, HartreeFock.f90:929-1004, with its window
+ * W = [[NP-1, NP-1], [NP+3, NP+3]] (Main.f90:418-421, NP = NeutralityPoint).
+ * S1: numb*Nk x 2 (the two spins' sorted energies); out: 2*numb*Nk x 2
+ * (energy, spin label 1 or 2).  The reference fills its spin-2 lead-in from
+ * row ncount1 of spin 2 while advancing ncount2 (HartreeFock.f90:951-956);
+ * that is kept.  Its merge conditions read one row past the end of a column
+ * before testing the row bound; the bound decides either way, so it is tested
+ * first here. */
+void orc_sort_2spins(const orc_params* p, const double* S1, double* out) {
+  i64 Nk = p->numk * p->numk, N = (p->nUpper - p->nLower + 1) * Nk;
+  i64 NP = (p->ndim / 2 - p->nLower) + 1;
+  i64 w1 = NP - 1, w2 = NP + 3;
+  i64 nb2 = 2 * N, cb = 0, c1 = 1, c2 = 1;
+#define E1(c) S1[(c) - 1]
+#define E2(c) S1[(c) - 1 + N]
+#define PUT(v, lab) do { out[cb] = (v); out[cb + nb2] = (lab); ++cb; } while (0)
+  for (i64 q = 1; q <= (w1 - 1) * Nk; ++q) { PUT(E1(c1), 1.0); ++c1; }
+  for (i64 q = 1; q <= (w1 - 1) * Nk; ++q) { PUT(E2(c1), 2.0); ++c2; }
+  while (c1 <= Nk * (w2 - 1) && c2 <= Nk * (w2 - 1)) {
+    if (E1(c1) <= E2(c2)) {
+      PUT(E1(c1), 1.0); ++c1;
+      while (c1 <= N && E1(c1) <= E2(c2)) { PUT(E1(c1), 1.0); ++c1; }
+      PUT(E2(c2), 2.0); ++c2;
+    } else {
+      PUT(E2(c2), 2.0); ++c2;
+      while (c2 <= N && E2(c2) <= E1(c1)) { PUT(E2(c2), 2.0); ++c2; }
+      PUT(E1(c1), 1.0); ++c1;
+    }
+  }
+  while (c1 <= N) { PUT(E1(c1), 1.0); ++c1; }
+  while (c2 <= N) { PUT(E2(c2), 2.0); ++c2; }
+#undef E1
+#undef E2
+#undef PUT
+}
+
+/* The two-spin Fermi block, Main.f90:416-458: nF = 2 nint(Nk NP + Nk/2 nfilling)
+ * over the merged list, per-spin occupied and partially occupied counts, and
+ * one DegFactor (0 without partially occupied states). */
+void orc_fermi_2spins(const orc_params* p, const double* S2, i64* nF, double* EF, i64* nOcc,
+                      i64* nPart, double* Deg) {
+  i64 Nk = p->numk * p->numk, N2 = 2 * (p->nUpper - p->nLower + 1) * Nk;
+  i64 NP = (p->ndim / 2 - p->nLower) + 1;
+  *nF = 2 * f_nint((double)(Nk * NP) + (double)Nk / 2.0 * p->nfilling);
+  *EF = (S2[*nF - 1] + S2[*nF]) / 2.0;
+  nOcc[0] = nOcc[1] = nPart[0] = nPart[1] = 0;
+  for (i64 q = 0; q < N2; ++q) {
+    int s = S2[q + N2] == 1.0 ? 0 : 1;
+    if ((S2[q] - *EF) < -p->energy_tol) ++nOcc[s];
+    else if (fabs(S2[q] - *EF) < p->energy_tol) { ++nPart[s]; ++nOcc[s]; }
+  }
+  if (nPart[0] != 0 || nPart[1] != 0)
+    *Deg = 1.0 / (double)(nPart[0] + nPart[1]) *
+           (double)(*nF + nPart[0] + nPart[1] - nOcc[0] - nOcc[1]);
+  else
+    *Deg = 0.0;
+}
+
+/* Energies of two spin densities (Main.f90:301-312, HartreeFock.f90:1162-1221
+ * for numS = 2): e = {Ekin1 + Ekin2, EF1 + EF2, EH(d1, d2), EU(d1, d2)}; the
+ * total is alpha EF + Ekin + U EU + alphaH EH (factor 3 - numS = 1). */
+static void energies_2spins(const orc_params* p, const zc* z1, const zc* z2, const double* d1,
+                            const double* d2, const double* C, const double* L, const i64* u1,
+                            const i64* u2, const double* tn, double* e) {
+  e[0] = orc_kinetic_energy(p, z1, C, u1, u2, tn) + orc_kinetic_energy(p, z2, C, u1, u2, tn);
+  e[1] = orc_fock_energy(p, z1, C, u1, u2, tn) + orc_fock_energy(p, z2, C, u1, u2, tn);
+  orc_hartree_hubbard(p, L, d1, d2, &e[2], &e[3]);
+}
+
+static double total_2spins(const orc_params* p, const double* e) {
+  return p->alpha * e[1] + e[0] + p->U * e[3] + p->alphaH * e[2];
+}
+
+/* The SCF sequence of Main.f90:236-679 for numS = 2 (reference weights and
+ * ODA mixing).  rho is n x n x nnc x 2.  rho_start NULL: the first
+ * diagonalisation of spin 1 (Main.f90:238-288) copied to spin 2; else both
+ * spins' densities (the reference's two checkpoint files, Main.f90:201-235).
+ * traj rows as orc_scf: it, E_in, Ekin1+Ekin2, EF1+EF2, EH, EU (input state),
+ * max over spins of max|dDensity|, Step, E_in - E_new, E_Fermi,
+ * nOcc1 + nOcc2, nPart1 + nPart2 (Main.f90:616-639).  Returns the iterations
+ * run, negative on an eigensolver error. */
+i64 orc_scf2(const orc_params* p, const double* C, const double* tn, const double* MomVal,
+             const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
+             const zc* rho_start, i64 itmax, double* traj, zc* rho_out, int nthreads) {
+  i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
+  i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
+  size_t nn = (size_t)n * n, tot = nn * nnc;
+  zc* bulk = (zc*)malloc(sizeof(zc) * tot * 2);
+  zc* rho = (zc*)malloc(sizeof(zc) * tot * 2);
+  zc* rhoIn = (zc*)malloc(sizeof(zc) * tot * 2);
+  zc* tmp = (zc*)malloc(sizeof(zc) * tot * 2);
+  zc* zEig = (zc*)malloc(sizeof(zc) * n * N * 2);
+  zc* zSorted = (zc*)malloc(sizeof(zc) * n * N * 2);
+  i64* nSM = (i64*)malloc(sizeof(i64) * 2 * N * 2);
+  double* Bands = (double*)malloc(sizeof(double) * N * 2);
+  double* S1 = (double*)malloc(sizeof(double) * N * 2);
+  double* S2 = (double*)malloc(sizeof(double) * N * 4);
+  double* V = (double*)calloc(2 * n, sizeof(double));
+  double* D = (double*)malloc(sizeof(double) * 2 * n);
+  double* Din = (double*)malloc(sizeof(double) * 2 * n);
+  double* dt = (double*)malloc(sizeof(double) * 2 * n);
+  double e[4], ein[4], Etot, EtotIn, EF = 0, Deg = 0;
+  i64 nF = 0, nOcc[2] = {0, 0}, nPart[2] = {0, 0}, it = 0;
+  int info;
+#define DENS(r, d) do { for (int s_ = 0; s_ < 2; ++s_) for (i64 a = 0; a < n; ++a) \
+                        (d)[a + n * s_] = creal((r)[tot * s_ + a + n * a]); } while (0)
+#define SUB(d, o) do { for (i64 a = 0; a < 2 * n; ++a) (o)[a] = (d)[a] - 0.5; } while (0)
+#define ENERGIES(r, d, out) energies_2spins(p, (r), (r) + tot, (d), (d) + n, C, L, u1, u2, tn, (out))
+#define POTENTIAL()                                                                     \
+  do {                                                                                  \
+    for (i64 a = 0; a < n; ++a) {                                                       \
+      double s_ = 0;                                                                    \
+      for (i64 b = 0; b < n; ++b)                                                       \
+        s_ += (D[b] + D[b + n] - 2.0 * 0.5) * p->alphaH * L[a + n * b];                 \
+      V[a + n] = s_ + p->U * (D[a] - 0.5);                                              \
+      V[a] = s_ + p->U * (D[a + n] - 0.5);                                              \
+    }                                                                                   \
+  } while (0)
+
+  if (rho_start == NULL) {
+    memset(rhoIn, 0, sizeof(zc) * tot);
+    info = orc_solve(p, bulk, rhoIn, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads);
+    if (info) return -1;
+    orc_sort_energies(p, Bands, zEig, nMomComp, S1, zSorted, nSM, NULL);
+    i64 nF1, o1, q1;
+    orc_fermi(p, S1, N, &nF1, &EF, &o1, &q1, &Deg);
+    memset(rho, 0, sizeof(zc) * tot);
+    orc_getfock(p, rho, zSorted, nSM, Deg, o1, q1, C, MomVal, u1, u2, 0, NULL, 0);
+    for (size_t q = 0; q < tot; ++q) rho[q] = (rho[q] + bulk[q]) / (zc)(double)Nk;
+    memcpy(rho + tot, rho, sizeof(zc) * tot);
+  } else {
+    memcpy(rho, rho_start, sizeof(zc) * tot * 2);
+  }
+  DENS(rho, D);
+  SUB(D, dt);
+  ENERGIES(rho, dt, e);
+  Etot = total_2spins(p, e);
+  POTENTIAL();
+  memcpy(Din, D, sizeof(double) * 2 * n);
+  memcpy(rhoIn, rho, sizeof(zc) * tot * 2);
+
+  while (it < itmax) {
+    ++it;
+    for (int s = 0; s < 2; ++s) {
+      info = orc_solve(p, bulk + tot * s, rho + tot * s, V + n * s, Bands + N * s, zEig + n * N * s,
+                       C, MomVal, nMomComp, u1, u2, tn, nthreads);
+      if (info) return -it;
+      orc_sort_energies(p, Bands + N * s, zEig + n * N * s, nMomComp, S1 + N * s,
+                        zSorted + n * N * s, nSM + 2 * N * s, NULL);
+    }
+    orc_sort_2spins(p, S1, S2);
+    orc_fermi_2spins(p, S2, &nF, &EF, nOcc, nPart, &Deg);
+    for (int s = 0; s < 2; ++s) {
+      zc* r = rho + tot * s;
+      memset(r, 0, sizeof(zc) * tot);
+      orc_getfock(p, r, zSorted + n * N * s, nSM + 2 * N * s, Deg, nOcc[s], nPart[s], C, MomVal,
+                  u1, u2, 0, NULL, 0);
+      for (size_t q = 0; q < tot; ++q) r[q] = (r[q] + bulk[tot * s + q]) / (zc)(double)Nk;
+    }
+    memcpy(ein, e, sizeof(e));
+    EtotIn = Etot;
+    DENS(rho, D);
+    /* OptimalStep for numS = 2 (HartreeFock.f90:1148-1227) */
+    double c0 = EtotIn, t1, t2, step;
+    SUB(D, dt);
+    ENERGIES(rho, dt, e);
+    t1 = total_2spins(p, e);
+    for (i64 a = 0; a < 2 * n; ++a) dt[a] = 0.5 * (D[a] + Din[a]) - 0.5;
+    for (size_t q = 0; q < 2 * tot; ++q) tmp[q] = 0.5 * (rhoIn[q] + rho[q]);
+    ENERGIES(tmp, dt, e);
+    t2 = total_2spins(p, e);
+    double k1 = 4.0 * t2 - t1 - 3.0 * c0, k2 = t1 - c0 - k1;
+    if (k2 > 0 && fabs(-k1 / 2.0 / k2 - 0.5) < 0.5) step = -k1 / 2.0 / k2;
+    else if (t1 < c0) step = 1.0;
+    else step = p->step_alt;
+    for (i64 a = 0; a < 2 * n; ++a) dt[a] = (1 - step) * Din[a] + step * D[a] - 0.5;
+    for (size_t q = 0; q < 2 * tot; ++q) tmp[q] = (1.0 - step) * rhoIn[q] + step * rho[q];
+    ENERGIES(tmp, dt, e);
+    Etot = total_2spins(p, e);
+    for (size_t q = 0; q < 2 * tot; ++q) rho[q] = step * rho[q] + (1.0 - step) * rhoIn[q];
+    DENS(rho, D);
+    double dconv = 0;
+    for (i64 a = 0; a < 2 * n; ++a)
+      if (dconv < fabs(D[a] - Din[a])) dconv = fabs(D[a] - Din[a]);
+    memcpy(Din, D, sizeof(double) * 2 * n);
+    memcpy(rhoIn, rho, sizeof(zc) * tot * 2);
+    POTENTIAL();
+    if (traj) {
+      double* r = traj + 12 * (it - 1);
+      r[0] = (double)it; r[1] = EtotIn; r[2] = ein[0]; r[3] = ein[1]; r[4] = ein[2]; r[5] = ein[3];
+      r[6] = dconv; r[7] = step; r[8] = EtotIn - Etot; r[9] = EF;
+      r[10] = (double)(nOcc[0] + nOcc[1]); r[11] = (double)(nPart[0] + nPart[1]);
+    }
+  }
+#undef DENS
+#undef SUB
+#undef ENERGIES
+#undef POTENTIAL
+  if (rho_out) memcpy(rho_out, rho, sizeof(zc) * tot * 2);
+  free(bulk); free(rho); free(rhoIn); free(tmp); free(zEig); free(zSorted); free(nSM); free(Bands);
+  free(S1); free(S2); free(V); free(D); free(Din); free(dt);
+  return it;
+}

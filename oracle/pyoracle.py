@@ -54,6 +54,7 @@ def lib():
         L.orc_optimal_step.restype = dbl
         L.orc_fermi_dirac.restype = dbl
         L.orc_scf.restype = i64
+        L.orc_scf2.restype = i64
         L.orc_c3_pairs.restype = i64
         _lib = L
     return _lib
@@ -265,6 +266,27 @@ class System:
                                       ptr(self.Coords),
                                       ptr(self.longrange()), ptr(self.u1), ptr(self.u2), ptr(self.tn))
         return step, e
+
+    def scf2(self, itmax, rho_start=None, nthreads=0):
+        """The numS = 2 SCF (reference weights, ODA): rho_start (n, n, nnc, 2)
+        or None for the first diagonalisation copied to both spins.  Returns
+        (iterations, trajectory rows, rho (n, n, nnc, 2))."""
+        traj = np.zeros((itmax, 12))
+        rho_out = np.zeros((self.n, self.n, self.nnc, 2), np.complex128, order="F")
+        A = Args()
+        its = lib().orc_scf2(self.pp, ptr(self.Coords), ptr(self.tn), ptr(self.MomentaValues),
+                             ptr(self.nMomentaComponents), ptr(self.u1), ptr(self.u2),
+                             ptr(self.longrange()), A(rho_start, np.complex128), i64(itmax),
+                             ptr(traj), ptr(rho_out), C.c_int(nthreads))
+        return int(its), traj[:max(int(its), 0)], rho_out
+
+    def sort_2spins(self, S1):
+        """SortEnergies_2Spins of the two spins' sorted energies (numb*Nk x 2)."""
+        N = S1.shape[0]
+        out = np.zeros((2 * N, 2), order="F")
+        A = Args()
+        lib().orc_sort_2spins(self.pp, A(S1, float), ptr(out))
+        return out
 
     def c3_pairs(self):
         """nC3pairs (C3_RelatedPoints, Geometry.f90:1172-1223), 0-based, and

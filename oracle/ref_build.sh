@@ -17,9 +17,10 @@
 # compile.sh:1's file order is also wrong (Geometry.f90:5 uses TightBinding),
 # so the modules are compiled in dependency order here.
 #
-# Usage: [WRITEFOCK=1] [PRINTOUT=1] [C3=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
+# Usage: [WRITEFOCK=1] [PRINTOUT=1] [C3=1] [NUMS=2] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
 #   PRINTOUT=1 keeps the reference's nPrintOutputs = 1 (the Bands file, Main.f90:601-608).
 #   C3=1 sets nenforceC3 = 1 (Setup.f90:27): Solve_C3 / GetFock_C3 replace Solve / GetFock.
+#   NUMS=2 sets numS = numSIn = 2 (Setup.f90:13, 41): independent up/down spins.
 #   XI_NM is xi*0.246 (gate parameter in nm).  Toolchain: amdflang (ROCm) and
 #   MKL LP64 sequential from /opt/conda/lib.
 set -euo pipefail
@@ -27,6 +28,7 @@ CASE=$1; NTHETA=$2; NUMK=$3; NFILL=$4; ITMAX=$5; NREAD=$6; XINM=$7; NLAYERS=${8:
 WRITEFOCK=${WRITEFOCK:-0}
 PRINTOUT=${PRINTOUT:-0}
 C3=${C3:-0}
+NUMS=${NUMS:-1}
 REF=/root/reference
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref/$CASE
@@ -52,12 +54,15 @@ sed -i \
   -e "s/^integer(dp), parameter :: nPrintOutputs = .*/integer(dp), parameter :: nPrintOutputs = $PRINTOUT/" \
   -e "s/^integer(dp), parameter :: nWriteFock = .*/integer(dp), parameter :: nWriteFock = $WRITEFOCK/" \
   -e "s/^integer(dp), parameter:: nenforceC3 = .*/integer(dp), parameter:: nenforceC3 = $C3/" \
+  -e "s/^integer(dp), parameter :: numS = .*/integer(dp), parameter :: numS = $NUMS/" \
+  -e "s/^integer(dp), parameter :: numSIn = .*/integer(dp), parameter :: numSIn = $NUMS/" \
   -e "s/^real(dp), parameter:: xi = .*/real(dp), parameter:: xi = ${XINM}_dp\/0.246_dp/" \
   -e "s/^real(dp), parameter:: xiIn = .*/real(dp), parameter:: xiIn = ${XINM}_dp\/0.246_dp/" \
   "$SCR/Setup.f90"
 grep -q "ntheta = $NTHETA\$" "$SCR/Setup.f90"
 grep -q "numk = $NUMK\$" "$SCR/Setup.f90"
 grep -q "nenforceC3 = $C3\$" "$SCR/Setup.f90"
+grep -q "numS = $NUMS\$" "$SCR/Setup.f90"
 sed -i \
   -e '58s/nOccStates(numS), nPartOccStates(numS)/nOccStates(2), nPartOccStates(2)/' \
   -e '635s/KineticEnergyIn(2)/KineticEnergyIn(numS)/' \
@@ -71,5 +76,8 @@ for f in LapackRoutines Setup TightBinding HartreeFock Geometry; do
 done
 LIBS="-L$MKL -Wl,-rpath,$MKL -lmkl_gf_lp64 -lmkl_sequential -lmkl_core -lpthread -lm -ldl"
 $FC $FFLAGS -o "$OUT/forge.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o Main.f90 $LIBS
-$FC $FFLAGS -o "$OUT/harness.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o ref_harness.f90 $LIBS
+# the harness dumps the spin-singlet arrays only
+if [ "$NUMS" = "1" ]; then
+  $FC $FFLAGS -o "$OUT/harness.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o ref_harness.f90 $LIBS
+fi
 echo "built $OUT (ntheta=$NTHETA numk=$NUMK nfilling=$NFILL itmax=$ITMAX nRead=$NREAD xi=${XINM}nm nlayers=$NLAYERS)"

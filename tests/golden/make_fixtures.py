@@ -157,6 +157,51 @@ def c3_fixtures():
         print("wrote", cname, flush=True)
 
 
+# Two spins (numS = 2): name, ntheta, numk, nfilling, itmax, split start
+SPIN_CASES = [
+    ("t1k3", 1, 3, 1.0, 8, False),
+    ("t1k3s", 1, 3, 1.0, 8, True),
+    ("t2k3s", 2, 3, 1.0, 6, True),
+]
+SPLIT_SIGMA = 0.02
+
+
+def spin_split_start(ndim, nnc, nfill):
+    """Both spins' start densities of the split cases: the seeded start with
+    sigma 0.02 and two consecutive seeds (one per spin)."""
+    from forge_amd.seed import DEFAULT_SEED
+    return [seeded_rho(ndim, nnc, nfill, sigma=SPLIT_SIGMA, seed=DEFAULT_SEED + s) for s in (0, 1)]
+
+
+def spin_fixtures():
+    """Trajectories of the reference program built with numS = numSIn = 2
+    (Main.f90:364-679 with two spins): from its own first diagonalisation, or
+    from two checkpoint files with spin-split densities (Main.f90:201-235):
+    tests/golden/<case>_s2.traj.txt."""
+    e = dict(os.environ, NUMS="2")
+    for (name, ntheta, numk, nfill, itmax, split) in SPIN_CASES:
+        cname = name + "_s2"
+        subprocess.run([REF_BUILD, cname, str(ntheta), str(numk), f"{nfill:.1f}", str(itmax),
+                        "1" if split else "0", "40.0", "2"], check=True, stdout=subprocess.DEVNULL,
+                       env=e)
+        rdir = os.path.join(REF_OUT, cname, "run")
+        shutil.rmtree(rdir, ignore_errors=True)
+        os.makedirs(os.path.join(rdir, "output"))
+        os.makedirs(os.path.join(rdir, "dataFock"))
+        if split:
+            ndim = 2 * 2 * (3 * ntheta * ntheta + 3 * ntheta + 1)
+            for nspin, rho in enumerate(spin_split_start(ndim, 4, nfill), start=1):
+                fname = checkpoint_name(nspin, statename="SSS", nfilling=nfill, numS=2,
+                                        ntheta=ntheta, nlayers=2, nrelax=0, epsilon=10.0, U=4.0,
+                                        nscreen=2, xi_nm=40.0, delta=0.0, numI=1, numC=3, numk=numk)
+                write_checkpoint(os.path.join(rdir, "dataFock", fname), rho)
+        run("../forge.x > forge.log 2>&1", rdir)
+        write_traj(os.path.join(HERE, cname + ".traj.txt"), rdir,
+                   f"ntheta={ntheta} nlayers=2 numk={numk} nfilling={nfill} xi=40.0nm "
+                   f"itmax={itmax} start={'split' if split else 'reference'} numS=2")
+        print("wrote", cname, flush=True)
+
+
 def checkpoint_fixture():
     """A density checkpoint written by the reference itself (nWriteFock=1,
     Main.f90:647-677) after one iteration of case t1k3."""
@@ -192,7 +237,7 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    cases = [a for a in args if a not in ("ckpt", "bands", "c3")]
+    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin")]
     if not args or cases:
         main(cases)
     if not args or "ckpt" in args:
@@ -201,3 +246,5 @@ if __name__ == "__main__":
         bands_fixture()
     if not args or "c3" in args:
         c3_fixtures()
+    if not args or "spin" in args:
+        spin_fixtures()

@@ -201,6 +201,66 @@ def test_c3_rejects_a_grid_without_the_invariant_points():
     assert its < 0
 
 
+S2_CASES = ["t1k3_s2", "t1k3s_s2", "t2k3s_s2"]
+
+
+def spin_start(case, s):
+    """Both spins' start of a two-spin fixture case (None: first diagonalisation)."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_fixtures import spin_split_start
+    m = traj_meta(case)
+    if m["start"] != "split":
+        return None
+    return np.stack(spin_split_start(s.n, s.nnc, float(m["nfilling"])), axis=3)
+
+
+@pytest.mark.parametrize("case", S2_CASES)
+def test_two_spin_trajectory_matches_reference(case):
+    """numS = 2 (Main.f90:364-679 with two spins): the reference program built
+    with numS = numSIn = 2, from its own first diagonalisation (both spins
+    equal) or from two spin-split checkpoints.  Only the split start pins the
+    spin-exchanged Hubbard potential (swapping it moves these energies by
+    1.6e-3 eV); the joint Fermi level without its factor 2 is off by eV."""
+    m = traj_meta(case)
+    s = System(int(m["ntheta"]), int(m["numk"]), float(m["nfilling"]))
+    its, traj, _ = s.scf2(int(m["itmax"]), rho_start=spin_start(case, s))
+    E, Cv = load_traj(case)
+    assert its == int(m["itmax"]) == len(E)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=2e-9)
+    np.testing.assert_allclose(traj[:, 6], Cv[:, 1], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
+
+
+def test_sort_2spins_reproduces_the_reference_merge():
+    """SortEnergies_2Spins (HartreeFock.f90:929-1004) on interleaved spectra
+    (spin 1 even, spin 2 odd numbers), 28-site cell, 3 x 3 k: lead-in of
+    (NP-2) Nk rows of spin 1, then (NP-2) Nk copies of spin 2's row
+    (NP-2) Nk + 1 (the reference reads spin 2 at the spin-1 counter there),
+    then a sorted merge of both spins up to row Nk (NP+2) of either, then
+    the remaining rows of each spin in order.  Trajectories cannot see the
+    lead-in values (far below the Fermi level), so they are pinned here."""
+    s = System(1, 3, 1.0)
+    Nk, N = 9, 180
+    NP = (s.n // 2 - s.p.nLower) + 1
+    S1 = np.zeros((N, 2), order="F")
+    S1[:, 0] = 2.0 * np.arange(N)
+    S1[:, 1] = 2.0 * np.arange(N) + 1.0
+    out = s.sort_2spins(S1)
+    lead = (NP - 2) * Nk
+    assert out.shape == (2 * N, 2)
+    assert np.array_equal(out[:lead, 0], S1[:lead, 0]) and (out[:lead, 1] == 1).all()
+    assert (out[lead:2 * lead, 0] == S1[lead, 1]).all() and (out[lead:2 * lead, 1] == 2).all()
+    assert (out[:, 1] == 1).sum() == N and (out[:, 1] == 2).sum() == N
+    rest = out[2 * lead:]
+    for sp in (1, 2):
+        assert np.array_equal(rest[rest[:, 1] == sp, 0], S1[lead:, sp - 1])
+    # interleaved spectra merge into one ascending run until a spin passes row Nk (NP+2)
+    m = 2 * (Nk * (NP + 2) - lead)
+    assert np.all(np.diff(rest[:m, 0]) > 0)
+
+
 def test_ft_closed_forms():
     s = system_for("t1k3")
     a0, d0 = 1 / np.sqrt(3), 1.35772
