@@ -51,7 +51,7 @@ def test_file_name_matches_the_reference(path, tmp_path):
     head, energy, conv = _sections(path)
     meta = dict(kv.split("=") for kv in head[0].split(": ", 1)[1].split())
     fname = [h for h in head if h.startswith("# file: ")][0][len("# file: "):]
-    params = run_parameters("SSS", float(meta["nfilling"]), 1, int(meta["ntheta"]),
+    params = run_parameters("SSS", float(meta["nfilling"]), int(meta.get("numS", 1)), int(meta["ntheta"]),
                             int(meta["nlayers"]), 0, 10.0, 4.0, 2,
                             float(meta["xi"].rstrip("nm")), 0.0, 1, 3, int(meta["numk"]))
     assert "Energy" + params == fname
