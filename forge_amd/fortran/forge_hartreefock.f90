@@ -24,7 +24,7 @@ module forge_hartreefock
   implicit none
   private
   public :: forge_attach, forge_check, LongRangeInteraction, HamiltonianHartreeFock, Solve, GetFock
-  public :: C3_RelatedPoints, Solve_C3, GetFock_C3
+  public :: C3_RelatedPoints, Solve_C3, GetFock_C3, SortEnergies_2Spins
   public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
   public :: OptimalStep, TightBindingHamiltonian, zdiagRed
 
@@ -332,18 +332,18 @@ contains
     real(dp), intent(in) :: Density(ndim_, numS), DensityIn(ndim_, numS), DensitySub(ndim_)
     real(dp), intent(inout) :: HartreeEnergy, FockEnergy(numS), HubbardEnergy, KineticEnergy(numS)
     real(dp), intent(out) :: Step
-    complex(dp), allocatable :: zmix(:, :, :)
-    real(dp) :: d(ndim_), e(4), c0, c1, c2, E1, Eh
+    complex(dp), allocatable :: zmix(:, :, :, :)
+    real(dp) :: d(ndim_, numS), c0, c1, c2, E1, Eh
     call need(ndim_, numNeighborCells, 'OptimalStep')
-    c0 = total(FockEnergy(1), KineticEnergy(1), HartreeEnergy, HubbardEnergy)
-    d = Density(:, 1) - DensitySub
-    call forge_check(forge_energies(ctx, zFock, d, e), 'OptimalStep')
-    E1 = total(e(2), e(1), e(3), e(4))
-    allocate(zmix(ndim_, ndim_, numNeighborCells))
-    zmix = 0.5_dp * (zFockIn(:, :, :, 1) + zFock(:, :, :, 1))
-    d = 0.5_dp * (Density(:, 1) + DensityIn(:, 1)) - DensitySub
-    call forge_check(forge_energies(ctx, zmix, d, e), 'OptimalStep')
-    Eh = total(e(2), e(1), e(3), e(4))
+    c0 = total()
+    d = Density - spread(DensitySub, 2, numS)
+    call energies_of(zFock, d)
+    E1 = total()
+    allocate(zmix(ndim_, ndim_, numNeighborCells, numS))
+    zmix = 0.5_dp * (zFockIn + zFock)
+    d = 0.5_dp * (Density + DensityIn) - spread(DensitySub, 2, numS)
+    call energies_of(zmix, d)
+    Eh = total()
     c1 = 4.0_dp * Eh - E1 - 3.0_dp * c0
     c2 = E1 - c0 - c1
     if (c2 > 0.0_dp .and. abs(-c1 / 2.0_dp / c2 - 0.5_dp) < 0.5_dp) then
@@ -353,18 +353,95 @@ contains
     else
       Step = StepAlternative
     end if
-    zmix = (1.0_dp - Step) * zFockIn(:, :, :, 1) + Step * zFock(:, :, :, 1)
-    d = (1.0_dp - Step) * DensityIn(:, 1) + Step * Density(:, 1) - DensitySub
-    call forge_check(forge_energies(ctx, zmix, d, e), 'OptimalStep')
-    KineticEnergy(1) = e(1)
-    FockEnergy(1) = e(2)
-    HartreeEnergy = e(3)
-    HubbardEnergy = e(4)
+    zmix = (1.0_dp - Step) * zFockIn + Step * zFock
+    d = (1.0_dp - Step) * DensityIn + Step * Density - spread(DensitySub, 2, numS)
+    call energies_of(zmix, d)
   contains
-    real(dp) function total(ef, ek, eh_, eu)
-      real(dp), intent(in) :: ef, ek, eh_, eu
-      total = alpha * ef * real(3 - numS, dp) + ek * real(3 - numS, dp) + U * eu + alphaH * eh_
+    ! the energies of HartreeFock.f90:1162-1221 for one density per spin: per
+    ! spin Ekin and EF; EH and EU of the pair (d1, d2), or (d, d) for numS = 1
+    subroutine energies_of(z, dd)
+      complex(dp), intent(in) :: z(ndim_, ndim_, numNeighborCells, numS)
+      real(dp), intent(in) :: dd(ndim_, numS)
+      real(dp) :: e(4), hh(2)
+      integer(dp) :: s
+      do s = 1, numS
+        call forge_check(forge_energies(ctx, z(:, :, :, s), dd(:, s), e), 'OptimalStep')
+        KineticEnergy(s) = e(1)
+        FockEnergy(s) = e(2)
+      end do
+      if (numS == 1) then
+        HartreeEnergy = e(3)
+        HubbardEnergy = e(4)
+      else
+        call forge_check(forge_hartree_hubbard(ctx, dd(:, 1), dd(:, 2), hh), 'OptimalStep')
+        HartreeEnergy = hh(1)
+        HubbardEnergy = hh(2)
+      end if
+    end subroutine energies_of
+    real(dp) function total()
+      total = alpha * sum(FockEnergy) * real(3 - numS, dp) + sum(KineticEnergy) * real(3 - numS, dp) &
+              + U * HubbardEnergy + alphaH * HartreeEnergy
     end function total
   end subroutine OptimalStep
+
+  ! SortEnergies_2Spins, HartreeFock.f90:929-1004.  S1(:, s) holds spin s's
+  ! sorted window energies; S2 receives (energy, spin label) rows: the
+  ! lead-in rows of spin 1 and of spin 2 (the reference fills spin 2's lead-in
+  ! with spin 2's energy at the spin-1 counter; kept), then both spins merged
+  ! by energy while each counter stays within row Nk (W(2, s) - 1), then the
+  ! remaining rows of spin 1 and of spin 2.
+  subroutine SortEnergies_2Spins(numk_, numb_, W, S1, S2)
+    integer(dp), intent(in) :: numk_, numb_, W(2, 2)
+    real(dp), intent(in) :: S1(numb_ * numk_ * numk_, 2)
+    real(dp), intent(inout) :: S2(2 * numb_ * numk_ * numk_, 2)
+    integer(dp) :: nk2, last, c(2), r, q
+    nk2 = numk_ * numk_
+    last = numb_ * nk2
+    r = 0
+    c = 1
+    do q = 1, (W(1, 1) - 1) * nk2
+      call put(S1(c(1), 1), 1_dp)
+      c(1) = c(1) + 1
+    end do
+    do q = 1, (W(1, 2) - 1) * nk2
+      call put(S1(c(1), 2), 2_dp)
+      c(2) = c(2) + 1
+    end do
+    do while (c(1) <= nk2 * (W(2, 1) - 1) .and. c(2) <= nk2 * (W(2, 2) - 1))
+      if (S1(c(1), 1) <= S1(c(2), 2)) then
+        call run(1_dp, 2_dp)
+      else
+        call run(2_dp, 1_dp)
+      end if
+    end do
+    do q = 1, 2
+      do while (c(q) <= last)
+        call put(S1(c(q), q), int(q, dp))
+        c(q) = c(q) + 1
+      end do
+    end do
+  contains
+    subroutine put(e, lab)
+      real(dp), intent(in) :: e
+      integer(dp), intent(in) :: lab
+      r = r + 1
+      S2(r, 1) = e
+      S2(r, 2) = real(lab, dp)
+    end subroutine put
+    ! rows of spin a while they do not exceed the current row of spin o, then
+    ! that row of spin o
+    subroutine run(a, o)
+      integer(dp), intent(in) :: a, o
+      call put(S1(c(a), a), a)
+      c(a) = c(a) + 1
+      do while (c(a) <= last)
+        if (S1(c(a), a) > S1(c(o), o)) exit
+        call put(S1(c(a), a), a)
+        c(a) = c(a) + 1
+      end do
+      call put(S1(c(o), o), o)
+      c(o) = c(o) + 1
+    end subroutine run
+  end subroutine SortEnergies_2Spins
 
 end module forge_hartreefock

@@ -20,7 +20,10 @@
 ! refout (optional: a directory prefix such as output/, which must exist,
 ! receiving the reference's own Energy<parameters> and
 ! Convergence<parameters> files, Main.f90:352-358, 616-631) c3 (nenforceC3,
-! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it).
+! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it)
+! numS (1, or 2 for independent up and down spins in mode=reference with the
+! reference's weights and optimal damping) start2 (spin 2's checkpoint when
+! numS = 2; start= is then spin 1's).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -30,7 +33,7 @@ program forge_main
   implicit none
 
   character(len=16) :: mode = 'reference'
-  character(len=512) :: start = 'first', outfile = 'forge_traj.txt', refout = ''
+  character(len=512) :: start = 'first', start2 = '', outfile = 'forge_traj.txt', refout = ''
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
   integer(dp) :: nenforceC3 = 0
   integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:)
@@ -45,8 +48,8 @@ program forge_main
   complex(dp), allocatable :: zFockIn(:, :, :, :), zFockBulk(:, :, :, :)
   complex(dp), allocatable :: zEigenvectors(:, :, :, :), zSorted(:, :, :)
   real(dp) :: t1(2), t2(2), t3(2), g1(2), g12(2), tn(2, 3)
-  real(dp) :: KineticEnergy(numS), FockEnergy(numS), HartreeEnergy, HubbardEnergy, TotalEnergy
-  real(dp) :: KineticEnergyIn(numS), FockEnergyIn(numS), HartreeEnergyIn, HubbardEnergyIn
+  real(dp) :: KineticEnergy(2), FockEnergy(2), HartreeEnergy, HubbardEnergy, TotalEnergy
+  real(dp) :: KineticEnergyIn(2), FockEnergyIn(2), HartreeEnergyIn, HubbardEnergyIn
   real(dp) :: TotalEnergyIn, FermiEnergy, DegFactor, Step, DensityConvergence
   type(c_ptr) :: ctx
   type(forge_scf_options_t) :: opt
@@ -95,7 +98,18 @@ contains
     DensitySub = 0.5_dp
     Potential = 0.0_dp
     zFock = (0.0_dp, 0.0_dp)
-    if (trim(start) /= 'first') call read_fock(trim(start))
+    if (numS == 2 .and. (trim(mode) /= 'reference' .or. occ /= 0 .or. mix /= 0)) then
+      write(*, *) 'forge_main: numS = 2 runs in mode=reference with occ=0 mix=0'
+      error stop 1
+    end if
+    if (trim(start) /= 'first') call read_fock(trim(start), 1_dp)
+    if (numS == 2 .and. trim(start) /= 'first') then
+      if (len_trim(start2) == 0) then
+        write(*, *) 'forge_main: numS = 2 from checkpoints needs start2= (spin 2)'
+        error stop 1
+      end if
+      call read_fock(trim(start2), 2_dp)
+    end if
 
     open(newunit=fout, file=trim(outfile), status='replace')
     write(fout, '(A,I0,A,I0,A,I0,A,F4.1,A,A)') '# forge_main ntheta=', ntheta, ' nlayers=', nlayers, &
@@ -127,6 +141,8 @@ contains
                                    reshape([traj(1, 1:done), traj(7, 1:done), traj(8, 1:done), &
                                             traj(9, 1:done)], [int(done), 4], order=[1, 2]), &
                                    int(done, dp))
+    else if (numS == 2) then
+      call reference_sequence_2spins()
     else
       call reference_sequence()
     end if
@@ -144,13 +160,13 @@ contains
 
     ! initial state: first diagonalisation (Main.f90:236-288) unless read
     if (trim(start) == 'first') then
-      call solve_k()
+      call solve_k(1_dp)
       call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
                               SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
                               zEigenvectors(:, :, :, 1), Bands(:, :, 1))
       call fermi_level(SortedEnergies(:, 1), 1_dp)
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
-      call getfock_k()
+      call getfock_k(1_dp, nOcc, nPart)
       zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
     end if
     do i = 1, n
@@ -170,13 +186,13 @@ contains
     it = 0
     do while (it < itmax)
       it = it + 1
-      call solve_k()
+      call solve_k(1_dp)
       call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
                               SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
                               zEigenvectors(:, :, :, 1), Bands(:, :, 1))
       call fermi_level(SortedEnergies(:, 1), numS)
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
-      call getfock_k()
+      call getfock_k(1_dp, nOcc, nPart)
       zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
 
       HartreeEnergyIn = HartreeEnergy
@@ -235,7 +251,7 @@ contains
   end subroutine reference_sequence
 
   ! The run-parameter string of the reference's file names (Main.f90:180-183;
-  ! one spin, no relaxation, state label SSS, dp = 8).
+  ! no relaxation, state label SSS, dp = 8).
   function run_parameters() result(par)
     character(len=210) :: par
     character(len=1) :: dop
@@ -243,7 +259,7 @@ contains
     if (nfilling < 0.0_dp) dop = '-'
     write(par, '(A1,A3,A8,A1,F3.1,A5,I0,A2,I0,A8,I0,A6,I0,A4,F0.1,A2,F4.2,A7,I0,A3,F0.1,A6,F0.3,&
                &A5,I0,A5,I0,A5,I0,A3,I0,A4)') &
-        '-', 'SSS', '-filling', dop, abs(nfilling), '-numS', 1, '-i', ntheta, '-nlayers', nlayers, &
+        '-', 'SSS', '-filling', dop, abs(nfilling), '-numS', numS, '-i', ntheta, '-nlayers', nlayers, &
         '-relax', 0, '-eps', epsilon, '-U', U, '-screen', nscreen, '-xi', xi * 0.246_dp, &
         '-delta', Delta, '-numI', numI, '-numC', numC, '-numk', numk, '-dp', 8, '.dat'
   end function run_parameters
@@ -289,8 +305,9 @@ contains
   end subroutine fermi_level
 
   real(dp) function total_energy()
-    total_energy = alpha * sum(FockEnergy) * real(3 - numS, dp) + &
-                   sum(KineticEnergy) * real(3 - numS, dp) + U * HubbardEnergy + alphaH * HartreeEnergy
+    total_energy = alpha * sum(FockEnergy(1:numS)) * real(3 - numS, dp) + &
+                   sum(KineticEnergy(1:numS)) * real(3 - numS, dp) + U * HubbardEnergy + &
+                   alphaH * HartreeEnergy
   end function total_energy
 
   ! V = 2 alphaH L (D - 1/2) + U (D - 1/2)   (Main.f90:577-585)
@@ -302,8 +319,9 @@ contains
   ! Density matrix in the reference's checkpoint layout (Main.f90:187-235):
   ! direct access, one complex(8) per record; cell 1 as its lower triangle by
   ! rows (the upper triangle is the conjugate), then cells 2..nnc in full.
-  subroutine read_fock(path)
+  subroutine read_fock(path, sp)
     character(*), intent(in) :: path
+    integer(dp), intent(in) :: sp
     integer :: fu, ios
     integer(dp) :: rec, a, b, m
     complex(dp) :: z
@@ -318,19 +336,19 @@ contains
       do b = 1, a - 1
         rec = rec + 1
         read(fu, rec=rec) z
-        zFock(a, b, 1, 1) = z
-        zFock(b, a, 1, 1) = conjg(z)
+        zFock(a, b, 1, sp) = z
+        zFock(b, a, 1, sp) = conjg(z)
       end do
       rec = rec + 1
       read(fu, rec=rec) z
-      zFock(a, a, 1, 1) = z
+      zFock(a, a, 1, sp) = z
     end do
     do m = 2, nnc
       do a = 1, n
         do b = 1, n
           rec = rec + 1
           read(fu, rec=rec) z
-          zFock(a, b, m, 1) = z
+          zFock(a, b, m, sp) = z
         end do
       end do
     end do
@@ -361,28 +379,158 @@ contains
 
   ! Solve / Solve_C3 and GetFock / GetFock_C3 as Main.f90:244-250, 273-276,
   ! 375-378 and 500-505 choose between them.
-  subroutine solve_k()
+  subroutine solve_k(s)
+    integer(dp), intent(in) :: s
     if (nenforceC3 == 1) then
-      call Solve_C3(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
-                    zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+      call Solve_C3(zFockBulk(:, :, :, s), zFock(:, :, :, s), Potential(:, s), alpha, Bands(:, :, s), &
+                    zEigenvectors(:, :, :, s), Coords, MomentaValues, nMomentaComponents, nLower, &
                     nUpper, n, rotC3, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn, g1, g12, nC3pairs)
     else
-      call Solve(zFockBulk(:, :, :, 1), zFock(:, :, :, 1), Potential(:, 1), alpha, Bands(:, :, 1), &
-                 zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, nLower, &
+      call Solve(zFockBulk(:, :, :, s), zFock(:, :, :, s), Potential(:, s), alpha, Bands(:, :, s), &
+                 zEigenvectors(:, :, :, s), Coords, MomentaValues, nMomentaComponents, nLower, &
                  nUpper, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn)
     end if
   end subroutine solve_k
 
-  subroutine getfock_k()
+  subroutine getfock_k(s, no, np)
+    integer(dp), intent(in) :: s, no, np
     if (nenforceC3 == 1) then
-      call GetFock_C3(zFock(:, :, :, 1), zSorted(:, :, 1), n, rotC3, numb, numk, DegFactor, nOcc, &
-                      nPart, Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, &
-                      nSortedMomenta(:, :, 1), nC3pairs)
+      call GetFock_C3(zFock(:, :, :, s), zSorted(:, :, s), n, rotC3, numb, numk, DegFactor, no, &
+                      np, Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, &
+                      nSortedMomenta(:, :, s), nC3pairs)
     else
-      call GetFock(zFock(:, :, :, 1), zSorted(:, :, 1), n, numb, numk, DegFactor, nOcc, nPart, &
-                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, 1))
+      call GetFock(zFock(:, :, :, s), zSorted(:, :, s), n, numb, numk, DegFactor, no, np, &
+                   Coords, MomentaValues, nnc, nUnitCell_1, nUnitCell_2, nSortedMomenta(:, :, s))
     end if
   end subroutine getfock_k
+
+  ! The two-spin sequence (numS = 2, Main.f90:236-679): per spin Solve and
+  ! SortEnergies_1Spin; SortEnergies_2Spins and the two-spin Fermi block
+  ! (Main.f90:416-458); per spin GetFock with its own occupied counts and the
+  ! shared DegFactor; OptimalStep over both spins; the spin-exchanged Hubbard
+  ! potential (Main.f90:586-595).  The first diagonalisation fills spin 1 and
+  ! copies it to spin 2 (Main.f90:238-288).
+  subroutine reference_sequence_2spins()
+    real(dp), allocatable :: erow(:, :), crow(:, :), S2(:, :)
+    integer(dp) :: s, q, W(2, 2), nO(2), nP(2)
+    real(dp) :: dconv
+    allocate(LongRange(n, n), zFockBulk(n, n, nnc, 2), Bands(numb, Nk, 2))
+    allocate(zEigenvectors(n, numb, Nk, 2), zSorted(n, numb * Nk, 2))
+    allocate(SortedEnergies(numb * Nk, 2), nSortedMomenta(numb * Nk, 2, 2), S2(2 * numb * Nk, 2))
+    allocate(erow(6, itmax), crow(4, itmax))
+    call LongRangeInteraction(LongRange, Coords, n, t1, t2)
+    if (trim(start) == 'first') then
+      call solve_k(1_dp)
+      call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
+                              SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
+                              zEigenvectors(:, :, :, 1), Bands(:, :, 1))
+      call fermi_level(SortedEnergies(:, 1), 1_dp)
+      zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
+      call getfock_k(1_dp, nOcc, nPart)
+      zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      zFock(:, :, :, 2) = zFock(:, :, :, 1)
+    end if
+    call spin_densities()
+    do s = 1, 2
+      call GetKineticEnergy(KineticEnergy(s), Coords, zFock(:, :, :, s), Delta, nUnitCell_1, &
+                            nUnitCell_2, n, nnc, tn)
+      call GetFockEnergy(FockEnergy(s), Coords, zFock(:, :, :, s), nUnitCell_1, nUnitCell_2, n, nnc, tn)
+    end do
+    call GetHartreeHubbardEnergy(HartreeEnergy, HubbardEnergy, LongRange, Density(:, 1) - DensitySub, &
+                                 Density(:, 2) - DensitySub, n)
+    TotalEnergy = total_energy()
+    call update_potential_2spins()
+    DensityIn = Density
+    zFockIn = zFock
+    W = reshape([NeutralityPoint - 1, NeutralityPoint + 3, NeutralityPoint - 1, NeutralityPoint + 3], &
+                [2, 2])
+    it = 0
+    do while (it < itmax)
+      it = it + 1
+      do s = 1, 2
+        call solve_k(s)
+        call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, s), &
+                                SortedEnergies(:, s), nSortedMomenta(:, :, s), &
+                                zEigenvectors(:, :, :, s), Bands(:, :, s))
+      end do
+      call SortEnergies_2Spins(numk, numb, W, SortedEnergies, S2)
+      nFermiLevel = 2 * nint(real(Nk * NeutralityPoint, dp) + real(Nk, dp) / 2.0_dp * nfilling, dp)
+      FermiEnergy = (S2(nFermiLevel, 1) + S2(nFermiLevel + 1, 1)) / 2.0_dp
+      nO = 0
+      nP = 0
+      do q = 1, 2 * numb * Nk
+        s = nint(S2(q, 2), dp)
+        if (S2(q, 1) - FermiEnergy < -EnergyTolerance) then
+          nO(s) = nO(s) + 1
+        else if (abs(S2(q, 1) - FermiEnergy) < EnergyTolerance) then
+          nO(s) = nO(s) + 1
+          nP(s) = nP(s) + 1
+        end if
+      end do
+      if (nP(1) /= 0 .or. nP(2) /= 0) then
+        DegFactor = 1.0_dp / real(nP(1) + nP(2), dp) * &
+                    real(nFermiLevel + nP(1) + nP(2) - nO(1) - nO(2), dp)
+      else
+        DegFactor = 0.0_dp
+      end if
+      do s = 1, 2
+        zFock(:, :, :, s) = (0.0_dp, 0.0_dp)
+        call getfock_k(s, nO(s), nP(s))
+        zFock(:, :, :, s) = (zFock(:, :, :, s) + zFockBulk(:, :, :, s)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      end do
+      HartreeEnergyIn = HartreeEnergy
+      FockEnergyIn = FockEnergy
+      HubbardEnergyIn = HubbardEnergy
+      KineticEnergyIn = KineticEnergy
+      TotalEnergyIn = TotalEnergy
+      call spin_densities()
+      call OptimalStep(Step, HartreeEnergy, FockEnergy, HubbardEnergy, KineticEnergy, zFock, &
+                       zFockIn, Density, DensityIn, DensitySub, n, nnc, Coords, LongRange, Delta, &
+                       nUnitCell_1, nUnitCell_2, tn)
+      TotalEnergy = total_energy()
+      zFock = Step * zFock + (1.0_dp - Step) * zFockIn
+      call spin_densities()
+      dconv = maxval(abs(Density - DensityIn))
+      DensityIn = Density
+      zFockIn = zFock
+      call update_potential_2spins()
+      erow(:, it) = [real(it, dp), TotalEnergyIn, KineticEnergyIn(1) + KineticEnergyIn(2), &
+                     FockEnergyIn(1) + FockEnergyIn(2), HartreeEnergyIn, HubbardEnergyIn]
+      crow(:, it) = [real(it, dp), dconv, Step, TotalEnergyIn - TotalEnergy]
+      write(*, '(A,I0,A,F20.9,A,ES10.3)') 'it=', it, ' E=', TotalEnergy, ' dE=', &
+          TotalEnergyIn - TotalEnergy
+      if (etol > 0.0_dp .and. abs(TotalEnergyIn - TotalEnergy) < etol) exit
+    end do
+    write(fout, '(A)') '# Energy: it Etot Ekin EF EH EU'
+    do i = 1, min(it, itmax)
+      write(fout, '(F5.1,3X,5(F24.9,3X))') erow(:, i)
+    end do
+    write(fout, '(A)') '# Convergence: it maxdDensity Step dE'
+    do i = 1, min(it, itmax)
+      write(fout, '(F5.1,3X,F9.6,3X,F5.3,3X,F24.9,3X)') crow(:, i)
+    end do
+    if (len_trim(refout) > 0) &
+      call write_reference_files(erow(:, 1:min(it, itmax)), transpose(crow(:, 1:min(it, itmax))), &
+                                 min(it, itmax))
+  end subroutine reference_sequence_2spins
+
+  subroutine spin_densities()
+    integer(dp) :: s
+    do s = 1, numS
+      do i = 1, n
+        Density(i, s) = real(zFock(i, i, 1, s), dp)
+      end do
+    end do
+  end subroutine spin_densities
+
+  ! V_1 = alphaH L (D_1 + D_2 - 1) + U (D_2 - 1/2), V_2 likewise with D_1
+  ! (Main.f90:586-595)
+  subroutine update_potential_2spins()
+    real(dp) :: base(n)
+    base = alphaH * matmul(LongRange, Density(:, 1) + Density(:, 2) - 2.0_dp * DensitySub)
+    Potential(:, 2) = base + U * (Density(:, 1) - DensitySub)
+    Potential(:, 1) = base + U * (Density(:, 2) - DensitySub)
+  end subroutine update_potential_2spins
 
   subroutine parse_args()
     integer :: k, eq
@@ -417,12 +565,18 @@ contains
       case ('switch_de'); read(val, *) switch_de
       case ('out'); outfile = val
       case ('refout'); refout = val
+      case ('numS'); read(val, *) numS
+      case ('start2'); start2 = val
       case ('c3'); read(val, *) nenforceC3
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)
         error stop 1
       end select
     end do
+    if (numS /= 1 .and. numS /= 2) then
+      write(*, *) 'forge_main: numS must be 1 or 2'
+      error stop 1
+    end if
     if (trim(mode) /= 'reference' .and. trim(mode) /= 'device' .and. trim(mode) /= 'geometry') then
       write(*, *) 'forge_main: mode must be reference, device or geometry'
       error stop 1

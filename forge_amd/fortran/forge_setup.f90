@@ -2,15 +2,15 @@
 !
 ! Same names and meaning as the reference's Setup module (Setup.f90:6-74);
 ! here they are variables set once from the command line by setup_init, so a
-! single executable covers every cell size, k grid and filling.  Only the
-! spin-singlet case (numS = 1) without symmetry enforcement or relaxation is
-! covered, which is the hot path the engine replaces.
+! single executable covers every cell size, k grid and filling.  numS = 1
+! (spin singlet) and numS = 2 (independent up and down spins) are covered;
+! symmetry enforcement other than C3, and relaxation, are not.
 module forge_setup
   use iso_c_binding, only: c_int64_t
   use forge_c, only: forge_params_t
   implicit none
   integer, parameter :: dp = 8
-  integer(dp), parameter :: numS = 1
+  integer(dp) :: numS = 1
   real(dp), parameter :: pi = 4.0_dp * atan(1.0_dp)
   real(dp), parameter :: a1(2) = [0.5_dp, sqrt(3.0_dp) * 0.5_dp]
   real(dp), parameter :: a2(2) = [-0.5_dp, sqrt(3.0_dp) * 0.5_dp]

@@ -60,3 +60,10 @@ def seeded_rho(ndim: int, ncells: int, nfilling: float, sigma: float = 1e-3,
     d = seeded_site_density(ndim, nfilling, sigma, seed)
     rho[np.arange(ndim), np.arange(ndim), 0] = d
     return rho
+
+
+def seeded_rho_2spins(ndim: int, ncells: int, nfilling: float, sigma: float = 0.02,
+                      seed: int = DEFAULT_SEED) -> list:
+    """Spin-split start for numS = 2: the seeded start of each spin with its
+    own seed (seed, seed + 1) and a larger spread, so the two spins differ."""
+    return [seeded_rho(ndim, ncells, nfilling, sigma=sigma, seed=seed + s) for s in (0, 1)]

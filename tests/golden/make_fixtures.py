@@ -26,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 from forge_amd.checkpoint import checkpoint_name, write_checkpoint  # noqa: E402
-from forge_amd.seed import seeded_rho  # noqa: E402
+from forge_amd.seed import seeded_rho, seeded_rho_2spins  # noqa: E402
 
 REF_BUILD = os.path.join(REPO, "oracle", "ref_build.sh")
 REF_OUT = os.path.join(REPO, "oracle", "_ref")
@@ -163,14 +163,6 @@ SPIN_CASES = [
     ("t1k3s", 1, 3, 1.0, 8, True),
     ("t2k3s", 2, 3, 1.0, 6, True),
 ]
-SPLIT_SIGMA = 0.02
-
-
-def spin_split_start(ndim, nnc, nfill):
-    """Both spins' start densities of the split cases: the seeded start with
-    sigma 0.02 and two consecutive seeds (one per spin)."""
-    from forge_amd.seed import DEFAULT_SEED
-    return [seeded_rho(ndim, nnc, nfill, sigma=SPLIT_SIGMA, seed=DEFAULT_SEED + s) for s in (0, 1)]
 
 
 def spin_fixtures():
@@ -190,7 +182,7 @@ def spin_fixtures():
         os.makedirs(os.path.join(rdir, "dataFock"))
         if split:
             ndim = 2 * 2 * (3 * ntheta * ntheta + 3 * ntheta + 1)
-            for nspin, rho in enumerate(spin_split_start(ndim, 4, nfill), start=1):
+            for nspin, rho in enumerate(seeded_rho_2spins(ndim, 4, nfill), start=1):
                 fname = checkpoint_name(nspin, statename="SSS", nfilling=nfill, numS=2,
                                         ntheta=ntheta, nlayers=2, nrelax=0, epsilon=10.0, U=4.0,
                                         nscreen=2, xi_nm=40.0, delta=0.0, numI=1, numC=3, numk=numk)

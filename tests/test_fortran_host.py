@@ -8,7 +8,9 @@ mode=reference calls the drop-in shims where Main.f90 calls the reference
 routines, mode=device runs the loop device-resident; the seeded case goes
 through the reference's checkpoint format (nRead = 1 path); both also write
 the reference's own Energy/Convergence files (refout=); c3=1 (nenforceC3)
-reproduces the reference built with Solve_C3 / GetFock_C3.
+reproduces the reference built with Solve_C3 / GetFock_C3; numS=2 the
+reference built with two spins (first diagonalisation and spin-split
+checkpoints).
 """
 import os
 import subprocess
@@ -173,3 +175,53 @@ def test_fortran_host_c3_trajectory_matches_reference(case, mode, tmp_path):
     np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
     np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
     np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
+
+
+def _s2_args(case):
+    m = traj_meta(case)
+    return [f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}", f"numk={m['numk']}",
+            f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}", "numS=2"], m
+
+
+@pytest.mark.parametrize("case", ["t1k3_s2", "t2k3s_s2"])
+def test_fortran_two_spin_file_names(case, tmp_path):
+    """numS=2 enters the run-parameter string as the reference's numS2 (CPU)."""
+    args, _ = _s2_args(case)
+    out = tmp_path / "geom.txt"
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", *args, f"out={out}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    par = [l.split()[2] for l in open(out) if l.startswith("# parameters ")][0]
+    assert "Energy" + par == _reference_file_name(case)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["t1k3_s2", "t1k3s_s2", "t2k3s_s2"])
+def test_fortran_host_two_spin_trajectory_matches_reference(case, tmp_path):
+    from forge_amd.checkpoint import write_checkpoint
+    from forge_amd.geometry import build_geometry
+    from forge_amd.seed import seeded_rho_2spins
+    print("fortran host numS=2", case, flush=True)
+    args, m = _s2_args(case)
+    refdir = tmp_path / "output"
+    refdir.mkdir()
+    args = [FORTRAN_EXE, "mode=reference", *args, f"itmax={m['itmax']}",
+            f"out={tmp_path / 'traj.txt'}", f"refout={refdir}/"]
+    if m["start"] == "split":
+        g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
+        for sp, rho in enumerate(seeded_rho_2spins(g.ndim, g.nnc, float(m["nfilling"])), start=1):
+            write_checkpoint(str(tmp_path / f"s{sp}.ckpt"), rho)
+        args += [f"start={tmp_path / 's1.ckpt'}", f"start2={tmp_path / 's2.ckpt'}"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
+    np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
+    name = _reference_file_name(case)
+    Ef = np.array([[float(x) for x in l.split()] for l in open(refdir / name) if l.strip()])
+    np.testing.assert_array_equal(Ef, E)

@@ -206,13 +206,11 @@ S2_CASES = ["t1k3_s2", "t1k3s_s2", "t2k3s_s2"]
 
 def spin_start(case, s):
     """Both spins' start of a two-spin fixture case (None: first diagonalisation)."""
-    import sys
-    sys.path.insert(0, GOLDEN)
-    from make_fixtures import spin_split_start
+    from forge_amd.seed import seeded_rho_2spins
     m = traj_meta(case)
     if m["start"] != "split":
         return None
-    return np.stack(spin_split_start(s.n, s.nnc, float(m["nfilling"])), axis=3)
+    return np.stack(seeded_rho_2spins(s.n, s.nnc, float(m["nfilling"])), axis=3)
 
 
 @pytest.mark.parametrize("case", S2_CASES)
