@@ -765,6 +765,50 @@ void orc_getfock_c3(const orc_params* p, const i64* rot, const i64* perm, zc* zF
   free(zP); free(f);
 }
 
+/* ------------------------------------------------------------------ T / C2 / C2T */
+
+/* C2_RelatedPoints, Geometry.f90:1140-1167: within each layer, site i of the
+ * first sublattice half and site j of the second are partners when
+ * r_i = -r_j in the plane (within 1e-5); perm[i] = j and perm[j] = i
+ * (0-based, -1 when unmatched).  Returns the number of pairs (ndim/2 for a
+ * C2-symmetric cell). */
+i64 orc_c2_pairs(const double* C, i64 ndim, i64 nlayers, i64* perm) {
+  i64 per = ndim / nlayers, half = per / 2, cnt = 0;
+  for (i64 i = 0; i < ndim; ++i) perm[i] = -1;
+  for (i64 l = 0; l < nlayers; ++l)
+    for (i64 i = l * per; i < l * per + half; ++i)
+      for (i64 j = l * per + half; j < (l + 1) * per; ++j) {
+        double x = C[i] + C[j], y = C[i + ndim] + C[j + ndim];
+        if (sqrt(x * x + y * y) < 1e-5) { perm[i] = j; perm[j] = i; ++cnt; }
+      }
+  return cnt;
+}
+
+/* The symmetrisations of Main.f90:514-534, in that order, on every stored
+ * cell of rho: sym & 1 (nenforceT) keeps the real part; sym & 2 (nenforceC2)
+ * rho <- (rho + conj(rho(p, p))^T) / 2; sym & 4 (nenforceC2T)
+ * rho <- (rho + rho(p, p)^T) / 2, p = C2 partners. */
+void orc_symmetrize(const orc_params* p, zc* rho, int sym, const i64* c2) {
+  i64 n = p->ndim;
+  size_t nn = (size_t)n * n;
+  if (sym & 1)
+    for (size_t q = 0; q < nn * p->nnc; ++q) rho[q] = creal(rho[q]);
+  zc* t = (zc*)malloc(sizeof(zc) * nn);
+  for (int f = 2; f <= 4; f *= 2) {
+    if (!(sym & f)) continue;
+    for (i64 m = 0; m < p->nnc; ++m) {
+      zc* r = rho + nn * m;
+      memcpy(t, r, sizeof(zc) * nn);
+      for (i64 j = 0; j < n; ++j)
+        for (i64 i = 0; i < n; ++i) {
+          zc o = t[c2[j] + n * c2[i]];
+          r[i + n * j] = 0.5 * (t[i + n * j] + (f == 2 ? conj(o) : o));
+        }
+    }
+  }
+  free(t);
+}
+
 /* ------------------------------------------------------------------ energies */
 
 /* GetKineticEnergy, HartreeFock.f90:1012-1059. */
@@ -908,13 +952,15 @@ double orc_optimal_step(const orc_params* p, double* e_inout, const zc* rhoNew, 
  *   rot, c3perm NULL: Solve / GetFock; else nenforceC3 = 1 (Main.f90:244-250,
  *               273-276, 375-378, 500-505): Solve_C3 / GetFock_C3 with the
  *               layer rotation signs and the orc_c3_pairs permutation
+ *   sym, c2perm nenforceT / C2 / C2T bits (orc_symmetrize) applied to each
+ *               iteration's new density (Main.f90:514-534), C2 partners
  * Returns the number of iterations run (negative on an eigensolver error or,
  * with c3perm, numk % 3 != 0). */
 i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double* MomVal,
             const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
             const zc* rho_start, i64 itmax, int occ_mode, double kT, int mix_mode, double beta,
             double etol, double* traj, zc* rho_out, double* timing, int nthreads,
-            double* bands_last, const i64* rot, const i64* c3perm) {
+            double* bands_last, const i64* rot, const i64* c3perm, int sym, const i64* c2perm) {
   i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
   i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
   size_t nn = (size_t)n * n, tot = nn * nnc;
@@ -986,6 +1032,7 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
     tsolve += omp_get_wtime() - t;
     if (info) return -it;
     WINDOW_DENSITY();
+    if (sym) orc_symmetrize(p, rho, sym, c2perm);
     memcpy(ein, e, sizeof(e));
     EtotIn = Etot;
     for (i64 a = 0; a < n; ++a) D[a] = creal(rho[a + n * a]);

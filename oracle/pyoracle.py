@@ -56,6 +56,7 @@ def lib():
         L.orc_scf.restype = i64
         L.orc_scf2.restype = i64
         L.orc_c3_pairs.restype = i64
+        L.orc_c2_pairs.restype = i64
         _lib = L
     return _lib
 
@@ -295,11 +296,24 @@ class System:
         cnt = int(lib().orc_c3_pairs(ptr(self.Coords), i64(self.n), i64(self.nlayers), ptr(perm)))
         return perm, cnt
 
+    def c2_pairs(self):
+        """nC2pairs (C2_RelatedPoints, Geometry.f90:1140-1167), 0-based, and
+        the number of pairs matched."""
+        perm = np.zeros(self.n, np.int64)
+        cnt = int(lib().orc_c2_pairs(ptr(self.Coords), i64(self.n), i64(self.nlayers), ptr(perm)))
+        return perm, cnt
+
     def scf(self, itmax, rho_start=None, occ_mode=0, kT=0.0, mix_mode=0, beta=0.3, etol=0.0,
-            nthreads=0, keep_bands=False, c3=False):
+            nthreads=0, keep_bands=False, c3=False, sym=0):
         """keep_bands: the last iteration's window bands (numb x Nk) are left
         in self.last_bands.  c3: the reference's nenforceC3 = 1 path
-        (Solve_C3 / GetFock_C3)."""
+        (Solve_C3 / GetFock_C3).  sym: nenforceT (1) | nenforceC2 (2) |
+        nenforceC2T (4)."""
+        c2 = None
+        if sym:
+            c2, cnt = self.c2_pairs()
+            if cnt != self.n // 2:
+                raise ValueError(f"cell is not C2-symmetric ({cnt} of {self.n // 2} pairs)")
         perm = None
         if c3:
             perm, cnt = self.c3_pairs()
@@ -317,7 +331,8 @@ class System:
                             i64(itmax), C.c_int(occ_mode), dbl(kT), C.c_int(mix_mode), dbl(beta),
                             dbl(etol), ptr(traj), ptr(rho_out), ptr(timing), C.c_int(nthreads),
                             ptr(bands) if keep_bands else None,
-                            ptr(self.rot) if c3 else None, ptr(perm) if c3 else None)
+                            ptr(self.rot) if c3 else None, ptr(perm) if c3 else None,
+                            C.c_int(sym), ptr(c2) if sym else None)
         if keep_bands:
             self.last_bands = bands
         return int(its), traj[:max(int(its), 0)], rho_out, timing

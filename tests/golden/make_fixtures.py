@@ -194,6 +194,44 @@ def spin_fixtures():
         print("wrote", cname, flush=True)
 
 
+# Symmetry enforcement (Main.f90:514-534): name, ntheta, numk, nfilling, itmax,
+# seeded start, flags (T, C2, C2T)
+SYM_CASES = [
+    ("t1k3_T", 1, 3, 1.0, 8, False, (1, 0, 0)),
+    ("t1k3s_C2", 1, 3, 1.0, 8, True, (0, 1, 0)),
+    ("t1k3s_C2T", 1, 3, 1.0, 8, True, (0, 0, 1)),
+    ("t2k3s_TC2C2T", 2, 3, 1.0, 6, True, (1, 1, 1)),
+]
+
+
+def sym_fixtures():
+    """Trajectories of the reference program built with nenforceT /
+    nenforceC2 / nenforceC2T set: tests/golden/<case>.traj.txt."""
+    for (name, ntheta, numk, nfill, itmax, seeded, (ft, fc2, fc2t)) in SYM_CASES:
+        e = dict(os.environ, ENFT=str(ft), ENFC2=str(fc2), ENFC2T=str(fc2t))
+        subprocess.run([REF_BUILD, name, str(ntheta), str(numk), f"{nfill:.1f}", str(itmax),
+                        "1" if seeded else "0", "40.0", "2"], check=True, stdout=subprocess.DEVNULL,
+                       env=e)
+        rdir = os.path.join(REF_OUT, name, "run")
+        shutil.rmtree(rdir, ignore_errors=True)
+        os.makedirs(os.path.join(rdir, "output"))
+        os.makedirs(os.path.join(rdir, "dataFock"))
+        if seeded:
+            ndim = 2 * 2 * (3 * ntheta * ntheta + 3 * ntheta + 1)
+            fname = checkpoint_name(1, statename="SSS", nfilling=nfill, numS=1, ntheta=ntheta,
+                                    nlayers=2, nrelax=0, epsilon=10.0, U=4.0, nscreen=2,
+                                    xi_nm=40.0, delta=0.0, numI=1, numC=3, numk=numk)
+            write_checkpoint(os.path.join(rdir, "dataFock", fname), seeded_rho(ndim, 4, nfill))
+        run("../forge.x > forge.log 2>&1", rdir)
+        if "ERROR C2" in open(os.path.join(rdir, "forge.log")).read():
+            raise RuntimeError(f"{name}: the reference reports a cell that is not C2-symmetric")
+        write_traj(os.path.join(HERE, name + ".traj.txt"), rdir,
+                   f"ntheta={ntheta} nlayers=2 numk={numk} nfilling={nfill} xi=40.0nm "
+                   f"itmax={itmax} start={'seeded' if seeded else 'reference'} "
+                   f"nenforceT={ft} nenforceC2={fc2} nenforceC2T={fc2t}")
+        print("wrote", name, flush=True)
+
+
 def checkpoint_fixture():
     """A density checkpoint written by the reference itself (nWriteFock=1,
     Main.f90:647-677) after one iteration of case t1k3."""
@@ -229,7 +267,7 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin")]
+    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym")]
     if not args or cases:
         main(cases)
     if not args or "ckpt" in args:
@@ -240,3 +278,5 @@ if __name__ == "__main__":
         c3_fixtures()
     if not args or "spin" in args:
         spin_fixtures()
+    if not args or "sym" in args:
+        sym_fixtures()

@@ -259,6 +259,45 @@ def test_sort_2spins_reproduces_the_reference_merge():
     assert np.all(np.diff(rest[:m, 0]) > 0)
 
 
+SYM_CASES = ["t1k3_T", "t1k3s_C2", "t1k3s_C2T", "t2k3s_TC2C2T"]
+
+
+def sym_flags(m):
+    return int(m["nenforceT"]) | 2 * int(m["nenforceC2"]) | 4 * int(m["nenforceC2T"])
+
+
+@pytest.mark.parametrize("case", SYM_CASES)
+def test_symmetrised_trajectory_matches_reference(case):
+    """nenforceT / nenforceC2 / nenforceC2T (Main.f90:514-534): the reference
+    program built with those settings.  C2 and C2T differ only by the
+    conjugation, and their trajectories differ by eV from the second
+    iteration on, so either mistake fails here."""
+    m = traj_meta(case)
+    s = System(int(m["ntheta"]), int(m["numk"]), float(m["nfilling"]))
+    start = seeded_rho(s.n, s.nnc, float(m["nfilling"])) if m["start"] == "seeded" else None
+    its, traj, _, _ = s.scf(int(m["itmax"]), rho_start=start, sym=sym_flags(m))
+    E, Cv = load_traj(case)
+    assert its == int(m["itmax"]) == len(E)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=2e-9)
+    np.testing.assert_allclose(traj[:, 6], Cv[:, 1], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
+
+
+@pytest.mark.parametrize("ntheta,nlayers", [(1, 2), (2, 2), (1, 3)])
+def test_c2_pairs_is_the_inversion(ntheta, nlayers):
+    """nC2pairs is an involution that pairs each site of a layer's first
+    sublattice half with the site of its second half at -r (in the plane)."""
+    s = System(ntheta, 3, 1.0, nlayers=nlayers)
+    perm, cnt = s.c2_pairs()
+    n, per = s.n, s.n // nlayers
+    assert cnt == n // 2 and (perm[perm] == np.arange(n)).all()
+    first = (np.arange(n) % per) < per // 2
+    assert ((perm % per) < per // 2)[first].sum() == 0
+    xy = s.Coords[:, :2]
+    assert np.abs(xy[perm] + xy).max() < 1e-9
+
+
 def test_ft_closed_forms():
     s = system_for("t1k3")
     a0, d0 = 1 / np.sqrt(3), 1.35772
