@@ -191,6 +191,9 @@ struct forge_ctx {
   z2* c3_Psi = nullptr;
   double* c3_wph = nullptr;
   i64 c3_cap = 0;  // columns of c3_Psi
+  // nenforceT / C2 / C2T of the SCF loop (bits 1, 2, 4) and the C2 partners
+  int sym = 0;
+  int* c2_perm = nullptr;
 };
 
 static double total_energy(const forge_params& P, const double* e) {
@@ -369,7 +372,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
                   c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
-                  c->c3_Psi, c->c3_wph};
+                  c->c3_Psi, c->c3_wph, c->c2_perm};
   for (void* q : ptrs)
     if (q) hipFree(q);
   tridiag_release(c->trd);
@@ -1020,6 +1023,60 @@ extern "C" int forge_getfock_c3(forge_ctx* c, forge_z* zFock, const forge_z* zSo
   return 0;
 }
 
+// ---------------------------------------------------------------- T / C2 / C2T
+
+extern "C" int64_t forge_c2_pairs(const double* Coords, int64_t ndim, int64_t nlayers,
+                                  int64_t* nC2pairs) {
+  if (!Coords || !nC2pairs || ndim < 2 || nlayers < 1 || ndim % nlayers != 0 ||
+      (ndim / nlayers) % 2 != 0) {
+    set_error("forge_c2_pairs: invalid argument");
+    return -2;
+  }
+  const i64 cnt = c2_pairs(Coords, ndim, nlayers, nC2pairs);
+  for (i64 i = 0; i < ndim; ++i)
+    if (nC2pairs[i] >= 0) nC2pairs[i] += 1;
+  return cnt;
+}
+
+// C2 partners (1-based) onto the device: an involution of 1..ndim.
+static int c2_configure(forge_ctx* c, int sym, const int64_t* pairs, const char* who) {
+  if (sym < 0 || sym > 7) { set_error(std::string(who) + ": flags must combine 1 (T), 2 (C2), 4 (C2T)"); return -2; }
+  if (!(sym & 6)) return 0;
+  if (!pairs) { set_error(std::string(who) + ": C2 or C2T needs nC2pairs"); return -2; }
+  const i64 n = c->n;
+  std::vector<int> p(n);
+  for (i64 i = 0; i < n; ++i) {
+    const int64_t j = pairs[i] - 1;
+    if (j < 0 || j >= n) { set_error(std::string(who) + ": nC2pairs out of range"); return -2; }
+    p[i] = (int)j;
+  }
+  for (i64 i = 0; i < n; ++i)
+    if (p[p[i]] != i) { set_error(std::string(who) + ": nC2pairs is not an involution"); return -2; }
+  if (!c->c2_perm) CK(dalloc((void**)&c->c2_perm, sizeof(int) * n));
+  FORGE_HIP(hipMemcpy(c->c2_perm, p.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int forge_symmetrize(forge_ctx* c, forge_z* zFock, int32_t flags, const int64_t* nC2pairs) {
+  if (!c || !zFock) { set_error("forge_symmetrize: null argument"); return -2; }
+  CK(c2_configure(c, flags, nC2pairs, "forge_symmetrize"));
+  const i64 n = c->n, nnc = c->nnc;
+  const size_t nn = (size_t)n * n;
+  FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
+  CK(symmetrize(n, (int)nnc, c->rho_new, c->rho_new + nnc * nn, flags, c->c2_perm, c->s));
+  FORGE_HIP(hipMemcpyAsync(zFock, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_scf_set_symmetry(forge_ctx* c, int32_t flags, const int64_t* nC2pairs) {
+  if (!c) { set_error("forge_scf_set_symmetry: null context"); return -2; }
+  CK(c2_configure(c, flags, nC2pairs, "forge_scf_set_symmetry"));
+  c->sym = flags;
+  return 0;
+}
+
 extern "C" int forge_scf_set_c3(forge_ctx* c, const int64_t* RotateLayers, const int64_t* nC3pairs) {
   if (!c) { set_error("forge_scf_set_c3: null context"); return -2; }
   if (!RotateLayers && !nC3pairs) {
@@ -1348,6 +1405,8 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
   Occupation occ;
   CK(window_step(c, band, occ));
+  // symmetry enforcement on the new density (Main.f90:514-534)
+  if (c->sym) CK(symmetrize(n, (int)nnc, c->rho_new, c->rho_new + nnc * nn, c->sym, c->c2_perm, c->s));
   if (timed) {
     FORGE_HIP(hipEventRecord(c->ev[11], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[11]));

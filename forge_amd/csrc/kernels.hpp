@@ -118,4 +118,9 @@ int bulk_density_c3(const C3Geom& g, int K, const z2* C, size_t ldC, size_t sC, 
                     const C3Slot* slot, z2* rho, bool accumulate, hipStream_t s);
 i64 c3_pairs(const double* C, i64 n, i64 nl, i64* perm);
 
+// sym.hip: nenforceT / C2 / C2T on the nnc cells of rho (sym bits 1, 2, 4;
+// scratch: one n x n matrix; perm: device C2 partners, 0-based).
+int symmetrize(i64 n, int nnc, z2* rho, z2* scratch, int sym, const int* perm, hipStream_t s);
+i64 c2_pairs(const double* C, i64 n, i64 nl, i64* perm);
+
 }  // namespace forge

@@ -16,6 +16,7 @@ module forge_c
   public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
   public :: forge_sweep_create, forge_sweep_run, forge_sweep_destroy
   public :: forge_c3_pairs, forge_solve_c3, forge_getfock_c3, forge_scf_set_c3
+  public :: forge_c2_pairs, forge_symmetrize, forge_scf_set_symmetry
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -166,6 +167,31 @@ module forge_c
       type(c_ptr), value :: RotateLayers, nC3pairs
       integer(c_int) :: r
     end function forge_scf_set_c3
+    function forge_c2_pairs(Coords, ndim, nlayers, nC2pairs) bind(C, name="forge_c2_pairs") &
+        result(r)
+      import :: c_double, c_int64_t
+      real(c_double), intent(in) :: Coords(*)
+      integer(c_int64_t), value :: ndim, nlayers
+      integer(c_int64_t), intent(out) :: nC2pairs(*)
+      integer(c_int64_t) :: r
+    end function forge_c2_pairs
+    function forge_symmetrize(ctx, zFock, flags, nC2pairs) bind(C, name="forge_symmetrize") &
+        result(r)
+      import :: c_int, c_ptr, c_double_complex, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: zFock(*)
+      integer(c_int32_t), value :: flags
+      integer(c_int64_t), intent(in) :: nC2pairs(*)
+      integer(c_int) :: r
+    end function forge_symmetrize
+    function forge_scf_set_symmetry(ctx, flags, nC2pairs) bind(C, name="forge_scf_set_symmetry") &
+        result(r)
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: flags
+      integer(c_int64_t), intent(in) :: nC2pairs(*)
+      integer(c_int) :: r
+    end function forge_scf_set_symmetry
     function forge_scf_set_options(ctx, opt) bind(C, name="forge_scf_set_options") result(r)
       import :: c_int, c_ptr, forge_scf_options_t
       type(c_ptr), value :: ctx

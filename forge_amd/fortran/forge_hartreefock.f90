@@ -4,7 +4,7 @@
 ! routine it replaces, so a host written against the reference calls it
 ! unchanged; the body calls the MI355X engine through forge_c.  Arguments
 ! the engine already holds (the geometry, uploaded once by forge_ctx_create)
-! are accepted and checked for size only.  numS = 1.
+! are accepted and checked for size only.  numS = 1 or 2 (OptimalStep).
 !
 !   LongRangeInteraction     HartreeFock.f90:14-53
 !   HamiltonianHartreeFock   HartreeFock.f90:57-127
@@ -17,6 +17,11 @@
 !   OptimalStep              HartreeFock.f90:1148-1227
 !   TightBindingHamiltonian  TightBinding.f90:13-55
 !   zdiagRed                 LapackRoutines.f90:381-433
+!   C3_RelatedPoints         Geometry.f90:1172-1223    (host)
+!   C2_RelatedPoints         Geometry.f90:1140-1167    (host)
+!   Solve_C3                 HartreeFock.f90:131-398
+!   GetFock_C3               HartreeFock.f90:564-881
+!   SortEnergies_2Spins      HartreeFock.f90:929-1004  (host)
 module forge_hartreefock
   use iso_c_binding
   use forge_c
@@ -24,7 +29,7 @@ module forge_hartreefock
   implicit none
   private
   public :: forge_attach, forge_check, LongRangeInteraction, HamiltonianHartreeFock, Solve, GetFock
-  public :: C3_RelatedPoints, Solve_C3, GetFock_C3, SortEnergies_2Spins
+  public :: C3_RelatedPoints, Solve_C3, GetFock_C3, SortEnergies_2Spins, C2_RelatedPoints
   public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
   public :: OptimalStep, TightBindingHamiltonian, zdiagRed
 
@@ -165,6 +170,18 @@ contains
     cnt = forge_c3_pairs(Coords, ndim_, int(nlayers, c_int64_t), nC3pairs)
     if (cnt < ndim_) write(*, *) 'ERROR C3. cnt=', cnt
   end subroutine C3_RelatedPoints
+
+  ! C2_RelatedPoints(nC2pairs, Coords, ndim), Geometry.f90:1140-1167 (host
+  ! code of the library; prints the reference's message when the cell is not
+  ! C2-symmetric).
+  subroutine C2_RelatedPoints(nC2pairs, Coords, ndim_)
+    integer(dp), intent(in) :: ndim_
+    real(dp), intent(in) :: Coords(ndim_, 3)
+    integer(dp), intent(inout) :: nC2pairs(ndim_)
+    integer(dp) :: cnt
+    cnt = forge_c2_pairs(Coords, ndim_, int(nlayers, c_int64_t), nC2pairs)
+    if (cnt < ndim_ / 2) write(*, *) 'ERROR C2. cnt=', cnt
+  end subroutine C2_RelatedPoints
 
   subroutine Solve_C3(zFockBulk, zFock, Potential, alpha_, Bands, zEigenvectors, Coords, &
                       MomentaValues, nMomentaComponents, nLower_, nUpper_, ndim_, RotateLayers_, &

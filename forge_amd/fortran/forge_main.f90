@@ -23,7 +23,9 @@
 ! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it)
 ! numS (1, or 2 for independent up and down spins in mode=reference with the
 ! reference's weights and optimal damping) start2 (spin 2's checkpoint when
-! numS = 2; start= is then spin 1's).
+! numS = 2; start= is then spin 1's) T C2 C2T (nenforceT / nenforceC2 /
+! nenforceC2T, Setup.f90:28-30: 1 = symmetrise each new density as
+! Main.f90:514-534 does).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -37,6 +39,8 @@ program forge_main
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
   integer(dp) :: nenforceC3 = 0
   integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:)
+  integer(c_int64_t), allocatable :: nC2pairs(:)
+  integer(c_int32_t) :: symflags = 0
   real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp, switch_de = 0.0_dp
   integer(dp) :: nnc, it, n, nFermiLevel, nOcc, nPart, icount, i, nU
   integer(dp), allocatable :: nUnitCell_1(:), nUnitCell_2(:), nMomentaComponents(:, :)
@@ -69,6 +73,10 @@ program forge_main
   call build_cell(Coords, t1, t2, t3, g1, g12)
   call SampleBZ(nMomentaComponents, MomentaValues, numk, g1, g12)
   tn = reshape([t1, t2, t3], [2, 3])
+  ! C2_RelatedPoints (Main.f90:162)
+  allocate(nC2pairs(n))
+  nC2pairs = 0
+  if (symflags /= 0) call C2_RelatedPoints(nC2pairs, Coords, n)
   if (nenforceC3 == 1) then
     ! C3_RelatedPoints (Main.f90:163)
     allocate(nC3pairs(n), rotC3(nlayers))
@@ -121,6 +129,8 @@ contains
       call forge_check(forge_scf_set_options(ctx, opt), 'forge_scf_set_options')
       if (nenforceC3 == 1) &
         call forge_check(forge_scf_set_c3(ctx, c_loc(rotC3), c_loc(nC3pairs)), 'forge_scf_set_c3')
+      if (symflags /= 0) &
+        call forge_check(forge_scf_set_symmetry(ctx, symflags, nC2pairs), 'forge_scf_set_symmetry')
       if (trim(start) == 'first') then
         call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
       else
@@ -194,6 +204,7 @@ contains
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
       call getfock_k(1_dp, nOcc, nPart)
       zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      call symmetrize_spin(1_dp)
 
       HartreeEnergyIn = HartreeEnergy
       FockEnergyIn = FockEnergy
@@ -477,6 +488,7 @@ contains
         zFock(:, :, :, s) = (0.0_dp, 0.0_dp)
         call getfock_k(s, nO(s), nP(s))
         zFock(:, :, :, s) = (zFock(:, :, :, s) + zFockBulk(:, :, :, s)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+        call symmetrize_spin(s)
       end do
       HartreeEnergyIn = HartreeEnergy
       FockEnergyIn = FockEnergy
@@ -513,6 +525,14 @@ contains
       call write_reference_files(erow(:, 1:min(it, itmax)), transpose(crow(:, 1:min(it, itmax))), &
                                  min(it, itmax))
   end subroutine reference_sequence_2spins
+
+  ! nenforceT / nenforceC2 / nenforceC2T on spin s's new density
+  ! (Main.f90:514-534), on the device
+  subroutine symmetrize_spin(s)
+    integer(dp), intent(in) :: s
+    if (symflags /= 0) &
+      call forge_check(forge_symmetrize(ctx, zFock(:, :, :, s), symflags, nC2pairs), 'symmetrize')
+  end subroutine symmetrize_spin
 
   subroutine spin_densities()
     integer(dp) :: s
@@ -567,6 +587,9 @@ contains
       case ('refout'); refout = val
       case ('numS'); read(val, *) numS
       case ('start2'); start2 = val
+      case ('T'); read(val, *) i; if (i == 1) symflags = ior(symflags, 1_c_int32_t)
+      case ('C2'); read(val, *) i; if (i == 1) symflags = ior(symflags, 2_c_int32_t)
+      case ('C2T'); read(val, *) i; if (i == 1) symflags = ior(symflags, 4_c_int32_t)
       case ('c3'); read(val, *) nenforceC3
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)

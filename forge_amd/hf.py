@@ -192,6 +192,36 @@ class Engine:
         else:
             L.check(self.lib.forge_scf_set_c3(self.ctx, None, None), "forge_scf_set_c3")
 
+    # -- symmetry enforcement (nenforceT / nenforceC2 / nenforceC2T) ----------
+    def c2_pairs(self):
+        """nC2pairs of C2_RelatedPoints (Geometry.f90:1140-1167), 1-based."""
+        g = self.geom
+        out = np.zeros(self.n, np.int64)
+        cnt = self.lib.forge_c2_pairs(_p(g.Coords), self.n, g.nlayers, _p(out))
+        if cnt < 0:
+            raise RuntimeError("forge_c2_pairs: " + L.last_error())
+        if cnt != self.n // 2:
+            raise ValueError(f"cell is not C2-symmetric ({cnt} of {self.n // 2} pairs)")
+        return out
+
+    def _c2_table(self):
+        if getattr(self, "_c2", None) is None:
+            self._c2 = self.c2_pairs()
+        return self._c2
+
+    def symmetrize(self, zFock, flags):
+        """Main.f90:514-534 on a density: flags 1 (T) | 2 (C2) | 4 (C2T)."""
+        out = np.array(zFock, dtype=np.complex128, order="F", copy=True)
+        pairs = self._c2_table() if flags & 6 else None
+        L.check(self.lib.forge_symmetrize(self.ctx, _p(out), int(flags), _p(pairs)), "forge_symmetrize")
+        return out
+
+    def set_symmetry(self, flags):
+        """Symmetrisation of each new density in the device-resident loop."""
+        pairs = self._c2_table() if flags & 6 else None
+        L.check(self.lib.forge_scf_set_symmetry(self.ctx, int(flags), _p(pairs)),
+                "forge_scf_set_symmetry")
+
     def energies(self, zFock, dens):
         """(Ekin, EF, EH, EU) of GetKineticEnergy/GetFockEnergy/GetHartreeHubbardEnergy."""
         out = np.zeros(4)

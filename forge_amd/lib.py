@@ -54,6 +54,9 @@ _SIGS = {
     "forge_solve_c3": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "forge_getfock_c3": (C.c_int, [vp, vp, vp, vp, dbl, i64, i64, vp, vp]),
     "forge_scf_set_c3": (C.c_int, [vp, vp, vp]),
+    "forge_c2_pairs": (i64, [vp, i64, i64, vp]),
+    "forge_symmetrize": (C.c_int, [vp, vp, i32, vp]),
+    "forge_scf_set_symmetry": (C.c_int, [vp, i32, vp]),
     "forge_scf_init": (C.c_int, [vp, vp]),
     "forge_scf_init_guess": (C.c_int, [vp, vp]),
     "forge_scf_init_guess_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),

@@ -145,6 +145,24 @@ int forge_getfock_c3(forge_ctx* ctx, forge_z* zFock, const forge_z* zSortedEigen
                      int64_t PartOccStates, const int64_t* RotateLayers,
                      const int64_t* nC3pairs);
 
+/* ---- Symmetry enforcement (nenforceT / nenforceC2 / nenforceC2T,
+ * Setup.f90:28-30) ----
+ * flags: 1 = T, 2 = C2, 4 = C2T (any combination, applied in that order);
+ * nC2pairs: ndim site indices, 1-based, as C2_RelatedPoints returns them
+ * (needed for C2 and C2T). */
+
+/* C2_RelatedPoints(nC2pairs, Coords, ndim), Geometry.f90:1140-1167, on the
+ * host (no device work): within each layer, sites of the two sublattice
+ * halves at r and -r (within 1e-5) are partners.  Returns the number of pairs
+ * (ndim/2 for a C2-symmetric cell; unmatched entries are -1). */
+int64_t forge_c2_pairs(const double* Coords, int64_t ndim, int64_t nlayers, int64_t* nC2pairs);
+
+/* The symmetrisations of Main.f90:514-534 on a density (ndim x ndim x nnc,
+ * in place, on the device): T keeps the real part; C2 sets
+ * zFock = (zFock + conjg(transpose(zFock(nC2pairs, nC2pairs))))/2 and C2T the
+ * same without conjg, on every cell. */
+int forge_symmetrize(forge_ctx* ctx, forge_z* zFock, int32_t flags, const int64_t* nC2pairs);
+
 /* GetHartreeHubbardEnergy(fsumH, fsumU, LongRange, dens1, dens2, ndim),
  * HartreeFock.f90:1066-1092, with the context's device copy of LongRange:
  * out2 = {1/2 (d1+d2).L.(d1+d2), d1.d2}. */
@@ -159,6 +177,10 @@ int forge_scf_set_options(forge_ctx* ctx, const forge_scf_options* opt);
  * (Main.f90:244-250, 273-276, 375-378, 500-505).  Both pointers null: back to
  * Solve / GetFock.  Not available in sweeps. */
 int forge_scf_set_c3(forge_ctx* ctx, const int64_t* RotateLayers, const int64_t* nC3pairs);
+
+/* The symmetrisations of forge_symmetrize applied by forge_scf_run to every
+ * iteration's new density, where Main.f90:514-534 applies them (flags 0: off). */
+int forge_scf_set_symmetry(forge_ctx* ctx, int32_t flags, const int64_t* nC2pairs);
 
 /* Initial state: rho_start (ndim x ndim x nnc) as read from a checkpoint
  * (Main.f90:187-235), or NULL for the first diagonalisation

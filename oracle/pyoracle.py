@@ -296,6 +296,13 @@ class System:
         cnt = int(lib().orc_c3_pairs(ptr(self.Coords), i64(self.n), i64(self.nlayers), ptr(perm)))
         return perm, cnt
 
+    def symmetrize(self, rho, sym):
+        """orc_symmetrize: Main.f90:514-534 on a copy of rho."""
+        c2, _ = self.c2_pairs()
+        out = np.array(rho, dtype=np.complex128, order="F", copy=True)
+        lib().orc_symmetrize(self.pp, ptr(out), C.c_int(sym), ptr(c2))
+        return out
+
     def c2_pairs(self):
         """nC2pairs (C2_RelatedPoints, Geometry.f90:1140-1167), 0-based, and
         the number of pairs matched."""

@@ -10,7 +10,8 @@ through the reference's checkpoint format (nRead = 1 path); both also write
 the reference's own Energy/Convergence files (refout=); c3=1 (nenforceC3)
 reproduces the reference built with Solve_C3 / GetFock_C3; numS=2 the
 reference built with two spins (first diagonalisation and spin-split
-checkpoints).
+checkpoints); T= C2= C2T= the reference built with nenforceT / nenforceC2 /
+nenforceC2T.
 """
 import os
 import subprocess
@@ -225,3 +226,46 @@ def test_fortran_host_two_spin_trajectory_matches_reference(case, tmp_path):
     name = _reference_file_name(case)
     Ef = np.array([[float(x) for x in l.split()] for l in open(refdir / name) if l.strip()])
     np.testing.assert_array_equal(Ef, E)
+
+
+def _sym_args(case):
+    m = traj_meta(case)
+    return [f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}", f"numk={m['numk']}",
+            f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}", f"T={m['nenforceT']}",
+            f"C2={m['nenforceC2']}", f"C2T={m['nenforceC2T']}"], m
+
+
+def test_fortran_c2_pairs_at_setup(tmp_path):
+    """C2=1 runs C2_RelatedPoints at set-up (Main.f90:162) through the
+    library's host code; every site is paired, so no 'ERROR C2' (CPU)."""
+    args, _ = _sym_args("t2k3s_TC2C2T")
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", *args, f"out={tmp_path / 'g.txt'}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ERROR C2" not in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,mode", [("t1k3s_C2", "reference"), ("t2k3s_TC2C2T", "device"),
+                                       ("t1k3_T", "reference")])
+def test_fortran_host_symmetrised_trajectory_matches_reference(case, mode, tmp_path):
+    from forge_amd.checkpoint import write_checkpoint
+    from forge_amd.geometry import build_geometry
+    from forge_amd.seed import seeded_rho
+    print("fortran host symmetry", case, mode, flush=True)
+    args, m = _sym_args(case)
+    args = [FORTRAN_EXE, f"mode={mode}", *args, f"itmax={m['itmax']}", f"out={tmp_path / 'traj.txt'}"]
+    if m["start"] == "seeded":
+        g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
+        write_checkpoint(str(tmp_path / "s.ckpt"), seeded_rho(g.ndim, g.nnc, float(m["nfilling"])))
+        args.append(f"start={tmp_path / 's.ckpt'}")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
+    np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)

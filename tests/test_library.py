@@ -102,3 +102,17 @@ def test_c3_pairs_rejects_an_odd_layer_split():
     out = np.zeros(27, np.int64)
     z = np.zeros((27, 3), order="F")
     assert L.load().forge_c3_pairs(z.ctypes.data, 27, 2, out.ctypes.data) < 0
+
+
+@pytest.mark.parametrize("ntheta,nlayers", [(1, 2), (2, 2), (1, 3)])
+def test_c2_pairs_host_matches_oracle(ntheta, nlayers):
+    """forge_c2_pairs (C2_RelatedPoints, host code of the library) against the
+    oracle's, which the reference's symmetrised trajectories pin."""
+    from oracle.pyoracle import System
+    s = System(ntheta, 3, 1.0, nlayers=nlayers)
+    perm, cnt = s.c2_pairs()
+    g = build_geometry(ntheta, nlayers, 3)
+    out = np.zeros(g.ndim, np.int64)
+    got = L.load().forge_c2_pairs(g.Coords.ctypes.data, g.ndim, nlayers, out.ctypes.data)
+    assert got == cnt == g.ndim // 2
+    assert np.array_equal(out - 1, perm)
