@@ -1156,11 +1156,14 @@ static double total_2spins(const orc_params* p, const double* e) {
  * spins' densities (the reference's two checkpoint files, Main.f90:201-235).
  * traj rows as orc_scf: it, E_in, Ekin1+Ekin2, EF1+EF2, EH, EU (input state),
  * max over spins of max|dDensity|, Step, E_in - E_new, E_Fermi,
- * nOcc1 + nOcc2, nPart1 + nPart2 (Main.f90:616-639).  Returns the iterations
- * run, negative on an eigensolver error. */
+ * nOcc1 + nOcc2, nPart1 + nPart2 (Main.f90:616-639).  rot/c3perm and
+ * sym/c2perm as in orc_scf (Solve_C3 / GetFock_C3 per spin; symmetrisation
+ * of each spin's new density).  Returns the iterations run, negative on an
+ * eigensolver error. */
 i64 orc_scf2(const orc_params* p, const double* C, const double* tn, const double* MomVal,
              const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
-             const zc* rho_start, i64 itmax, double* traj, zc* rho_out, int nthreads) {
+             const zc* rho_start, i64 itmax, double* traj, zc* rho_out, int nthreads,
+             const i64* rot, const i64* c3perm, int sym, const i64* c2perm) {
   i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
   i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
   size_t nn = (size_t)n * n, tot = nn * nnc;
@@ -1198,13 +1201,18 @@ i64 orc_scf2(const orc_params* p, const double* C, const double* tn, const doubl
 
   if (rho_start == NULL) {
     memset(rhoIn, 0, sizeof(zc) * tot);
-    info = orc_solve(p, bulk, rhoIn, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads);
+    info = c3perm ? orc_solve_c3(p, rot, c3perm, bulk, rhoIn, V, Bands, zEig, C, MomVal, nMomComp,
+                                 u1, u2, tn, nthreads)
+                  : orc_solve(p, bulk, rhoIn, V, Bands, zEig, C, MomVal, nMomComp, u1, u2, tn, nthreads);
     if (info) return -1;
     orc_sort_energies(p, Bands, zEig, nMomComp, S1, zSorted, nSM, NULL);
     i64 nF1, o1, q1;
     orc_fermi(p, S1, N, &nF1, &EF, &o1, &q1, &Deg);
     memset(rho, 0, sizeof(zc) * tot);
-    orc_getfock(p, rho, zSorted, nSM, Deg, o1, q1, C, MomVal, u1, u2, 0, NULL, 0);
+    if (c3perm)
+      orc_getfock_c3(p, rot, c3perm, rho, zSorted, nSM, Deg, o1, q1, C, MomVal, u1, u2, 0, NULL, 0);
+    else
+      orc_getfock(p, rho, zSorted, nSM, Deg, o1, q1, C, MomVal, u1, u2, 0, NULL, 0);
     for (size_t q = 0; q < tot; ++q) rho[q] = (rho[q] + bulk[q]) / (zc)(double)Nk;
     memcpy(rho + tot, rho, sizeof(zc) * tot);
   } else {
@@ -1221,8 +1229,11 @@ i64 orc_scf2(const orc_params* p, const double* C, const double* tn, const doubl
   while (it < itmax) {
     ++it;
     for (int s = 0; s < 2; ++s) {
-      info = orc_solve(p, bulk + tot * s, rho + tot * s, V + n * s, Bands + N * s, zEig + n * N * s,
-                       C, MomVal, nMomComp, u1, u2, tn, nthreads);
+      info = c3perm ? orc_solve_c3(p, rot, c3perm, bulk + tot * s, rho + tot * s, V + n * s,
+                                   Bands + N * s, zEig + n * N * s, C, MomVal, nMomComp, u1, u2,
+                                   tn, nthreads)
+                    : orc_solve(p, bulk + tot * s, rho + tot * s, V + n * s, Bands + N * s,
+                                zEig + n * N * s, C, MomVal, nMomComp, u1, u2, tn, nthreads);
       if (info) return -it;
       orc_sort_energies(p, Bands + N * s, zEig + n * N * s, nMomComp, S1 + N * s,
                         zSorted + n * N * s, nSM + 2 * N * s, NULL);
@@ -1232,9 +1243,14 @@ i64 orc_scf2(const orc_params* p, const double* C, const double* tn, const doubl
     for (int s = 0; s < 2; ++s) {
       zc* r = rho + tot * s;
       memset(r, 0, sizeof(zc) * tot);
-      orc_getfock(p, r, zSorted + n * N * s, nSM + 2 * N * s, Deg, nOcc[s], nPart[s], C, MomVal,
-                  u1, u2, 0, NULL, 0);
+      if (c3perm)
+        orc_getfock_c3(p, rot, c3perm, r, zSorted + n * N * s, nSM + 2 * N * s, Deg, nOcc[s],
+                       nPart[s], C, MomVal, u1, u2, 0, NULL, 0);
+      else
+        orc_getfock(p, r, zSorted + n * N * s, nSM + 2 * N * s, Deg, nOcc[s], nPart[s], C, MomVal,
+                    u1, u2, 0, NULL, 0);
       for (size_t q = 0; q < tot; ++q) r[q] = (r[q] + bulk[tot * s + q]) / (zc)(double)Nk;
+      if (sym) orc_symmetrize(p, r, sym, c2perm);
     }
     memcpy(ein, e, sizeof(e));
     EtotIn = Etot;

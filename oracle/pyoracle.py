@@ -268,17 +268,21 @@ class System:
                                       ptr(self.longrange()), ptr(self.u1), ptr(self.u2), ptr(self.tn))
         return step, e
 
-    def scf2(self, itmax, rho_start=None, nthreads=0):
+    def scf2(self, itmax, rho_start=None, nthreads=0, c3=False, sym=0):
         """The numS = 2 SCF (reference weights, ODA): rho_start (n, n, nnc, 2)
-        or None for the first diagonalisation copied to both spins.  Returns
-        (iterations, trajectory rows, rho (n, n, nnc, 2))."""
+        or None for the first diagonalisation copied to both spins; c3 and sym
+        as in scf.  Returns (iterations, trajectory rows, rho (n, n, nnc, 2))."""
+        perm = self.c3_pairs()[0] if c3 else None
+        c2 = self.c2_pairs()[0] if sym else None
         traj = np.zeros((itmax, 12))
         rho_out = np.zeros((self.n, self.n, self.nnc, 2), np.complex128, order="F")
         A = Args()
         its = lib().orc_scf2(self.pp, ptr(self.Coords), ptr(self.tn), ptr(self.MomentaValues),
                              ptr(self.nMomentaComponents), ptr(self.u1), ptr(self.u2),
                              ptr(self.longrange()), A(rho_start, np.complex128), i64(itmax),
-                             ptr(traj), ptr(rho_out), C.c_int(nthreads))
+                             ptr(traj), ptr(rho_out), C.c_int(nthreads),
+                             ptr(self.rot) if c3 else None, ptr(perm) if c3 else None,
+                             C.c_int(sym), ptr(c2) if sym else None)
         return int(its), traj[:max(int(its), 0)], rho_out
 
     def sort_2spins(self, S1):

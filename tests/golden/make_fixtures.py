@@ -232,6 +232,50 @@ def sym_fixtures():
         print("wrote", name, flush=True)
 
 
+# Combinations: name, ntheta, numk, nfilling, itmax, start ("reference",
+# "split": one seeded checkpoint per spin), settings
+COMBO_CASES = [
+    ("t1k3_c3T", 1, 3, 1.0, 8, "reference", dict(numS=1, nenforceC3=1, nenforceT=1)),
+    ("t1k3s_s2C2", 1, 3, 1.0, 8, "split", dict(numS=2, nenforceC2=1)),
+    ("t1k3s_s2c3", 1, 3, 1.0, 8, "split", dict(numS=2, nenforceC3=1)),
+]
+
+
+def combo_fixtures():
+    """Trajectories of the reference program built with several of the
+    settings at once (C3 with T; two spins with C2 or with C3):
+    tests/golden/<case>.traj.txt.  The header lists every setting."""
+    keys = ("numS", "nenforceC3", "nenforceT", "nenforceC2", "nenforceC2T")
+    env_of = dict(numS="NUMS", nenforceC3="C3", nenforceT="ENFT", nenforceC2="ENFC2",
+                  nenforceC2T="ENFC2T")
+    for (name, ntheta, numk, nfill, itmax, start, st) in COMBO_CASES:
+        full = {k: st.get(k, 1 if k == "numS" else 0) for k in keys}
+        e = dict(os.environ, **{env_of[k]: str(v) for k, v in full.items()})
+        subprocess.run([REF_BUILD, name, str(ntheta), str(numk), f"{nfill:.1f}", str(itmax),
+                        "0" if start == "reference" else "1", "40.0", "2"], check=True,
+                       stdout=subprocess.DEVNULL, env=e)
+        rdir = os.path.join(REF_OUT, name, "run")
+        shutil.rmtree(rdir, ignore_errors=True)
+        os.makedirs(os.path.join(rdir, "output"))
+        os.makedirs(os.path.join(rdir, "dataFock"))
+        if start == "split":
+            ndim = 2 * 2 * (3 * ntheta * ntheta + 3 * ntheta + 1)
+            rhos = seeded_rho_2spins(ndim, 4, nfill) if full["numS"] == 2 else [seeded_rho(ndim, 4, nfill)]
+            for nspin, rho in enumerate(rhos, start=1):
+                fname = checkpoint_name(nspin, statename="SSS", nfilling=nfill, numS=full["numS"],
+                                        ntheta=ntheta, nlayers=2, nrelax=0, epsilon=10.0, U=4.0,
+                                        nscreen=2, xi_nm=40.0, delta=0.0, numI=1, numC=3, numk=numk)
+                write_checkpoint(os.path.join(rdir, "dataFock", fname), rho)
+        run("../forge.x > forge.log 2>&1", rdir)
+        log = open(os.path.join(rdir, "forge.log")).read()
+        if "ERROR C2" in log or "ERROR C3" in log:
+            raise RuntimeError(f"{name}: the reference reports a cell that is not symmetric")
+        write_traj(os.path.join(HERE, name + ".traj.txt"), rdir,
+                   f"ntheta={ntheta} nlayers=2 numk={numk} nfilling={nfill} xi=40.0nm "
+                   f"itmax={itmax} start={start} " + " ".join(f"{k}={v}" for k, v in full.items()))
+        print("wrote", name, flush=True)
+
+
 def checkpoint_fixture():
     """A density checkpoint written by the reference itself (nWriteFock=1,
     Main.f90:647-677) after one iteration of case t1k3."""
@@ -267,7 +311,7 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym")]
+    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym", "combo")]
     if not args or cases:
         main(cases)
     if not args or "ckpt" in args:
@@ -280,3 +324,5 @@ if __name__ == "__main__":
         spin_fixtures()
     if not args or "sym" in args:
         sym_fixtures()
+    if not args or "combo" in args:
+        combo_fixtures()

@@ -269,3 +269,30 @@ def test_fortran_host_symmetrised_trajectory_matches_reference(case, mode, tmp_p
     np.testing.assert_allclose(Cv[:, 1], Cref[:, 1], rtol=0, atol=2e-6)
     np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)
     np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["t1k3s_s2C2", "t1k3s_s2c3"])
+def test_fortran_host_two_spins_with_symmetry_matches_reference(case, tmp_path):
+    """Two spins with C2 enforcement or with the C3-reduced k-set."""
+    from forge_amd.checkpoint import write_checkpoint
+    from forge_amd.geometry import build_geometry
+    from forge_amd.seed import seeded_rho_2spins
+    print("fortran host combination", case, flush=True)
+    m = traj_meta(case)
+    g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
+    for sp, rho in enumerate(seeded_rho_2spins(g.ndim, g.nnc, float(m["nfilling"])), start=1):
+        write_checkpoint(str(tmp_path / f"s{sp}.ckpt"), rho)
+    args = [FORTRAN_EXE, "mode=reference", f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}",
+            f"numk={m['numk']}", f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}",
+            "numS=2", f"c3={m['nenforceC3']}", f"T={m['nenforceT']}", f"C2={m['nenforceC2']}",
+            f"C2T={m['nenforceC2T']}", f"itmax={m['itmax']}", f"out={tmp_path / 'traj.txt'}",
+            f"start={tmp_path / 's1.ckpt'}", f"start2={tmp_path / 's2.ckpt'}"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
+    np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)

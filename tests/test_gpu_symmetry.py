@@ -62,3 +62,20 @@ def test_scf_symmetrised_trajectory_matches_reference(case):
     np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
     np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=1e-8)
     e.close()
+
+
+def test_scf_c3_with_time_reversal_matches_reference():
+    """C3-reduced k-set and T enforcement together in the device loop."""
+    say("scf c3 + T")
+    m = traj_meta("t1k3_c3T")
+    e = Engine(Setup(ntheta=1, numk=3, nfilling=1.0, xi_nm=40.0))
+    e.set_options()
+    e.set_c3(True)
+    e.set_symmetry(1)
+    e.scf_init(None)
+    traj = e.scf_run(int(m["itmax"]))
+    E, Cv = load_traj("t1k3_c3T")
+    assert len(traj) == len(E)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=1e-8)
+    e.close()

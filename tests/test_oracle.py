@@ -298,6 +298,32 @@ def test_c2_pairs_is_the_inversion(ntheta, nlayers):
     assert np.abs(xy[perm] + xy).max() < 1e-9
 
 
+COMBO_CASES = ["t1k3_c3T", "t1k3s_s2C2", "t1k3s_s2c3"]
+
+
+@pytest.mark.parametrize("case", COMBO_CASES)
+def test_combined_settings_match_reference(case):
+    """Several settings at once (C3 with T; two spins with C2 or with C3):
+    the reference program built with all of them."""
+    from forge_amd.seed import seeded_rho_2spins
+    m = traj_meta(case)
+    s = System(int(m["ntheta"]), int(m["numk"]), float(m["nfilling"]))
+    c3, sym = m["nenforceC3"] == "1", sym_flags(m)
+    if m["numS"] == "2":
+        start = (np.stack(seeded_rho_2spins(s.n, s.nnc, float(m["nfilling"])), axis=3)
+                 if m["start"] == "split" else None)
+        its, traj, _ = s.scf2(int(m["itmax"]), rho_start=start, c3=c3, sym=sym)
+    else:
+        start = seeded_rho(s.n, s.nnc, float(m["nfilling"])) if m["start"] == "split" else None
+        its, traj, _, _ = s.scf(int(m["itmax"]), rho_start=start, c3=c3, sym=sym)
+    E, Cv = load_traj(case)
+    assert its == int(m["itmax"]) == len(E)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=2e-9)
+    np.testing.assert_allclose(traj[:, 6], Cv[:, 1], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
+
+
 def test_ft_closed_forms():
     s = system_for("t1k3")
     a0, d0 = 1 / np.sqrt(3), 1.35772
