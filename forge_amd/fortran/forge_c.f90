@@ -16,7 +16,9 @@ module forge_c
   public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
   public :: forge_sweep_create, forge_sweep_run, forge_sweep_destroy
   public :: forge_c3_pairs, forge_solve_c3, forge_getfock_c3, forge_scf_set_c3
-  public :: forge_c2_pairs, forge_symmetrize, forge_scf_set_symmetry
+  public :: forge_c2_pairs, forge_symmetrize, forge_scf_set_symmetry, forge_hartree_hubbard
+  public :: forge_scf_init_guess, forge_scf_init_guess_update, forge_scf_init_update
+  public :: forge_mlp_param_count, forge_mlp_loss_grad, forge_mlp_train, forge_mlp_infer
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -192,6 +194,72 @@ module forge_c
       integer(c_int64_t), intent(in) :: nC2pairs(*)
       integer(c_int) :: r
     end function forge_scf_set_symmetry
+    function forge_hartree_hubbard(ctx, dens1, dens2, out2) bind(C, name="forge_hartree_hubbard") &
+        result(r)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dens1(*), dens2(*)
+      real(c_double), intent(out) :: out2(*)
+      integer(c_int) :: r
+    end function forge_hartree_hubbard
+    function forge_scf_init_guess(ctx, rho_guess) bind(C, name="forge_scf_init_guess") result(r)
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: rho_guess(*)
+      integer(c_int) :: r
+    end function forge_scf_init_guess
+    function forge_scf_init_guess_update(ctx, diag, nupd, row, col, cell, delta) &
+        bind(C, name="forge_scf_init_guess_update") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: diag(*)
+      integer(c_int64_t), value :: nupd
+      integer(c_int64_t), intent(in) :: row(*), col(*), cell(*)
+      complex(c_double_complex), intent(in) :: delta(*)
+      integer(c_int) :: r
+    end function forge_scf_init_guess_update
+    function forge_scf_init_update(ctx, diag, nupd, row, col, cell, delta) &
+        bind(C, name="forge_scf_init_update") result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: diag(*)
+      integer(c_int64_t), value :: nupd
+      integer(c_int64_t), intent(in) :: row(*), col(*), cell(*)
+      complex(c_double_complex), intent(in) :: delta(*)
+      integer(c_int) :: r
+    end function forge_scf_init_update
+    function forge_mlp_param_count(din, dout) bind(C, name="forge_mlp_param_count") result(r)
+      import :: c_int64_t
+      integer(c_int64_t), value :: din, dout
+      integer(c_int64_t) :: r
+    end function forge_mlp_param_count
+    function forge_mlp_loss_grad(params, din, dout, X, T, mask, N, loss, grad) &
+        bind(C, name="forge_mlp_loss_grad") result(r)
+      import :: c_int, c_float, c_int64_t
+      real(c_float), intent(in) :: params(*), X(*), T(*), mask(*)
+      integer(c_int64_t), value :: din, dout, N
+      real(c_float), intent(out) :: loss, grad(*)
+      integer(c_int) :: r
+    end function forge_mlp_loss_grad
+    function forge_mlp_train(params, din, dout, X, T, mask, N, steps, batch, lr, seed, loss_hist) &
+        bind(C, name="forge_mlp_train") result(r)
+      import :: c_int, c_float, c_double, c_int64_t
+      real(c_float), intent(inout) :: params(*)
+      integer(c_int64_t), value :: din, dout, N, steps, batch, seed
+      real(c_float), intent(in) :: X(*), T(*), mask(*)
+      real(c_double), value :: lr
+      real(c_float), intent(out) :: loss_hist(*)
+      integer(c_int) :: r
+    end function forge_mlp_train
+    function forge_mlp_infer(params, din, dout, X, N, bf16, Y) bind(C, name="forge_mlp_infer") &
+        result(r)
+      import :: c_int, c_float, c_int32_t, c_int64_t
+      real(c_float), intent(in) :: params(*), X(*)
+      integer(c_int64_t), value :: din, dout, N
+      integer(c_int32_t), value :: bf16
+      real(c_float), intent(out) :: Y(*)
+      integer(c_int) :: r
+    end function forge_mlp_infer
     function forge_scf_set_options(ctx, opt) bind(C, name="forge_scf_set_options") result(r)
       import :: c_int, c_ptr, forge_scf_options_t
       type(c_ptr), value :: ctx

@@ -36,16 +36,6 @@ module forge_hartreefock
   type(c_ptr), save :: ctx = c_null_ptr
   integer(dp), save :: ctx_ndim = -1, ctx_nnc = -1
 
-  interface
-    function forge_hartree_hubbard(c, dens1, dens2, out2) bind(C, name="forge_hartree_hubbard") &
-        result(r)
-      import :: c_int, c_ptr, c_double
-      type(c_ptr), value :: c
-      real(c_double), intent(in) :: dens1(*), dens2(*)
-      real(c_double), intent(out) :: out2(2)
-      integer(c_int) :: r
-    end function forge_hartree_hubbard
-  end interface
 
 contains
 

@@ -296,3 +296,13 @@ def test_fortran_host_two_spins_with_symmetry_matches_reference(case, tmp_path):
     assert E.shape == Eref.shape
     np.testing.assert_allclose(E, Eref, rtol=0, atol=2e-8)
     np.testing.assert_allclose(Cv[:, 3], Cref[:, 3], rtol=0, atol=2e-8)
+
+
+def test_fortran_module_binds_every_header_symbol():
+    """forge_c.f90 has a bind(C) interface for every entry point that
+    include/forge_hf.h declares, and for nothing else."""
+    import re
+    from forge_amd import lib as L
+    src = open(os.path.join(os.path.dirname(FORTRAN_EXE), "..", "fortran", "forge_c.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(\w+)"\)', src))
+    assert bound == set(L.header_symbols())
