@@ -14,9 +14,12 @@
  * reference declares it; complex values are C99 double complex
  * (= Fortran complex(8)); indices are int64 (= integer(dp), Setup.f90:6).
  *
- * The C3-reduced k-set (nenforceC3 = 1: orc_c3_pairs, orc_solve_c3,
- * orc_getfock_c3) is pinned against the reference program built with that
- * setting (the tests/golden/ files named <case>_c3.traj.txt).
+ * The reference's variants are pinned the same way, against the reference
+ * program built with the setting (trajectory files under tests/golden/):
+ * the C3-reduced k-set (nenforceC3 = 1: orc_c3_pairs, orc_solve_c3,
+ * orc_getfock_c3; <case>_c3), two spins (numS = 2: orc_sort_2spins,
+ * orc_fermi_2spins, orc_scf2; <case>_s2), the T / C2 / C2T symmetrisations
+ * (orc_c2_pairs, orc_symmetrize) and combinations of them.
  *
  * Extensions with no reference counterpart (used by the configurations that
  * ask for them): Fermi-Dirac window occupations (occ_mode 2), physical
