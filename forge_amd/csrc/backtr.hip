@@ -273,8 +273,8 @@ static int tuned_width(const TridiagWork& t, const double* Z, size_t ldZ, size_t
       const double per = ms / nb;
       if (bt == BT_CAND[0] || per < best_per) { best_per = per; best = bt; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
   std::lock_guard<std::mutex> g(g_bt_mu);
   g_bt_width[key] = best;

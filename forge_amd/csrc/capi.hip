@@ -367,19 +367,20 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
 
 extern "C" int forge_ctx_destroy(forge_ctx* c) {
   if (!c) return 0;
-  hipStreamSynchronize(c->s);
+  // teardown: errors here leave nothing for the caller to act on
+  (void)hipStreamSynchronize(c->s);
   void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
                   c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
                   c->c3_Psi, c->c3_wph, c->c2_perm};
   for (void* q : ptrs)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   tridiag_release(c->trd);
-  for (i64 q = 0; q < c->trd.nev; ++q) hipEventDestroy(c->trd.evs[q]);
+  for (i64 q = 0; q < c->trd.nev; ++q) (void)hipEventDestroy(c->trd.evs[q]);
   delete[] c->trd.evs;
-  for (int q = 0; q < 16; ++q) hipEventDestroy(c->ev[q]);
-  hipStreamDestroy(c->s);
+  for (int q = 0; q < 16; ++q) (void)hipEventDestroy(c->ev[q]);
+  (void)hipStreamDestroy(c->s);
   delete c;
   return 0;
 }
@@ -475,9 +476,9 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
     FORGE_HIP(hipEventRecord(c->ev[5], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[5]));
     float a, b, d;
-    hipEventElapsedTime(&a, c->ev[2], c->ev[3]);
-    hipEventElapsedTime(&b, c->ev[3], c->ev[4]);
-    hipEventElapsedTime(&d, c->ev[4], c->ev[5]);
+    FORGE_HIP(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
+    FORGE_HIP(hipEventElapsedTime(&b, c->ev[3], c->ev[4]));
+    FORGE_HIP(hipEventElapsedTime(&d, c->ev[4], c->ev[5]));
     c->tm[1] += a;
     c->tm[2] += b;
     c->tm[3] += d;
@@ -495,41 +496,49 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
     return -2;
   }
   if (forge_device_count() < 1) { set_error("forge_zdiagred: no HIP device"); return -3; }
-  // a stand-alone one-matrix solve
-  hipStream_t s;
+  // a stand-alone one-matrix solve; every buffer is released on every path
+  hipStream_t s = nullptr;
   FORGE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  const size_t nn = (size_t)n * n;
   z2* H = nullptr; void* tw = nullptr; void* dw = nullptr; double* Zq = nullptr; double* ev = nullptr;
   z2* Cv = nullptr; void* bw = nullptr;
-  const i64 ld = pad_ld(n);
-  CK(dalloc((void**)&H, sizeof(z2) * ld * n));
-  CK(dalloc(&tw, tridiag_workspace_bytes(n, 1)));
-  CK(dalloc(&dw, dc_workspace_bytes(n, 1, iu)));
-  CK(dalloc((void**)&Zq, sizeof(double) * n * iu));
-  CK(dalloc((void**)&ev, sizeof(double) * n));
-  CK(dalloc((void**)&Cv, sizeof(z2) * n * iu));
-  CK(dalloc(&bw, backtr_workspace_bytes(n, 1, iu)));
   TridiagWork t{};
-  CK(tridiag_bind(t, tw, n, 1));
-  t.A = H;
-  t.sA = (size_t)ld * n;
-  t.lda = ld;
-  FORGE_HIP(hipMemcpy2DAsync(H, sizeof(z2) * ld, A, sizeof(z2) * n, sizeof(z2) * n, n,
-                             hipMemcpyHostToDevice, s));
-  CK(tridiag(t, s, nullptr, nullptr, nullptr));
-  int fails = dc_solve(t.d, t.e, n, 1, iu, ev, Zq, dw, s);
-  if (fails < 0) return fails;
-  CK(backtransform(t, Zq, n, (size_t)n * iu, iu, Cv, n, (size_t)n * iu, bw, s));
-  const i64 m = iu - il + 1;
-  FORGE_HIP(hipMemcpyAsync(A, Cv + n * (il - 1), sizeof(z2) * n * m, hipMemcpyDeviceToHost, s));
-  std::vector<double> hv(n);
-  FORGE_HIP(hipMemcpyAsync(hv.data(), ev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  FORGE_HIP(hipStreamSynchronize(s));
-  for (i64 q = 0; q < m; ++q) eigenvalues[q] = hv[il - 1 + q];
-  if (info) *info = fails;
-  hipFree(H); hipFree(tw); hipFree(dw); hipFree(Zq); hipFree(ev); hipFree(Cv); hipFree(bw);
-  hipStreamDestroy(s);
-  return fails;
+  const int rc = [&]() -> int {
+    const i64 ld = pad_ld(n);
+    CK(dalloc((void**)&H, sizeof(z2) * ld * n));
+    CK(dalloc(&tw, tridiag_workspace_bytes(n, 1)));
+    CK(dalloc(&dw, dc_workspace_bytes(n, 1, iu)));
+    CK(dalloc((void**)&Zq, sizeof(double) * n * iu));
+    CK(dalloc((void**)&ev, sizeof(double) * n));
+    CK(dalloc((void**)&Cv, sizeof(z2) * n * iu));
+    CK(dalloc(&bw, backtr_workspace_bytes(n, 1, iu)));
+    CK(tridiag_bind(t, tw, n, 1));
+    t.A = H;
+    t.sA = (size_t)ld * n;
+    t.lda = ld;
+    FORGE_HIP(hipMemcpy2DAsync(H, sizeof(z2) * ld, A, sizeof(z2) * n, sizeof(z2) * n, n,
+                               hipMemcpyHostToDevice, s));
+    CK(tridiag(t, s, nullptr, nullptr, nullptr));
+    const int fails = dc_solve(t.d, t.e, n, 1, iu, ev, Zq, dw, s);
+    if (fails < 0) return fails;
+    CK(backtransform(t, Zq, n, (size_t)n * iu, iu, Cv, n, (size_t)n * iu, bw, s));
+    const i64 m = iu - il + 1;
+    FORGE_HIP(hipMemcpyAsync(A, Cv + n * (il - 1), sizeof(z2) * n * m, hipMemcpyDeviceToHost, s));
+    std::vector<double> hv(n);
+    FORGE_HIP(hipMemcpyAsync(hv.data(), ev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    FORGE_HIP(hipStreamSynchronize(s));
+    for (i64 q = 0; q < m; ++q) eigenvalues[q] = hv[il - 1 + q];
+    if (info) *info = fails;
+    return fails;
+  }();
+  // teardown: errors here leave nothing for the caller to act on
+  (void)hipStreamSynchronize(s);
+  for (void* q : {(void*)H, tw, dw, (void*)Zq, (void*)ev, (void*)Cv, bw})
+    if (q) (void)hipFree(q);
+  tridiag_release(t);
+  for (i64 q = 0; q < t.nev; ++q) (void)hipEventDestroy(t.evs[q]);
+  delete[] t.evs;
+  (void)hipStreamDestroy(s);
+  return rc;
 }
 
 // ---------------------------------------------------------------- C3-reduced k-set
@@ -690,7 +699,7 @@ static int solve_device_c3(forge_ctx* c, const z2* rho_dev, const double* pot_de
       FORGE_HIP(hipEventRecord(c->ev[1], c->s));
       FORGE_HIP(hipEventSynchronize(c->ev[1]));
       float a;
-      hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      FORGE_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
       c->tm[0] += a;
       c->tm[7] += 1;
       c->tm[8] += a;
@@ -711,7 +720,7 @@ static int solve_device_c3(forge_ctx* c, const z2* rho_dev, const double* pot_de
       FORGE_HIP(hipEventRecord(c->ev[7], c->s));
       FORGE_HIP(hipEventSynchronize(c->ev[7]));
       float a;
-      hipEventElapsedTime(&a, c->ev[6], c->ev[7]);
+      FORGE_HIP(hipEventElapsedTime(&a, c->ev[6], c->ev[7]));
       c->tm[4] += a;
     }
     FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
@@ -754,7 +763,7 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
         FORGE_HIP(hipEventRecord(c->ev[1], c->s));
         FORGE_HIP(hipEventSynchronize(c->ev[1]));
         float a;
-        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        FORGE_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
         c->tm[0] += a;
         c->tm[7] += 1;
         c->tm[8] += a;
@@ -778,7 +787,7 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
       FORGE_HIP(hipEventRecord(c->ev[7], c->s));
       FORGE_HIP(hipEventSynchronize(c->ev[7]));
       float a;
-      hipEventElapsedTime(&a, c->ev[6], c->ev[7]);
+      FORGE_HIP(hipEventElapsedTime(&a, c->ev[6], c->ev[7]));
       c->tm[4] += a;
     }
     FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
@@ -1411,7 +1420,7 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
     FORGE_HIP(hipEventRecord(c->ev[11], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[11]));
     float a;
-    hipEventElapsedTime(&a, c->ev[10], c->ev[11]);
+    FORGE_HIP(hipEventElapsedTime(&a, c->ev[10], c->ev[11]));
     c->tm[4] += a;
     FORGE_HIP(hipEventRecord(c->ev[12], c->s));
   }
@@ -1453,7 +1462,7 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
     FORGE_HIP(hipEventRecord(c->ev[13], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[13]));
     float a;
-    hipEventElapsedTime(&a, c->ev[12], c->ev[13]);
+    FORGE_HIP(hipEventElapsedTime(&a, c->ev[12], c->ev[13]));
     c->tm[5] += a;
   }
   ++c->it;
@@ -1495,7 +1504,7 @@ extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64
   FORGE_HIP(hipEventRecord(c->ev[9], c->s));
   FORGE_HIP(hipEventSynchronize(c->ev[9]));
   float tt;
-  hipEventElapsedTime(&tt, c->ev[8], c->ev[9]);
+  FORGE_HIP(hipEventElapsedTime(&tt, c->ev[8], c->ev[9]));
   c->tm[6] = tt;
   if (iters_done) *iters_done = done;
   return fails;
@@ -1616,12 +1625,12 @@ extern "C" int forge_sweep_create(forge_ctx* const* ctxs, int64_t nsys, forge_sw
 
 extern "C" int forge_sweep_destroy(forge_sweep* w) {
   if (!w) return 0;
-  if (w->s) hipStreamSynchronize(w->s);
+  if (w->s) (void)hipStreamSynchronize(w->s);
   tridiag_release(w->trd);
-  if (w->H) hipFree(w->H);
-  if (w->S) hipFree(w->S);
-  if (w->trd_ws) hipFree(w->trd_ws);
-  if (w->s) hipStreamDestroy(w->s);
+  if (w->H) (void)hipFree(w->H);
+  if (w->S) (void)hipFree(w->S);
+  if (w->trd_ws) (void)hipFree(w->trd_ws);
+  if (w->s) (void)hipStreamDestroy(w->s);
   delete w;
   return 0;
 }

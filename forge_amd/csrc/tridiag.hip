@@ -834,7 +834,7 @@ struct TrdGraph {
 void tridiag_release(TridiagWork& w) {
   auto* g = (TrdGraph*)w.graph;
   if (!g) return;
-  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
   delete g;
   w.graph = nullptr;
 }
@@ -850,7 +850,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     const int rc = enqueue_reduction(w, s, nullptr, nullptr, nullptr, true);
     const hipError_t ce = hipStreamEndCapture(s, &graph);
     if (rc != 0) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return rc;
     }
     if (ce != hipSuccess) {
@@ -859,7 +859,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
     }
     g = new TrdGraph();
     const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
       delete g;
       set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
