@@ -173,7 +173,7 @@ struct forge_ctx {
   std::vector<double> h_da;
   int it = 0;
   double tm[16] = {};
-  hipEvent_t ev[16];
+  hipEvent_t ev[16] = {};
   // C3-reduced k-set (nenforceC3 = 1): tables built by c3_configure from
   // RotateLayers and nC3pairs; c3 = the SCF loop uses Solve_C3 / GetFock_C3
   bool c3 = false, c3_ready = false;
@@ -243,132 +243,142 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
     return -3;
   }
   forge_ctx* c = new forge_ctx();
-  c->P = *p;
-  c->pp = pair_params(*p);
-  c->n = p->ndim;
-  c->nnc = p->nnc;
-  c->numk = p->numk;
-  c->Nk = p->numk * p->numk;
-  c->nL = p->nLower;
-  c->nU = p->nUpper;
-  c->nb = p->nUpper - p->nLower + 1;
-  const i64 n = c->n, Nk = c->Nk, nnc = c->nnc;
-  c->Coords.assign(Coords, Coords + 3 * n);
-  c->tn.assign(tn, tn + 6);
-  c->MomVal.assign(MomentaValues, MomentaValues + 2 * Nk);
-  c->nMomComp.assign(nMomentaComponents, nMomentaComponents + 2 * Nk);
-  c->u1.assign(nUnitCell_1, nUnitCell_1 + nnc);
-  c->u2.assign(nUnitCell_2, nUnitCell_2 + nnc);
-  c->opt.occ_mode = 0;
-  c->opt.mix_mode = 0;
-  c->opt.beta = 0.3;
-  c->opt.etol = 0.0;
-  FORGE_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-  for (int q = 0; q < 16; ++q) FORGE_HIP(hipEventCreate(&c->ev[q]));
-  const size_t nn = (size_t)n * n;
-  CK(dalloc((void**)&c->X, sizeof(double) * 3 * n));
-  FORGE_HIP(hipMemcpy(c->X, Coords, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  c->g.n = n;
-  c->g.nnc = (int)nnc;
-  c->g.X = c->X;
-  c->g.Y = c->X + n;
-  c->g.Z = c->X + 2 * n;
-  for (int q = 0; q < MAXCELL; ++q) {
-    c->g.cells.n1[q] = q < nnc ? (double)nUnitCell_1[q] : 0.0;
-    c->g.cells.n2[q] = q < nnc ? (double)nUnitCell_2[q] : 0.0;
-  }
-  c->g.cells.t1x = tn[0];
-  c->g.cells.t1y = tn[1];
-  c->g.cells.t2x = tn[2];
-  c->g.cells.t2y = tn[3];
-  CK(dalloc((void**)&c->V, sizeof(double) * nnc * nn));
-  CK(dalloc((void**)&c->T, sizeof(double) * nnc * nn));
-  CK(pair_tables(c->g, c->pp, c->V, c->T, c->s));
-  // layer bias, HamiltonianHartreeFock.f90:119-126
-  std::vector<double> hb(n, 0.0);
-  if (p->nlayers > 1) {
-    i64 per = n / p->nlayers;
-    for (i64 l = 0; l < p->nlayers; ++l)
-      for (i64 i = 0; i < per; ++i)
-        hb[i + l * per] = (double)((float)l - (float)(p->nlayers - 1) / 2.0f) * p->Delta /
-                          (double)(float)(p->nlayers - 1);
-  }
-  CK(dalloc((void**)&c->bias, sizeof(double) * n));
-  FORGE_HIP(hipMemcpy(c->bias, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  CK(dalloc((void**)&c->pot, sizeof(double) * n));
-  FORGE_HIP(hipMemset(c->pot, 0, sizeof(double) * n));
-  CK(dalloc((void**)&c->rho, sizeof(z2) * nnc * nn));
-  CK(dalloc((void**)&c->rho_new, sizeof(z2) * (nnc + 1) * nn));
-  // k phases: assembly uses k.T_c (HartreeFock.f90:90), cells use the exact
-  // grid phase -2 pi (i1 n1 + i2 (n1 + n2)) / numk (HartreeFock.f90:471)
-  c->h_ph_asm.assign(Nk * nnc * 2, 0.0);
-  c->h_ph_cell.assign(Nk * nnc * 2, 0.0);
-  for (i64 k = 0; k < Nk; ++k) {
-    i64 i1 = nMomentaComponents[k], i2 = nMomentaComponents[k + Nk];
-    double vx = MomentaValues[i1 + c->numk * i2], vy = MomentaValues[i1 + c->numk * i2 + Nk];
-    for (i64 cc = 0; cc < nnc; ++cc) {
-      double n1 = (double)nUnitCell_1[cc], n2 = (double)nUnitCell_2[cc];
-      double th = vx * (n1 * tn[0] + n2 * tn[2]) + vy * (n1 * tn[1] + n2 * tn[3]);
-      c->h_ph_asm[(k * nnc + cc) * 2] = cos(th);
-      c->h_ph_asm[(k * nnc + cc) * 2 + 1] = sin(th);
-      double a = -2.0 * PI *
-                 (double)(i1 * nUnitCell_1[cc] + i2 * nUnitCell_1[cc] + i2 * nUnitCell_2[cc]) /
-                 (double)c->numk;
-      c->h_ph_cell[(k * nnc + cc) * 2] = cos(a);
-      c->h_ph_cell[(k * nnc + cc) * 2 + 1] = sin(a);
+  // set-up; a failure part-way releases what was made (forge_ctx_destroy
+  // accepts a partly built context)
+  const int rc = [&]() -> int {
+    c->P = *p;
+    c->pp = pair_params(*p);
+    c->n = p->ndim;
+    c->nnc = p->nnc;
+    c->numk = p->numk;
+    c->Nk = p->numk * p->numk;
+    c->nL = p->nLower;
+    c->nU = p->nUpper;
+    c->nb = p->nUpper - p->nLower + 1;
+    const i64 n = c->n, Nk = c->Nk, nnc = c->nnc;
+    c->Coords.assign(Coords, Coords + 3 * n);
+    c->tn.assign(tn, tn + 6);
+    c->MomVal.assign(MomentaValues, MomentaValues + 2 * Nk);
+    c->nMomComp.assign(nMomentaComponents, nMomentaComponents + 2 * Nk);
+    c->u1.assign(nUnitCell_1, nUnitCell_1 + nnc);
+    c->u2.assign(nUnitCell_2, nUnitCell_2 + nnc);
+    c->opt.occ_mode = 0;
+    c->opt.mix_mode = 0;
+    c->opt.beta = 0.3;
+    c->opt.etol = 0.0;
+    FORGE_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    for (int q = 0; q < 16; ++q) FORGE_HIP(hipEventCreate(&c->ev[q]));
+    const size_t nn = (size_t)n * n;
+    CK(dalloc((void**)&c->X, sizeof(double) * 3 * n));
+    FORGE_HIP(hipMemcpy(c->X, Coords, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    c->g.n = n;
+    c->g.nnc = (int)nnc;
+    c->g.X = c->X;
+    c->g.Y = c->X + n;
+    c->g.Z = c->X + 2 * n;
+    for (int q = 0; q < MAXCELL; ++q) {
+      c->g.cells.n1[q] = q < nnc ? (double)nUnitCell_1[q] : 0.0;
+      c->g.cells.n2[q] = q < nnc ? (double)nUnitCell_2[q] : 0.0;
     }
+    c->g.cells.t1x = tn[0];
+    c->g.cells.t1y = tn[1];
+    c->g.cells.t2x = tn[2];
+    c->g.cells.t2y = tn[3];
+    CK(dalloc((void**)&c->V, sizeof(double) * nnc * nn));
+    CK(dalloc((void**)&c->T, sizeof(double) * nnc * nn));
+    CK(pair_tables(c->g, c->pp, c->V, c->T, c->s));
+    // layer bias, HamiltonianHartreeFock.f90:119-126
+    std::vector<double> hb(n, 0.0);
+    if (p->nlayers > 1) {
+      i64 per = n / p->nlayers;
+      for (i64 l = 0; l < p->nlayers; ++l)
+        for (i64 i = 0; i < per; ++i)
+          hb[i + l * per] = (double)((float)l - (float)(p->nlayers - 1) / 2.0f) * p->Delta /
+                            (double)(float)(p->nlayers - 1);
+    }
+    CK(dalloc((void**)&c->bias, sizeof(double) * n));
+    FORGE_HIP(hipMemcpy(c->bias, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    CK(dalloc((void**)&c->pot, sizeof(double) * n));
+    FORGE_HIP(hipMemset(c->pot, 0, sizeof(double) * n));
+    CK(dalloc((void**)&c->rho, sizeof(z2) * nnc * nn));
+    CK(dalloc((void**)&c->rho_new, sizeof(z2) * (nnc + 1) * nn));
+    // k phases: assembly uses k.T_c (HartreeFock.f90:90), cells use the exact
+    // grid phase -2 pi (i1 n1 + i2 (n1 + n2)) / numk (HartreeFock.f90:471)
+    c->h_ph_asm.assign(Nk * nnc * 2, 0.0);
+    c->h_ph_cell.assign(Nk * nnc * 2, 0.0);
+    for (i64 k = 0; k < Nk; ++k) {
+      i64 i1 = nMomentaComponents[k], i2 = nMomentaComponents[k + Nk];
+      double vx = MomentaValues[i1 + c->numk * i2], vy = MomentaValues[i1 + c->numk * i2 + Nk];
+      for (i64 cc = 0; cc < nnc; ++cc) {
+        double n1 = (double)nUnitCell_1[cc], n2 = (double)nUnitCell_2[cc];
+        double th = vx * (n1 * tn[0] + n2 * tn[2]) + vy * (n1 * tn[1] + n2 * tn[3]);
+        c->h_ph_asm[(k * nnc + cc) * 2] = cos(th);
+        c->h_ph_asm[(k * nnc + cc) * 2 + 1] = sin(th);
+        double a = -2.0 * PI *
+                   (double)(i1 * nUnitCell_1[cc] + i2 * nUnitCell_1[cc] + i2 * nUnitCell_2[cc]) /
+                   (double)c->numk;
+        c->h_ph_cell[(k * nnc + cc) * 2] = cos(a);
+        c->h_ph_cell[(k * nnc + cc) * 2 + 1] = sin(a);
+      }
+    }
+    CK(dalloc((void**)&c->ph_asm, sizeof(double) * Nk * nnc * 2));
+    CK(dalloc((void**)&c->ph_cell, sizeof(double) * Nk * nnc * 2));
+    FORGE_HIP(hipMemcpy(c->ph_asm, c->h_ph_asm.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
+    FORGE_HIP(hipMemcpy(c->ph_cell, c->h_ph_cell.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
+    // eigensolver batch size from free memory
+    size_t fr = 0, tot = 0;
+    FORGE_HIP(hipMemGetInfo(&fr, &tot));
+    const i64 nU = c->nU;
+    size_t per_k = sizeof(z2) * pad_ld(n) * n + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
+                   sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
+    size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + ((size_t)64 << 20);
+    size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
+    i64 Kb = (i64)(budget / per_k);
+    if (Kb < 1) {
+      set_error("forge_ctx_create: not enough device memory for one k-point");
+      return -4;
+    }
+    c->Kb = (int)std::min<i64>(Kb, Nk);
+    const int Kb_ = c->Kb;
+    c->ldH = pad_ld(n);
+    CK(dalloc((void**)&c->H, sizeof(z2) * c->ldH * n * Kb_));
+    CK(dalloc(&c->trd_ws, tridiag_workspace_bytes(n, Kb_)));
+    CK(tridiag_bind(c->trd, c->trd_ws, n, Kb_));
+    c->trd.A = c->H;
+    c->trd.sA = (size_t)c->ldH * n;
+    c->trd.lda = c->ldH;
+    CK(dalloc(&c->dc_ws, dc_workspace_bytes(n, Kb_, nU)));
+    CK(dalloc((void**)&c->Zq, sizeof(double) * n * nU * Kb_));
+    CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
+    CK(dalloc((void**)&c->Cv, sizeof(z2) * n * nU * Kb_));
+    CK(dalloc(&c->bt_ws, backtr_workspace_bytes(n, Kb_, nU)));
+    CK(dalloc((void**)&c->Cwin, sizeof(z2) * n * c->nb * Nk));
+    CK(dalloc((void**)&c->Psi, sizeof(z2) * n * c->nb * Nk));
+    CK(dalloc((void**)&c->wph, sizeof(double) * c->nb * Nk * nnc * 2));
+    CK(dalloc((void**)&c->gidx, sizeof(int) * c->nb * Nk));
+    CK(dalloc((void**)&c->part, sizeof(double) * 5 * 2048));
+    CK(dalloc((void**)&c->e5, sizeof(double) * 16));
+    CK(dalloc((void**)&c->dots, sizeof(double) * 16));
+    CK(dalloc((void**)&c->da, sizeof(double) * n));
+    CK(dalloc((void**)&c->db, sizeof(double) * n));
+    CK(dalloc((void**)&c->ya, sizeof(double) * n));
+    CK(dalloc((void**)&c->yb, sizeof(double) * n));
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    return 0;
+  }();
+  if (rc != 0) {
+    forge_ctx_destroy(c);
+    return rc;
   }
-  CK(dalloc((void**)&c->ph_asm, sizeof(double) * Nk * nnc * 2));
-  CK(dalloc((void**)&c->ph_cell, sizeof(double) * Nk * nnc * 2));
-  FORGE_HIP(hipMemcpy(c->ph_asm, c->h_ph_asm.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
-  FORGE_HIP(hipMemcpy(c->ph_cell, c->h_ph_cell.data(), sizeof(double) * Nk * nnc * 2, hipMemcpyHostToDevice));
-  // eigensolver batch size from free memory
-  size_t fr = 0, tot = 0;
-  FORGE_HIP(hipMemGetInfo(&fr, &tot));
-  const i64 nU = c->nU;
-  size_t per_k = sizeof(z2) * pad_ld(n) * n + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
-                 sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
-  size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + ((size_t)64 << 20);
-  size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
-  i64 Kb = (i64)(budget / per_k);
-  if (Kb < 1) {
-    set_error("forge_ctx_create: not enough device memory for one k-point");
-    return -4;
-  }
-  c->Kb = (int)std::min<i64>(Kb, Nk);
-  const int Kb_ = c->Kb;
-  c->ldH = pad_ld(n);
-  CK(dalloc((void**)&c->H, sizeof(z2) * c->ldH * n * Kb_));
-  CK(dalloc(&c->trd_ws, tridiag_workspace_bytes(n, Kb_)));
-  CK(tridiag_bind(c->trd, c->trd_ws, n, Kb_));
-  c->trd.A = c->H;
-  c->trd.sA = (size_t)c->ldH * n;
-  c->trd.lda = c->ldH;
-  CK(dalloc(&c->dc_ws, dc_workspace_bytes(n, Kb_, nU)));
-  CK(dalloc((void**)&c->Zq, sizeof(double) * n * nU * Kb_));
-  CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
-  CK(dalloc((void**)&c->Cv, sizeof(z2) * n * nU * Kb_));
-  CK(dalloc(&c->bt_ws, backtr_workspace_bytes(n, Kb_, nU)));
-  CK(dalloc((void**)&c->Cwin, sizeof(z2) * n * c->nb * Nk));
-  CK(dalloc((void**)&c->Psi, sizeof(z2) * n * c->nb * Nk));
-  CK(dalloc((void**)&c->wph, sizeof(double) * c->nb * Nk * nnc * 2));
-  CK(dalloc((void**)&c->gidx, sizeof(int) * c->nb * Nk));
-  CK(dalloc((void**)&c->part, sizeof(double) * 5 * 2048));
-  CK(dalloc((void**)&c->e5, sizeof(double) * 16));
-  CK(dalloc((void**)&c->dots, sizeof(double) * 16));
-  CK(dalloc((void**)&c->da, sizeof(double) * n));
-  CK(dalloc((void**)&c->db, sizeof(double) * n));
-  CK(dalloc((void**)&c->ya, sizeof(double) * n));
-  CK(dalloc((void**)&c->yb, sizeof(double) * n));
-  FORGE_HIP(hipStreamSynchronize(c->s));
   *out = c;
   return 0;
 }
 
 extern "C" int forge_ctx_destroy(forge_ctx* c) {
   if (!c) return 0;
-  // teardown: errors here leave nothing for the caller to act on
-  (void)hipStreamSynchronize(c->s);
+  // teardown (also of a partly built context): errors here leave nothing
+  // for the caller to act on
+  if (c->s) (void)hipStreamSynchronize(c->s);
   void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
@@ -379,8 +389,9 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
   tridiag_release(c->trd);
   for (i64 q = 0; q < c->trd.nev; ++q) (void)hipEventDestroy(c->trd.evs[q]);
   delete[] c->trd.evs;
-  for (int q = 0; q < 16; ++q) (void)hipEventDestroy(c->ev[q]);
-  (void)hipStreamDestroy(c->s);
+  for (int q = 0; q < 16; ++q)
+    if (c->ev[q]) (void)hipEventDestroy(c->ev[q]);
+  if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return 0;
 }
