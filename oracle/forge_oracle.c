@@ -54,6 +54,8 @@ extern void zgemm_(const char*, const char*, const int*, const int*, const int*,
                    const zc*, const int*, const zc*, const int*, const zc*, zc*, const int*,
                    size_t, size_t);
 extern double dlamch_(const char*, size_t);
+extern void zheev_(const char*, const char*, const int*, zc*, const int*, double*, zc*, const int*,
+                   double*, int*, size_t, size_t);
 
 static const double PI = 3.14159265358979323846264338327950288;
 
@@ -812,6 +814,113 @@ void orc_symmetrize(const orc_params* p, zc* rho, int sym, const i64* c2) {
   free(t);
 }
 
+/* ------------------------------------------------------------------ valley projection */
+
+/* ftvalley, TightBinding.f90:235-270: i sign(cos 3 phi) / sqrt(27) for sites
+ * 0.2 < |r_i - r_j| < 1.2 apart in the plane, phi the angle of
+ * R (r_i - r_j) (R the 2x2 rotation given row-major), else 0. */
+static zc ftvalley(const double* ri, const double* rj, const double* R) {
+  double dx = ri[0] - rj[0], dy = ri[1] - rj[1];
+  double rij = sqrt(dx * dx + dy * dy);
+  if (!(rij < 1.2 && rij > 0.2)) return 0.0;
+  double x = R[0] * dx + R[1] * dy, y = R[2] * dx + R[3] * dy, phi;
+  if (x > 0) phi = atan(y / x);
+  else if (x < 0 && y > 0) phi = atan(y / x) + PI;
+  else phi = -PI + atan(y / x);
+  if (x == 0) phi = y > 0 ? PI / 2.0 : -PI / 2.0;
+  return I * copysign(1.0, cos(3.0 * phi)) / sqrt(27.0);
+}
+
+/* ValleyPhase, TightBinding.f90:60-165, for fphase = 0: V(i,j) over the
+ * sites i, j of one sublattice half of one layer, home cell plus both
+ * images in every other stored cell, with R = RotMatrix for layers rotated
+ * +1 and its transpose for -1 (other layers contribute nothing).
+ * RotMatrix = [[c, s], [-s, c]], c, s = cos, sin(acos(cs)/2) (Main.f90:133). */
+static void valley_phase(const orc_params* p, const i64* rot, const double* C, const double* tn,
+                         const i64* u1, const i64* u2, const double* vk, zc* V) {
+  i64 n = p->ndim, per = n / p->nlayers, half = per / 2;
+  double aM = (double)(n / (2 * p->nlayers));  /* 3 ntheta^2 + 3 ntheta + 1 */
+  double cs = 1.0 - 1.0 / (2.0 * aM);
+  double c = cos(0.5 * acos(cs)), sn = sin(0.5 * acos(cs));
+  const double Rp[4] = {c, sn, -sn, c}, Rm[4] = {c, -sn, sn, c};
+  memset(V, 0, sizeof(zc) * n * n);
+  for (i64 l = 0; l < p->nlayers; ++l) {
+    const double* R = rot[l] == -1 ? Rm : (rot[l] == 1 ? Rp : NULL);
+    if (!R) continue;
+    for (int hf = 0; hf < 2; ++hf) {
+      i64 lo = l * per + hf * half, hi = lo + half;
+      for (i64 i = lo; i < hi; ++i)
+        for (i64 j = lo; j < hi; ++j) {
+          double ri[2] = {C[i], C[i + n]}, rj[2] = {C[j], C[j + n]};
+          zc v = ftvalley(ri, rj, R);
+          for (i64 cc = 1; cc < p->nnc; ++cc) {
+            double Tx = (double)u1[cc] * tn[0] + (double)u2[cc] * tn[2];
+            double Ty = (double)u1[cc] * tn[1] + (double)u2[cc] * tn[3];
+            double ph = vk[0] * Tx + vk[1] * Ty;
+            double rjs[2] = {rj[0] - Tx, rj[1] - Ty}, ris[2] = {ri[0] - Tx, ri[1] - Ty};
+            v += ftvalley(ri, rjs, R) * cexp(-I * ph);
+            v += ftvalley(ris, rj, R) * cexp(I * ph);
+          }
+          V[i + n * j] += v;
+        }
+    }
+  }
+}
+
+/* ValleyTransform, TightBinding.f90:170-211: per k, P = C^H V C over the
+ * window columns, zheev('V', 'L') of P (zdiagFull, LapackRoutines.f90:498-533),
+ * S = U sign(Lambda) U^H (unitarize, LapackRoutines.f90:865-893; sign(1, x)
+ * keeps the sign of a negative zero), C <- C S, each product as the
+ * reference's BLAS call forms it (the SCF amplifies their rounding
+ * differences).  zEig: n x nb x Nk. */
+void orc_valley_transform(const orc_params* p, const i64* rot, const double* C, const double* tn,
+                          const double* MomVal, const i64* nMomComp, const i64* u1, const i64* u2,
+                          zc* zEig) {
+  i64 n = p->ndim, numk = p->numk, Nk = numk * numk, nb = p->nUpper - p->nLower + 1;
+  int ni = (int)n, mb = (int)nb;
+  zc* V = (zc*)malloc(sizeof(zc) * n * n);
+  zc* D = (zc*)malloc(sizeof(zc) * n * nb);
+  zc* Cc = (zc*)malloc(sizeof(zc) * n * nb);
+  zc* P = (zc*)malloc(sizeof(zc) * nb * nb);
+  zc* A = (zc*)malloc(sizeof(zc) * nb * nb);
+  zc* Bd = (zc*)malloc(sizeof(zc) * nb * nb);
+  zc* S = (zc*)malloc(sizeof(zc) * nb * nb);
+  zc* T = (zc*)malloc(sizeof(zc) * n * nb);
+  double* w = (double*)malloc(sizeof(double) * nb);
+  double* rwork = (double*)malloc(sizeof(double) * 3 * nb);
+  const zc one = 1.0, zero = 0.0;
+  for (i64 k = 0; k < Nk; ++k) {
+    i64 i1 = nMomComp[k], i2 = nMomComp[k + Nk];
+    double vk[2] = {MomVal[i1 + numk * i2], MomVal[i1 + numk * i2 + Nk]};
+    zc* Ck = zEig + n * nb * k;
+    valley_phase(p, rot, C, tn, u1, u2, vk, V);
+    /* matrix_elements (LapackRoutines.f90:257-265): D = V C, P = conjg(C)^T D */
+    zgemm_("N", "N", &ni, &mb, &ni, &one, V, &ni, Ck, &ni, &zero, D, &ni, 1, 1);
+    for (i64 q = 0; q < n * nb; ++q) Cc[q] = conj(Ck[q]);
+    zgemm_("T", "N", &mb, &mb, &ni, &one, Cc, &ni, D, &ni, &zero, P, &mb, 1, 1);
+    /* unitarize: zheev('V', 'L'), B = diag(sign w), A = U^H, S = conjg(A)^T (B A) */
+    int lwork = -1, info;
+    zc wq;
+    zheev_("V", "L", &mb, P, &mb, w, &wq, &lwork, rwork, &info, 1, 1);
+    lwork = (int)creal(wq);
+    zc* work = (zc*)malloc(sizeof(zc) * (lwork > 1 ? lwork : 1));
+    zheev_("V", "L", &mb, P, &mb, w, work, &lwork, rwork, &info, 1, 1);
+    free(work);
+    for (i64 b = 0; b < nb; ++b)
+      for (i64 a = 0; a < nb; ++a) {
+        A[a + nb * b] = conj(P[b + nb * a]);
+        Bd[a + nb * b] = a == b ? copysign(1.0, w[a]) : 0.0;
+      }
+    zgemm_("N", "N", &mb, &mb, &mb, &one, Bd, &mb, A, &mb, &zero, Cc, &mb, 1, 1);
+    for (i64 q = 0; q < nb * nb; ++q) A[q] = conj(A[q]);
+    zgemm_("T", "N", &mb, &mb, &mb, &one, A, &mb, Cc, &mb, &zero, S, &mb, 1, 1);
+    /* matrix_mul: C S */
+    zgemm_("N", "N", &ni, &mb, &mb, &one, Ck, &ni, S, &mb, &zero, T, &ni, 1, 1);
+    memcpy(Ck, T, sizeof(zc) * n * nb);
+  }
+  free(V); free(D); free(Cc); free(P); free(A); free(Bd); free(S); free(T); free(w); free(rwork);
+}
+
 /* ------------------------------------------------------------------ energies */
 
 /* GetKineticEnergy, HartreeFock.f90:1012-1059. */
@@ -956,14 +1065,17 @@ double orc_optimal_step(const orc_params* p, double* e_inout, const zc* rhoNew, 
  *               273-276, 375-378, 500-505): Solve_C3 / GetFock_C3 with the
  *               layer rotation signs and the orc_c3_pairs permutation
  *   sym, c2perm nenforceT / C2 / C2T bits (orc_symmetrize) applied to each
- *               iteration's new density (Main.f90:514-534), C2 partners
+ *               iteration's new density (Main.f90:514-534), C2 partners;
+ *               bit 8: nenforceValley (orc_valley_transform with the layer
+ *               rotation signs rot_v; reference weights)
  * Returns the number of iterations run (negative on an eigensolver error or,
  * with c3perm, numk % 3 != 0). */
 i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double* MomVal,
             const i64* nMomComp, const i64* u1, const i64* u2, const double* L,
             const zc* rho_start, i64 itmax, int occ_mode, double kT, int mix_mode, double beta,
             double etol, double* traj, zc* rho_out, double* timing, int nthreads,
-            double* bands_last, const i64* rot, const i64* c3perm, int sym, const i64* c2perm) {
+            double* bands_last, const i64* rot, const i64* c3perm, int sym, const i64* c2perm,
+            const i64* rot_v) {
   i64 n = p->ndim, nnc = p->nnc, Nk = p->numk * p->numk;
   i64 nb = p->nUpper - p->nLower + 1, N = nb * Nk;
   size_t nn = (size_t)n * n, tot = nn * nnc;
@@ -1034,8 +1146,21 @@ i64 orc_scf(const orc_params* p, const double* C, const double* tn, const double
     info = SOLVE(rho);
     tsolve += omp_get_wtime() - t;
     if (info) return -it;
-    WINDOW_DENSITY();
-    if (sym) orc_symmetrize(p, rho, sym, c2perm);
+    if (sym & 8) {
+      /* nenforceValley (Main.f90:469-494): GetFock, ValleyTransform, sort,
+       * GetFock again, rho = (rho / 2 + bulk) / Nk */
+      orc_sort_energies(p, Bands, zEig, nMomComp, S, zSorted, nSM, NULL);
+      orc_fermi(p, S, N, &nF, &EF, &nOcc, &nPart, &Deg);
+      memset(rho, 0, sizeof(zc) * tot);
+      GETFOCK(occ_mode, NULL, 0);
+      orc_valley_transform(p, rot_v, C, tn, MomVal, nMomComp, u1, u2, zEig);
+      orc_sort_energies(p, Bands, zEig, nMomComp, S, zSorted, nSM, NULL);
+      GETFOCK(occ_mode, NULL, 0);
+      for (size_t q = 0; q < tot; ++q) rho[q] = (0.5 * rho[q] + bulk[q]) / (zc)(double)Nk;
+    } else {
+      WINDOW_DENSITY();
+    }
+    if (sym & 7) orc_symmetrize(p, rho, sym & 7, c2perm);
     memcpy(ein, e, sizeof(e));
     EtotIn = Etot;
     for (i64 a = 0; a < n; ++a) D[a] = creal(rho[a + n * a]);

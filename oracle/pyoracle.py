@@ -319,9 +319,9 @@ class System:
         """keep_bands: the last iteration's window bands (numb x Nk) are left
         in self.last_bands.  c3: the reference's nenforceC3 = 1 path
         (Solve_C3 / GetFock_C3).  sym: nenforceT (1) | nenforceC2 (2) |
-        nenforceC2T (4)."""
+        nenforceC2T (4) | nenforceValley (8)."""
         c2 = None
-        if sym:
+        if sym & 7:
             c2, cnt = self.c2_pairs()
             if cnt != self.n // 2:
                 raise ValueError(f"cell is not C2-symmetric ({cnt} of {self.n // 2} pairs)")
@@ -343,7 +343,7 @@ class System:
                             dbl(etol), ptr(traj), ptr(rho_out), ptr(timing), C.c_int(nthreads),
                             ptr(bands) if keep_bands else None,
                             ptr(self.rot) if c3 else None, ptr(perm) if c3 else None,
-                            C.c_int(sym), ptr(c2) if sym else None)
+                            C.c_int(sym), ptr(c2) if sym & 7 else None, ptr(self.rot))
         if keep_bands:
             self.last_bands = bands
         return int(its), traj[:max(int(its), 0)], rho_out, timing

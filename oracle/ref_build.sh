@@ -17,11 +17,12 @@
 # compile.sh:1's file order is also wrong (Geometry.f90:5 uses TightBinding),
 # so the modules are compiled in dependency order here.
 #
-# Usage: [WRITEFOCK=1] [PRINTOUT=1] [C3=1] [NUMS=2] [ENFT=1] [ENFC2=1] [ENFC2T=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
+# Usage: [WRITEFOCK=1] [PRINTOUT=1] [C3=1] [NUMS=2] [ENFT=1] [ENFC2=1] [ENFC2T=1] [ENFV=1] oracle/ref_build.sh CASE NTHETA NUMK NFILLING ITMAX NREAD XI_NM [NLAYERS]
 #   PRINTOUT=1 keeps the reference's nPrintOutputs = 1 (the Bands file, Main.f90:601-608).
 #   C3=1 sets nenforceC3 = 1 (Setup.f90:27): Solve_C3 / GetFock_C3 replace Solve / GetFock.
 #   NUMS=2 sets numS = numSIn = 2 (Setup.f90:13, 41): independent up/down spins.
 #   ENFT / ENFC2 / ENFC2T = 1 set nenforceT / nenforceC2 / nenforceC2T (Setup.f90:28-30).
+#   ENFV=1 sets nenforceValley = 1 (Setup.f90:31).
 #   XI_NM is xi*0.246 (gate parameter in nm).  Toolchain: amdflang (ROCm) and
 #   MKL LP64 sequential from /opt/conda/lib.
 set -euo pipefail
@@ -33,6 +34,7 @@ NUMS=${NUMS:-1}
 ENFT=${ENFT:-0}
 ENFC2=${ENFC2:-0}
 ENFC2T=${ENFC2T:-0}
+ENFV=${ENFV:-0}
 REF=/root/reference
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref/$CASE
@@ -62,6 +64,7 @@ sed -i \
   -e "s/^integer(dp), parameter:: nenforceT = .*/integer(dp), parameter:: nenforceT = $ENFT/" \
   -e "s/^integer(dp), parameter:: nenforceC2 = .*/integer(dp), parameter:: nenforceC2 = $ENFC2/" \
   -e "s/^integer(dp), parameter:: nenforceC2T = .*/integer(dp), parameter:: nenforceC2T = $ENFC2T/" \
+  -e "s/^integer(dp), parameter:: nenforceValley = .*/integer(dp), parameter:: nenforceValley = $ENFV/" \
   -e "s/^integer(dp), parameter :: numSIn = .*/integer(dp), parameter :: numSIn = $NUMS/" \
   -e "s/^real(dp), parameter:: xi = .*/real(dp), parameter:: xi = ${XINM}_dp\/0.246_dp/" \
   -e "s/^real(dp), parameter:: xiIn = .*/real(dp), parameter:: xiIn = ${XINM}_dp\/0.246_dp/" \
@@ -73,6 +76,7 @@ grep -q "numS = $NUMS\$" "$SCR/Setup.f90"
 grep -q "nenforceT = $ENFT\$" "$SCR/Setup.f90"
 grep -q "nenforceC2 = $ENFC2\$" "$SCR/Setup.f90"
 grep -q "nenforceC2T = $ENFC2T\$" "$SCR/Setup.f90"
+grep -q "nenforceValley = $ENFV\$" "$SCR/Setup.f90"
 sed -i \
   -e '58s/nOccStates(numS), nPartOccStates(numS)/nOccStates(2), nPartOccStates(2)/' \
   -e '635s/KineticEnergyIn(2)/KineticEnergyIn(numS)/' \

@@ -238,16 +238,19 @@ COMBO_CASES = [
     ("t1k3_c3T", 1, 3, 1.0, 8, "reference", dict(numS=1, nenforceC3=1, nenforceT=1)),
     ("t1k3s_s2C2", 1, 3, 1.0, 8, "split", dict(numS=2, nenforceC2=1)),
     ("t1k3s_s2c3", 1, 3, 1.0, 8, "split", dict(numS=2, nenforceC3=1)),
+    ("t1k3_V", 1, 3, 1.0, 6, "reference", dict(numS=1, nenforceValley=1)),
+    ("t1k3s_V", 1, 3, 1.0, 8, "split", dict(numS=1, nenforceValley=1)),
 ]
 
 
 def combo_fixtures():
     """Trajectories of the reference program built with several of the
-    settings at once (C3 with T; two spins with C2 or with C3):
-    tests/golden/<case>.traj.txt.  The header lists every setting."""
-    keys = ("numS", "nenforceC3", "nenforceT", "nenforceC2", "nenforceC2T")
+    settings at once (C3 with T; two spins with C2 or with C3), and with the
+    valley projection: tests/golden/<case>.traj.txt.  The header lists every
+    setting."""
+    keys = ("numS", "nenforceC3", "nenforceT", "nenforceC2", "nenforceC2T", "nenforceValley")
     env_of = dict(numS="NUMS", nenforceC3="C3", nenforceT="ENFT", nenforceC2="ENFC2",
-                  nenforceC2T="ENFC2T")
+                  nenforceC2T="ENFC2T", nenforceValley="ENFV")
     for (name, ntheta, numk, nfill, itmax, start, st) in COMBO_CASES:
         full = {k: st.get(k, 1 if k == "numS" else 0) for k in keys}
         e = dict(os.environ, **{env_of[k]: str(v) for k, v in full.items()})
