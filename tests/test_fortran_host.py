@@ -307,3 +307,53 @@ def test_fortran_module_binds_every_header_symbol():
     src = open(os.path.join(os.path.dirname(FORTRAN_EXE), "..", "fortran", "forge_c.f90")).read()
     bound = set(re.findall(r'bind\(C,\s*name="(forge_\w+)"\)', src))
     assert bound == set(L.header_symbols())
+
+
+SORT2_PROGRAM = """
+program sort2
+  use forge_setup, only: dp
+  use forge_hartreefock, only: SortEnergies_2Spins
+  implicit none
+  integer(dp) :: numk_, numb_, W(2, 2), nk2, q
+  real(dp), allocatable :: S1(:, :), S2(:, :)
+  read(*, *) numk_, numb_, W
+  nk2 = numk_ * numk_
+  allocate(S1(numb_ * nk2, 2), S2(2 * numb_ * nk2, 2))
+  read(*, *) S1
+  call SortEnergies_2Spins(numk_, numb_, W, S1, S2)
+  do q = 1, 2 * numb_ * nk2
+    write(*, '(ES25.17, 1X, F3.0)') S2(q, 1), S2(q, 2)
+  end do
+end program sort2
+"""
+
+
+def test_fortran_sort_2spins_matches_oracle(tmp_path):
+    """The host's SortEnergies_2Spins (forge_hartreefock.f90) on the
+    constructed spectra that pin the oracle's restatement of
+    HartreeFock.f90:929-1004 (its spin-2 lead-in quirk included), and on
+    random spectra with ties: the same rows, bit for bit (CPU; the program
+    links the library but makes no device call)."""
+    from forge_amd.build import FLANG, FORTRAN, FSRC, OUT
+    from oracle.pyoracle import System
+    src = tmp_path / "sort2.f90"
+    src.write_text(SORT2_PROGRAM)
+    exe = tmp_path / "sort2"
+    r = subprocess.run([FLANG, "-O2", "-ffp-contract=off", "-module-dir", str(tmp_path),
+                        *[os.path.join(FSRC, f) for f in FORTRAN if f != "forge_main.f90"],
+                        str(src), "-o", str(exe), f"-L{OUT}", "-lforge_hip", f"-Wl,-rpath,{OUT}"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s = System(1, 3, 1.0)
+    N = 180
+    NP = (s.n // 2 - s.p.nLower) + 1
+    rng = np.random.default_rng(5)
+    ties = np.sort(np.round(rng.standard_normal((N, 2)), 1), axis=0)
+    for S1 in (np.stack([2.0 * np.arange(N), 2.0 * np.arange(N) + 1.0], 1), ties):
+        S1 = np.asfortranarray(S1)
+        inp = f"3 20 {NP - 1} {NP + 3} {NP - 1} {NP + 3}\n" + " ".join(repr(float(x)) for x in S1.ravel(order="F"))
+        out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0, out.stderr
+        got = np.array([[float(v) for v in l.split()] for l in out.stdout.splitlines() if l.strip()])
+        ref = s.sort_2spins(S1)
+        assert np.array_equal(got, ref)
