@@ -194,6 +194,12 @@ struct forge_ctx {
   // nenforceT / C2 / C2T of the SCF loop (bits 1, 2, 4) and the C2 partners
   int sym = 0;
   int* c2_perm = nullptr;
+  // nenforceValley: the loop's window density is the mean of the plain and
+  // the valley-projected states' (Main.f90:469-494)
+  bool valley = false, valley_ready = false;
+  ValleyGeom vg{};
+  void* valley_ws = nullptr;
+  std::vector<double> h_vk;  // [Nk][2]
 };
 
 static double total_energy(const forge_params& P, const double* e) {
@@ -383,7 +389,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
                   c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
-                  c->c3_Psi, c->c3_wph, c->c2_perm};
+                  c->c3_Psi, c->c3_wph, c->c2_perm, c->valley_ws};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   tridiag_release(c->trd);
@@ -1043,6 +1049,87 @@ extern "C" int forge_getfock_c3(forge_ctx* c, forge_z* zFock, const forge_z* zSo
   return 0;
 }
 
+// ---------------------------------------------------------------- valley projection
+
+// ValleyGeom from RotateLayers (RotMatrix of Main.f90:133 from the cell order:
+// aMoire = ndim / (2 nlayers)) and the workspace, once per context.
+static int valley_configure(forge_ctx* c, const int64_t* rot, const char* who) {
+  const i64 n = c->n, nl = c->P.nlayers, Nk = c->Nk;
+  if (!rot) { set_error(std::string(who) + ": null RotateLayers"); return -2; }
+  if (nl < 1 || nl > VALLEY_MAXL || n % (2 * nl) != 0) {
+    set_error(std::string(who) + ": layers must split ndim into two equal sublattice halves");
+    return -2;
+  }
+  ValleyGeom g{};
+  g.n = n;
+  g.per = n / nl;
+  g.nl = (int)nl;
+  g.nnc = (int)c->nnc;
+  const double aM = (double)(n / (2 * nl));
+  const double cs = 1.0 - 1.0 / (2.0 * aM);
+  const double co = std::cos(0.5 * std::acos(cs)), sn = std::sin(0.5 * std::acos(cs));
+  const double Rp[4] = {co, sn, -sn, co}, Rm[4] = {co, -sn, sn, co};
+  for (int q = 0; q < 4; ++q) { g.Rp[q] = Rp[q]; g.Rm[q] = Rm[q]; }
+  for (i64 l = 0; l < nl; ++l) g.rot[l] = (signed char)((rot[l] == 1 || rot[l] == -1) ? rot[l] : 0);
+  g.cells = c->g.cells;
+  g.X = c->X;
+  g.Y = c->X + n;
+  c->vg = g;
+  if (!c->valley_ws) CK(dalloc(&c->valley_ws, valley_workspace_bytes(n, c->nb, (int)Nk)));
+  c->h_vk.assign(2 * Nk, 0.0);
+  for (i64 k = 0; k < Nk; ++k) {
+    const i64 i1 = c->nMomComp[k], i2 = c->nMomComp[k + Nk];
+    c->h_vk[2 * k] = c->MomVal[i1 + c->numk * i2];
+    c->h_vk[2 * k + 1] = c->MomVal[i1 + c->numk * i2 + Nk];
+  }
+  c->valley_ready = true;
+  return 0;
+}
+
+// c->Cwin (the window eigenvectors of all Nk momenta) <- ValleyTransform; the
+// valley operator is built in the scratch matrix after rho_new's cells.
+static int valley_transform_cwin(forge_ctx* c) {
+  const size_t nn = (size_t)c->n * c->n;
+  return valley_transform(c->vg, c->h_vk.data(), (int)c->Nk, c->nb, c->Cwin,
+                          c->rho_new + c->nnc * nn, c->valley_ws, c->s);
+}
+
+extern "C" int forge_valley_phase(forge_ctx* c, const double* vk, const int64_t* RotateLayers,
+                                  forge_z* zV) {
+  if (!c || !vk || !zV) { set_error("forge_valley_phase: null argument"); return -2; }
+  CK(valley_configure(c, RotateLayers, "forge_valley_phase"));
+  const size_t nn = (size_t)c->n * c->n;
+  z2* V = c->rho_new + c->nnc * nn;
+  CK(valley_phase(c->vg, vk[0], vk[1], V, c->s));
+  FORGE_HIP(hipMemcpyAsync(zV, V, sizeof(z2) * nn, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_valley_transform(forge_ctx* c, forge_z* zEigenvectors, const int64_t* RotateLayers) {
+  if (!c || !zEigenvectors) { set_error("forge_valley_transform: null argument"); return -2; }
+  CK(valley_configure(c, RotateLayers, "forge_valley_transform"));
+  const size_t bytes = sizeof(z2) * c->n * c->nb * c->Nk;
+  FORGE_HIP(hipMemcpyAsync(c->Cwin, zEigenvectors, bytes, hipMemcpyHostToDevice, c->s));
+  CK(valley_transform_cwin(c));
+  FORGE_HIP(hipMemcpyAsync(zEigenvectors, c->Cwin, bytes, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  c->scf_ready = false;
+  return 0;
+}
+
+extern "C" int forge_scf_set_valley(forge_ctx* c, const int64_t* RotateLayers) {
+  if (!c) { set_error("forge_scf_set_valley: null context"); return -2; }
+  if (!RotateLayers) {
+    c->valley = false;
+    return 0;
+  }
+  CK(valley_configure(c, RotateLayers, "forge_scf_set_valley"));
+  c->valley = true;
+  return 0;
+}
+
 // ---------------------------------------------------------------- T / C2 / C2T
 
 extern "C" int64_t forge_c2_pairs(const double* Coords, int64_t ndim, int64_t nlayers,
@@ -1240,9 +1327,41 @@ static int c3_window(forge_ctx* c, i64 S, const std::vector<i64>& cols,
   return 0;
 }
 
-// window density into c->rho_new (which holds the bulk): rho_new = (bulk + win) / Nk
-static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation& occ) {
+static int valley_transform_cwin(forge_ctx* c);
+
+// window density into c->rho_new (which holds the bulk): rho_new = (bulk + win) / Nk.
+// in_loop: an SCF iteration (the valley projection applies only there; the
+// initial states of Main.f90:238-288 and of the guesses use the plain states)
+static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation& occ,
+                       bool in_loop) {
   occupations(c, band, occ);
+  if (c->valley && in_loop) {
+    // Main.f90:469-494: GetFock of the states, ValleyTransform, GetFock of
+    // the transformed states (same energies, so the same sorted columns),
+    // rho_new = (bulk + (win + win_valley) / 2) / Nk
+    if (c->c3) { set_error("window_step: the valley projection with the C3-reduced k-set is not built"); return -2; }
+    const i64 n = c->n, S = (i64)occ.idx.size();
+    std::vector<double> half(occ.wph);
+    for (double& x : half) x *= 0.5;
+    if (S > 0) {
+      FORGE_HIP(hipMemcpyAsync(c->gidx, occ.idx.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->s));
+      FORGE_HIP(hipMemcpyAsync(c->wph, half.data(), sizeof(double) * half.size(), hipMemcpyHostToDevice, c->s));
+      hipLaunchKernelGGL(k_gather_cols, dim3(cdiv(n * S, 256)), dim3(256), 0, c->s, n, S, c->gidx,
+                         c->Cwin, c->Psi);
+      FORGE_LAUNCH();
+    }
+    CK(window_density(n, (int)c->nnc, S, c->Psi, c->wph, c->rho_new, 1.0, c->s));
+    CK(valley_transform_cwin(c));
+    if (S > 0) {
+      hipLaunchKernelGGL(k_gather_cols, dim3(cdiv(n * S, 256)), dim3(256), 0, c->s, n, S, c->gidx,
+                         c->Cwin, c->Psi);
+      FORGE_LAUNCH();
+    }
+    CK(window_density(n, (int)c->nnc, S, c->Psi, c->wph, c->rho_new, 1.0 / (double)c->Nk, c->s));
+    // half lives on the host until the copies above have run
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    return 0;
+  }
   if (c->c3) {
     const i64 S = (i64)occ.idx.size();
     std::vector<i64> cols(S), kof(S);
@@ -1283,7 +1402,7 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
                   : solve_device(c, nullptr, c->pot, band, false);
     if (fails < 0) return fails;
     Occupation occ;
-    CK(window_step(c, band, occ));
+    CK(window_step(c, band, occ, false));
     FORGE_HIP(hipMemcpyAsync(c->rho, c->rho_new, sizeof(z2) * nnc * nn, hipMemcpyDeviceToDevice, c->s));
   }
   CK(init_state_of_rho(c));
@@ -1328,7 +1447,7 @@ static int guess_diagonalisation(forge_ctx* c) {
                           : solve_device(c, c->rho, c->pot, band, false);
   if (fails < 0) return fails;
   Occupation occ;
-  CK(window_step(c, band, occ));
+  CK(window_step(c, band, occ, false));
   FORGE_HIP(hipMemcpyAsync(c->rho, c->rho_new, sizeof(z2) * tot, hipMemcpyDeviceToDevice, c->s));
   CK(init_state_of_rho(c));
   return fails;
@@ -1424,7 +1543,7 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   if (fails < 0) return fails;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[10], c->s));
   Occupation occ;
-  CK(window_step(c, band, occ));
+  CK(window_step(c, band, occ, true));
   // symmetry enforcement on the new density (Main.f90:514-534)
   if (c->sym) CK(symmetrize(n, (int)nnc, c->rho_new, c->rho_new + nnc * nn, c->sym, c->c2_perm, c->s));
   if (timed) {

@@ -118,6 +118,21 @@ int bulk_density_c3(const C3Geom& g, int K, const z2* C, size_t ldC, size_t sC, 
                     const C3Slot* slot, z2* rho, bool accumulate, hipStream_t s);
 i64 c3_pairs(const double* C, i64 n, i64 nl, i64* perm);
 
+// valley.hip: the valley projection (nenforceValley).
+constexpr int VALLEY_MAXL = 8;  // layers
+struct ValleyGeom {
+  i64 n, per;                    // sites, sites per layer
+  int nl, nnc;                   // layers, stored cells
+  double Rp[4], Rm[4];           // RotMatrix (row-major) and its transpose
+  signed char rot[VALLEY_MAXL];  // RotateLayers
+  CellShifts cells;
+  const double *X, *Y;           // device site coordinates
+};
+int valley_phase(const ValleyGeom& g, double vkx, double vky, z2* V, hipStream_t s);
+size_t valley_workspace_bytes(i64 n, i64 nb, int K);
+int valley_transform(const ValleyGeom& g, const double* vk, int K, i64 nb, z2* C, z2* V,
+                     void* work, hipStream_t s);
+
 // sym.hip: nenforceT / C2 / C2T on the nnc cells of rho (sym bits 1, 2, 4;
 // scratch: one n x n matrix; perm: device C2 partners, 0-based).
 int symmetrize(i64 n, int nnc, z2* rho, z2* scratch, int sym, const int* perm, hipStream_t s);

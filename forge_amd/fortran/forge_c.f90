@@ -19,6 +19,7 @@ module forge_c
   public :: forge_c2_pairs, forge_symmetrize, forge_scf_set_symmetry, forge_hartree_hubbard
   public :: forge_scf_init_guess, forge_scf_init_guess_update, forge_scf_init_update
   public :: forge_mlp_param_count, forge_mlp_loss_grad, forge_mlp_train, forge_mlp_infer
+  public :: forge_valley_phase, forge_valley_transform, forge_scf_set_valley
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -260,6 +261,29 @@ module forge_c
       real(c_float), intent(out) :: Y(*)
       integer(c_int) :: r
     end function forge_mlp_infer
+    function forge_valley_phase(ctx, vk, RotateLayers, zV) bind(C, name="forge_valley_phase") &
+        result(r)
+      import :: c_int, c_ptr, c_double, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: vk(*)
+      integer(c_int64_t), intent(in) :: RotateLayers(*)
+      complex(c_double_complex), intent(out) :: zV(*)
+      integer(c_int) :: r
+    end function forge_valley_phase
+    function forge_valley_transform(ctx, zEigenvectors, RotateLayers) &
+        bind(C, name="forge_valley_transform") result(r)
+      import :: c_int, c_ptr, c_double_complex, c_int64_t
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: zEigenvectors(*)
+      integer(c_int64_t), intent(in) :: RotateLayers(*)
+      integer(c_int) :: r
+    end function forge_valley_transform
+    function forge_scf_set_valley(ctx, RotateLayers) bind(C, name="forge_scf_set_valley") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: RotateLayers
+      integer(c_int) :: r
+    end function forge_scf_set_valley
     function forge_scf_set_options(ctx, opt) bind(C, name="forge_scf_set_options") result(r)
       import :: c_int, c_ptr, forge_scf_options_t
       type(c_ptr), value :: ctx

@@ -222,6 +222,29 @@ class Engine:
         L.check(self.lib.forge_scf_set_symmetry(self.ctx, int(flags), _p(pairs)),
                 "forge_scf_set_symmetry")
 
+    # -- valley projection (nenforceValley) --------------------------------------
+    def _rot(self):
+        return np.ascontiguousarray(self.geom.RotateLayers, dtype=np.int64)
+
+    def valley_phase(self, vk):
+        """ValleyPhase (TightBinding.f90:60-165): the valley operator at vk."""
+        out = np.zeros((self.n, self.n), np.complex128, order="F")
+        v, rot = _f(vk, np.float64), self._rot()
+        L.check(self.lib.forge_valley_phase(self.ctx, _p(v), _p(rot), _p(out)), "forge_valley_phase")
+        return out
+
+    def valley_transform(self, zEigenvectors):
+        """ValleyTransform (TightBinding.f90:170-211) of (n, numb, Nk) eigenvectors."""
+        out = np.array(zEigenvectors, dtype=np.complex128, order="F", copy=True)
+        rot = self._rot()
+        L.check(self.lib.forge_valley_transform(self.ctx, _p(out), _p(rot)), "forge_valley_transform")
+        return out
+
+    def set_valley(self, on=True):
+        """nenforceValley for the device-resident loop (Main.f90:469-494)."""
+        rot = self._rot() if on else None
+        L.check(self.lib.forge_scf_set_valley(self.ctx, _p(rot)), "forge_scf_set_valley")
+
     def energies(self, zFock, dens):
         """(Ekin, EF, EH, EU) of GetKineticEnergy/GetFockEnergy/GetHartreeHubbardEnergy."""
         out = np.zeros(4)

@@ -57,6 +57,9 @@ _SIGS = {
     "forge_c2_pairs": (i64, [vp, i64, i64, vp]),
     "forge_symmetrize": (C.c_int, [vp, vp, i32, vp]),
     "forge_scf_set_symmetry": (C.c_int, [vp, i32, vp]),
+    "forge_valley_phase": (C.c_int, [vp, vp, vp, vp]),
+    "forge_valley_transform": (C.c_int, [vp, vp, vp]),
+    "forge_scf_set_valley": (C.c_int, [vp, vp]),
     "forge_scf_init": (C.c_int, [vp, vp]),
     "forge_scf_init_guess": (C.c_int, [vp, vp]),
     "forge_scf_init_guess_update": (C.c_int, [vp, vp, i64, vp, vp, vp, vp]),

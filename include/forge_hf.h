@@ -163,6 +163,20 @@ int64_t forge_c2_pairs(const double* Coords, int64_t ndim, int64_t nlayers, int6
  * same without conjg, on every cell. */
 int forge_symmetrize(forge_ctx* ctx, forge_z* zFock, int32_t flags, const int64_t* nC2pairs);
 
+/* ---- Valley projection (nenforceValley, Setup.f90:31) ----
+ * RotateLayers as for the C3 entry points; RotMatrix (Main.f90:133) follows
+ * from the cell (aMoire = ndim / (2 nlayers)). */
+
+/* ValleyPhase(zV, Coords, nUnitCell_1, nUnitCell_2, ndim, numNeighborCells,
+ * vk, tn, RotMatrix), TightBinding.f90:60-165 (fphase = 0): the valley
+ * operator at momentum vk (ndim x ndim). */
+int forge_valley_phase(forge_ctx* ctx, const double* vk, const int64_t* RotateLayers, forge_z* zV);
+
+/* ValleyTransform(zEigenvectors, ...), TightBinding.f90:170-211: per k,
+ * zEigenvectors(:, :, k) (ndim x numb, all Nk in place) <- C sign(C^H V C)
+ * (unitarize, LapackRoutines.f90:865-893). */
+int forge_valley_transform(forge_ctx* ctx, forge_z* zEigenvectors, const int64_t* RotateLayers);
+
 /* GetHartreeHubbardEnergy(fsumH, fsumU, LongRange, dens1, dens2, ndim),
  * HartreeFock.f90:1066-1092, with the context's device copy of LongRange:
  * out2 = {1/2 (d1+d2).L.(d1+d2), d1.d2}. */
@@ -181,6 +195,11 @@ int forge_scf_set_c3(forge_ctx* ctx, const int64_t* RotateLayers, const int64_t*
 /* The symmetrisations of forge_symmetrize applied by forge_scf_run to every
  * iteration's new density, where Main.f90:514-534 applies them (flags 0: off). */
 int forge_scf_set_symmetry(forge_ctx* ctx, int32_t flags, const int64_t* nC2pairs);
+
+/* nenforceValley = 1 for the device-resident loop (Main.f90:469-494): each
+ * iteration's window density is the mean of those of the states and of their
+ * ValleyTransform.  Null: off.  Not with the C3-reduced k-set. */
+int forge_scf_set_valley(forge_ctx* ctx, const int64_t* RotateLayers);
 
 /* Initial state: rho_start (ndim x ndim x nnc) as read from a checkpoint
  * (Main.f90:187-235), or NULL for the first diagonalisation

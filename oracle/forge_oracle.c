@@ -867,6 +867,12 @@ static void valley_phase(const orc_params* p, const i64* rot, const double* C, c
   }
 }
 
+/* valley_phase for one momentum (exported for the tests). */
+void orc_valley_phase(const orc_params* p, const i64* rot, const double* C, const double* tn,
+                      const i64* u1, const i64* u2, const double* vk, zc* V) {
+  valley_phase(p, rot, C, tn, u1, u2, vk, V);
+}
+
 /* ValleyTransform, TightBinding.f90:170-211: per k, P = C^H V C over the
  * window columns, zheev('V', 'L') of P (zdiagFull, LapackRoutines.f90:498-533),
  * S = U sign(Lambda) U^H (unitarize, LapackRoutines.f90:865-893; sign(1, x)

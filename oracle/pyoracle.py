@@ -211,6 +211,20 @@ class System:
                              C.c_int(occ_mode), A(weights, float), i64(cnt))
         return rho
 
+    def valley_phase(self, vk):
+        V = fz((self.n, self.n), np.complex128)
+        A = Args()
+        lib().orc_valley_phase(self.pp, ptr(self.rot), ptr(self.Coords), ptr(self.tn), ptr(self.u1),
+                               ptr(self.u2), A(vk, float), ptr(V))
+        return V
+
+    def valley_transform(self, zEig):
+        out = np.array(zEig, dtype=np.complex128, order="F", copy=True)
+        lib().orc_valley_transform(self.pp, ptr(self.rot), ptr(self.Coords), ptr(self.tn),
+                                   ptr(self.MomentaValues), ptr(self.nMomentaComponents),
+                                   ptr(self.u1), ptr(self.u2), ptr(out))
+        return out
+
     def sort(self, Bands, zEig):
         nb, Nk = Bands.shape
         N = nb * Nk

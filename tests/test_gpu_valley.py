@@ -1,0 +1,61 @@
+"""GPU parity of the valley projection (nenforceValley): forge_valley_phase
+and forge_valley_transform against the oracle on the same inputs, and the
+device-resident loop with forge_scf_set_valley against the trajectories of
+the reference built with nenforceValley = 1.
+
+Tolerances:
+* the valley operator is built elementwise, so 1e-14;
+* the transform is the matrix sign of C^H V C, whose smallest eigenvalue
+  magnitude is 2.6e-6 for the test state, so rounding of 1e-15 moves it
+  by about 1e-9: 1e-7;
+* the SCF amplifies rounding (a 1e-13 change in the start moves the energy
+  by 1e-9 within five iterations, see test_oracle), so 1e-8 for the first
+  three iterations and 1e-5 for the rest.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from golden_io import load_case, load_traj, traj_meta
+from forge_amd.hf import Engine, Setup
+from forge_amd.seed import seeded_rho
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_valley_phase_and_transform_match_oracle():
+    from oracle.pyoracle import System
+    say("valley phase + transform, t1k3 state")
+    f = load_case("t1k3")
+    s = System(1, 3, 1.0)
+    e = Engine(Setup(ntheta=1, numk=3, nfilling=1.0, xi_nm=40.0))
+    for k in (0, 1, 4):
+        np.testing.assert_allclose(e.valley_phase(s.vk(k)), s.valley_phase(s.vk(k)), rtol=0, atol=1e-14)
+    _, _, vecs, info = s.solve(f["rho0"], f["Potential0"])
+    assert info == 0
+    np.testing.assert_allclose(e.valley_transform(vecs), s.valley_transform(vecs), rtol=0, atol=1e-7)
+    e.close()
+
+
+@pytest.mark.parametrize("case", ["t1k3_V", "t1k3s_V"])
+def test_scf_valley_trajectory_matches_reference(case):
+    say("scf valley", case)
+    m = traj_meta(case)
+    e = Engine(Setup(ntheta=1, numk=3, nfilling=1.0, xi_nm=40.0))
+    e.set_options()
+    e.set_valley(True)
+    start = seeded_rho(e.n, e.nnc, 1.0) if m["start"] == "split" else None
+    e.scf_init(start)
+    traj = e.scf_run(int(m["itmax"]))
+    E, Cv = load_traj(case)
+    assert len(traj) == len(E)
+    np.testing.assert_allclose(traj[:3, 1:6], E[:3, 1:6], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(traj[:, 1:6], E[:, 1:6], rtol=0, atol=1e-5)
+    np.testing.assert_allclose(traj[:, 7], Cv[:, 2], rtol=0, atol=1e-3)
+    e.close()
