@@ -1125,6 +1125,10 @@ extern "C" int forge_scf_set_valley(forge_ctx* c, const int64_t* RotateLayers) {
     c->valley = false;
     return 0;
   }
+  if (c->c3) {
+    set_error("forge_scf_set_valley: the valley projection with the C3-reduced k-set is not built");
+    return -2;
+  }
   CK(valley_configure(c, RotateLayers, "forge_scf_set_valley"));
   c->valley = true;
   return 0;
@@ -1189,6 +1193,10 @@ extern "C" int forge_scf_set_c3(forge_ctx* c, const int64_t* RotateLayers, const
   if (!RotateLayers && !nC3pairs) {
     c->c3 = false;
     return 0;
+  }
+  if (c->valley) {
+    set_error("forge_scf_set_c3: the C3-reduced k-set with the valley projection is not built");
+    return -2;
   }
   CK(c3_configure(c, RotateLayers, nC3pairs, "forge_scf_set_c3"));
   c->c3 = true;
