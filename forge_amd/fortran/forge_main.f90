@@ -23,7 +23,9 @@
 ! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it)
 ! numS (1, or 2 for independent up and down spins in mode=reference with the
 ! reference's weights and optimal damping) start2 (spin 2's checkpoint when
-! numS = 2; start= is then spin 1's) T C2 C2T (nenforceT / nenforceC2 /
+! numS = 2; start= is then spin 1's) nrelax (1: relaxed bilayer lattice,
+! Main.f90:156-160; relaxtable= overrides forge_amd/data/relaxation_fourier.txt)
+! T C2 C2T (nenforceT / nenforceC2 /
 ! nenforceC2T, Setup.f90:28-30: 1 = symmetrise each new density as
 ! Main.f90:514-534 does).
 program forge_main
@@ -36,6 +38,7 @@ program forge_main
 
   character(len=16) :: mode = 'reference'
   character(len=512) :: start = 'first', start2 = '', outfile = 'forge_traj.txt', refout = ''
+  character(len=512) :: relaxtable = ''
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
   integer(dp) :: nenforceC3 = 0
   integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:)
@@ -61,6 +64,7 @@ program forge_main
   integer :: fout
 
   call parse_args()
+  if (len_trim(relaxtable) == 0) call default_relaxtable()
   call setup_init()
   n = ndim
   nU = nUpper - nLower + 1
@@ -71,6 +75,8 @@ program forge_main
   call OrderNeighborCells(nUnitCell_1, nUnitCell_2, numI, nnc)
   allocate(Coords(n, 3), MomentaValues(numk, numk, 2), nMomentaComponents(Nk, 2))
   call build_cell(Coords, t1, t2, t3, g1, g12)
+  ! LatticeRelaxation for bilayers (Main.f90:156-160)
+  if (nrelax == 1 .and. nlayers == 2) call lattice_relaxation(Coords, g1, g12, trim(relaxtable))
   call SampleBZ(nMomentaComponents, MomentaValues, numk, g1, g12)
   tn = reshape([t1, t2, t3], [2, 3])
   ! C2_RelatedPoints (Main.f90:162)
@@ -271,7 +277,7 @@ contains
     write(par, '(A1,A3,A8,A1,F3.1,A5,I0,A2,I0,A8,I0,A6,I0,A4,F0.1,A2,F4.2,A7,I0,A3,F0.1,A6,F0.3,&
                &A5,I0,A5,I0,A5,I0,A3,I0,A4)') &
         '-', 'SSS', '-filling', dop, abs(nfilling), '-numS', numS, '-i', ntheta, '-nlayers', nlayers, &
-        '-relax', 0, '-eps', epsilon, '-U', U, '-screen', nscreen, '-xi', xi * 0.246_dp, &
+        '-relax', nrelax, '-eps', epsilon, '-U', U, '-screen', nscreen, '-xi', xi * 0.246_dp, &
         '-delta', Delta, '-numI', numI, '-numC', numC, '-numk', numk, '-dp', 8, '.dat'
   end function run_parameters
 
@@ -552,6 +558,19 @@ contains
     Potential(:, 1) = base + U * (Density(:, 2) - DensitySub)
   end subroutine update_potential_2spins
 
+  ! forge_amd/data/relaxation_fourier.txt next to the executable's directory
+  subroutine default_relaxtable()
+    character(len=512) :: exe
+    integer :: k
+    call get_command_argument(0, exe)
+    k = index(exe, '/', back=.true.)
+    if (k > 0) then
+      relaxtable = exe(1:k)//'../data/relaxation_fourier.txt'
+    else
+      relaxtable = '../data/relaxation_fourier.txt'
+    end if
+  end subroutine default_relaxtable
+
   subroutine parse_args()
     integer :: k, eq
     character(len=512) :: arg, key, val
@@ -587,6 +606,8 @@ contains
       case ('refout'); refout = val
       case ('numS'); read(val, *) numS
       case ('start2'); start2 = val
+      case ('nrelax'); read(val, *) nrelax
+      case ('relaxtable'); relaxtable = val
       case ('T'); read(val, *) i; if (i == 1) symflags = ior(symflags, 1_c_int32_t)
       case ('C2'); read(val, *) i; if (i == 1) symflags = ior(symflags, 2_c_int32_t)
       case ('C2T'); read(val, *) i; if (i == 1) symflags = ior(symflags, 4_c_int32_t)

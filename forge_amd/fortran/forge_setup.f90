@@ -23,7 +23,7 @@ module forge_setup
   integer(dp), parameter :: fphase = 0
 
   integer(dp) :: nlayers = 2, ntheta = 5, numk = 6, numI = 1, numC = 3, nscreen = 2
-  integer(dp) :: itmax = 2, nRead = 0
+  integer(dp) :: itmax = 2, nRead = 0, nrelax = 0
   integer(dp), allocatable :: RotateLayers(:)
   real(dp) :: nfilling = 0.0_dp, U = 4.0_dp, epsilon = 10.0_dp, Delta = 0.0_dp
   real(dp) :: xi = 10.0_dp / 0.246_dp

@@ -10,6 +10,9 @@ Vectorised with numpy; the arithmetic per site is the reference's.
 """
 from __future__ import annotations
 
+import math
+import os
+
 from dataclasses import dataclass
 
 import numpy as np
@@ -128,8 +131,78 @@ def rot_apply(R, v):
     return np.array([R[0, 0] * v[0] + R[0, 1] * v[1], R[1, 0] * v[0] + R[1, 1] * v[1]])
 
 
+_RELAX_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                            "relaxation_fourier.txt")
+
+
+def relaxation_components(ntheta: int):
+    """(uqx, uqy) as 5 x 5 arrays indexed (n-1, m-1): the Fourier components of
+    the in-plane relaxation profile the reference holds for this cell
+    (Geometry.f90:309-1056; zero for cells it has no table for)."""
+    uqx = np.zeros((5, 5))
+    uqy = np.zeros((5, 5))
+    for line in open(_RELAX_TABLE):
+        if line.startswith("#") or not line.strip():
+            continue
+        t, n, m, ux, uy = line.split()
+        if int(t) == ntheta:
+            uqx[int(n) - 1, int(m) - 1] = float(ux)
+            uqy[int(n) - 1, int(m) - 1] = float(uy)
+    return uqx, uqy
+
+
+def lattice_relaxation(C: np.ndarray, g1: np.ndarray, g12: np.ndarray, ntheta: int) -> np.ndarray:
+    """LatticeRelaxation (Geometry.f90:279-1135) of a bilayer cell: layer 1
+    moves by half the Fourier displacement sum_{nn, n, m} -2 u sin(k.r) along
+    the three reciprocal directions, layer 2 becomes layer 1 mirrored in x,
+    re-ordered to the unrelaxed site order of each sublattice half (nearest
+    site within 0.25; the half's corner takes (x, -y) of its own mirrored
+    site).  Scalar math in the reference's order of operations."""
+    n = C.shape[0]
+    uqx, uqy = relaxation_components(ntheta)
+    Gn = [np.asarray(g1, float), np.asarray(g12, float)]
+    Gn.append(Gn[1] - Gn[0])
+    Gn += [-Gn[0], -Gn[1], -Gn[2], Gn[0], Gn[1]]
+    R = np.array(C, dtype=float, order="F", copy=True)
+    for i in range(n // 2):
+        rx, ry = R[i, 0], R[i, 1]
+        d0 = d1 = 0.0
+        for nn in (1, 3, 5):
+            ca, sa = math.cos((nn - 1) * math.pi / 3.0), math.sin((nn - 1) * math.pi / 3.0)
+            ga, gb = Gn[nn - 1], Gn[nn + 1]
+            for N in range(2, 6):
+                for M in range(1, N):
+                    vx = (N - 1) * ga[0] + (M - 1) * gb[0]
+                    vy = (N - 1) * ga[1] + (M - 1) * gb[1]
+                    ux = ca * uqx[N - 1, M - 1] - sa * uqy[N - 1, M - 1]
+                    uy = sa * uqx[N - 1, M - 1] + ca * uqy[N - 1, M - 1]
+                    sk = math.sin(vx * rx + vy * ry)
+                    d0 = d0 - 2 * ux * sk
+                    d1 = d1 - 2 * uy * sk
+        R[i, 0] = rx + d0 / 2
+        R[i, 1] = ry + d1 / 2
+        R[i + n // 2, 0] = -R[i, 0]
+        R[i + n // 2, 1] = R[i, 1]
+    out = R.copy()
+    for lo, hi in ((n // 2, 3 * n // 4), (3 * n // 4, n)):
+        matched = 0
+        for i in range(lo, hi):
+            for j in range(lo, hi):
+                if math.sqrt((R[i, 0] - C[j, 0]) ** 2 + (R[i, 1] - C[j, 1]) ** 2) < 0.25:
+                    matched += 1
+                    out[j, :] = R[i, :]
+                    break
+        matched += 1
+        out[hi - 1, 0] = R[hi - 1, 0]
+        out[hi - 1, 1] = -R[hi - 1, 1]
+        out[hi - 1, 2] = R[hi - 1, 2]
+        if matched != n // 4:
+            raise RuntimeError(f"lattice relaxation: {matched} of {n // 4} sites re-ordered")
+    return np.asfortranarray(out)
+
+
 def build_geometry(ntheta: int, nlayers: int = 2, numk: int = 3, numI: int = 1,
-                   rotate=None, tz: float = 1.35772) -> Geometry:
+                   rotate=None, tz: float = 1.35772, nrelax: int = 0) -> Geometry:
     if rotate is None:
         rotate = [-1, 1] if nlayers == 2 else [(-1) ** (l + 1) for l in range(nlayers)]
     nnc = number_neighbor_cells(numI)
@@ -163,6 +236,9 @@ def build_geometry(ntheta: int, nlayers: int = 2, numk: int = 3, numI: int = 1,
     n = ndim_of(ntheta, nlayers)
     if C.shape[0] != n:
         raise RuntimeError(f"Wigner-Seitz cell holds {C.shape[0]} sites, expected {n}")
+    if nrelax == 1 and nlayers == 2:
+        # Main.f90:156-160: only bilayers are relaxed
+        C = lattice_relaxation(C, g1, g12, ntheta)
     t1r = rot_apply(R, t1)
     t2r = rot_apply(R, t2)
     tn = np.asfortranarray(np.stack([t1r, t2r, t2r - t1r], 1))

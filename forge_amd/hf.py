@@ -31,6 +31,7 @@ class Setup:
     numI: int = 1
     numC: int = 3
     nscreen: int = 2
+    nrelax: int = 0            # 1: relaxed bilayer lattice (Setup.f90:12, Main.f90:156-160)
     energy_tol: float = 1e-4   # EnergyTolerance, Setup.f90:51
     step_alt: float = 1.0      # StepAlternative, Setup.f90:52
 
@@ -67,7 +68,8 @@ def _f(a, dtype):
 class Engine:
     def __init__(self, setup: Setup, geom: Geometry | None = None):
         self.setup = setup
-        self.geom = geom or build_geometry(setup.ntheta, setup.nlayers, setup.numk, setup.numI)
+        self.geom = geom or build_geometry(setup.ntheta, setup.nlayers, setup.numk, setup.numI,
+                                           nrelax=setup.nrelax)
         g = self.geom
         self.p = setup.params(g.nnc)
         self.n = g.ndim

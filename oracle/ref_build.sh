@@ -1,7 +1,8 @@
 #!/usr/bin/env bash
 # Build the reference FORGE program (Fortran 90, /root/reference) for one
-# compile-time configuration, plus the project-written harness
-# oracle/ref_harness.f90 that calls the reference modules and dumps arrays.
+# compile-time configuration, plus the project-written harnesses
+# oracle/ref_harness.f90 (calls the reference modules and dumps arrays) and
+# oracle/ref_relax.f90 (the cell before and after LatticeRelaxation).
 #
 # The reference is configured only through `parameter`s in Setup.f90
 # (Setup.f90:6-74); this recipe edits those values with sed in a scratch copy
@@ -83,6 +84,7 @@ sed -i \
   -e '636s/FockEnergyIn(2)/FockEnergyIn(numS)/' \
   "$SCR/Main.f90"
 cp "$HERE/ref_harness.f90" "$SCR/ref_harness.f90"
+cp "$HERE/ref_relax.f90" "$SCR/ref_relax.f90"
 cd "$SCR"
 FFLAGS="-O2 -fopenmp -w"
 for f in LapackRoutines Setup TightBinding HartreeFock Geometry; do
@@ -93,5 +95,6 @@ $FC $FFLAGS -o "$OUT/forge.x" LapackRoutines.o Setup.o TightBinding.o HartreeFoc
 # the harness dumps the spin-singlet arrays only
 if [ "$NUMS" = "1" ]; then
   $FC $FFLAGS -o "$OUT/harness.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o ref_harness.f90 $LIBS
+  $FC $FFLAGS -o "$OUT/relax.x" LapackRoutines.o Setup.o TightBinding.o HartreeFock.o Geometry.o ref_relax.f90 $LIBS
 fi
 echo "built $OUT (ntheta=$NTHETA numk=$NUMK nfilling=$NFILL itmax=$ITMAX nRead=$NREAD xi=${XINM}nm nlayers=$NLAYERS)"

@@ -279,6 +279,30 @@ def combo_fixtures():
         print("wrote", name, flush=True)
 
 
+RELAX_CELLS = [2, 3, 5]
+
+
+def relax_fixtures():
+    """The bilayer cell before and after the reference's LatticeRelaxation
+    (Geometry.f90:279-1135, applied as Main.f90:156-160), from the
+    project-written oracle/ref_relax.f90: tests/golden/relax_t<ntheta>.npz."""
+    for t in RELAX_CELLS:
+        name = f"relax_t{t}"
+        subprocess.run([REF_BUILD, name, str(t), "3", "1.0", "1", "0", "40.0", "2"], check=True,
+                       stdout=subprocess.DEVNULL)
+        d = os.path.join(REF_OUT, name, "relax")
+        shutil.rmtree(d, ignore_errors=True)
+        os.makedirs(d)
+        r = subprocess.run(["../relax.x"], cwd=d, capture_output=True, text=True, check=True)
+        if "ERROR" in r.stdout:
+            raise RuntimeError(f"{name}: {r.stdout}")
+        ndim = 2 * 2 * (3 * t * t + 3 * t + 1)
+        arrays = {k: np.fromfile(os.path.join(d, k + ".bin"), "<f8").reshape((ndim, 3), order="F")
+                  for k in ("Coords", "CoordsRelaxed")}
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), ntheta=np.array(t), **arrays)
+        print("wrote", name, flush=True)
+
+
 def checkpoint_fixture():
     """A density checkpoint written by the reference itself (nWriteFock=1,
     Main.f90:647-677) after one iteration of case t1k3."""
@@ -314,7 +338,7 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
-    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym", "combo")]
+    cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym", "combo", "relax")]
     if not args or cases:
         main(cases)
     if not args or "ckpt" in args:
@@ -329,3 +353,5 @@ if __name__ == "__main__":
         sym_fixtures()
     if not args or "combo" in args:
         combo_fixtures()
+    if not args or "relax" in args:
+        relax_fixtures()

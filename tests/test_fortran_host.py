@@ -357,3 +357,20 @@ def test_fortran_sort_2spins_matches_oracle(tmp_path):
         got = np.array([[float(v) for v in l.split()] for l in out.stdout.splitlines() if l.strip()])
         ref = s.sort_2spins(S1)
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ntheta", [2, 3, 5])
+def test_fortran_relaxed_geometry_is_bitwise_reference(ntheta, tmp_path):
+    """nrelax=1: the Fortran host's lattice relaxation, read from the same
+    table, against the reference's relaxed cell, bit for bit (CPU: geometry
+    mode); the run parameters carry -relax1."""
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             f"relax_t{ntheta}.npz"))
+    out = tmp_path / "geom.txt"
+    r = subprocess.run([FORTRAN_EXE, "mode=geometry", f"ntheta={ntheta}", "nlayers=2", "numk=3",
+                        "nrelax=1", f"out={out}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = _read_geometry(out)
+    np.testing.assert_array_equal(g["Coords"].reshape(-1, 3), f["CoordsRelaxed"])
+    par = [l.split()[2] for l in open(out) if l.startswith("# parameters ")][0]
+    assert "-relax1-" in par
