@@ -316,7 +316,7 @@ class Engine:
         from .checkpoint import run_parameters
         from .outputs import write_outputs
         st = self.setup
-        params = run_parameters(statename, st.nfilling, 1, st.ntheta, st.nlayers, 0, st.epsilon,
+        params = run_parameters(statename, st.nfilling, 1, st.ntheta, st.nlayers, st.nrelax, st.epsilon,
                                 st.U, st.nscreen, st.xi_nm, st.Delta, st.numI, st.numC, st.numk)
         return write_outputs(outdir, params, traj, append=append)
 
