@@ -243,6 +243,17 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
     set_error("forge_ctx_create: fphase != 0 is not supported (Setup.f90:25 uses 0)");
     return -2;
   }
+  {
+    // the Fermi block reads sorted window states nF and nF + 1 (Main.f90:400-401)
+    const i64 Nk = p->numk * p->numk, nbw = p->nUpper - p->nLower + 1;
+    const double x = (double)(Nk * ((p->ndim / 2 - p->nLower) + 1)) + (double)Nk / 2.0 * p->nfilling;
+    const i64 nF = (i64)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5));
+    if (nF < 1 || nF > nbw * Nk - 1) {
+      set_error("forge_ctx_create: the filling puts the Fermi level outside the band window "
+                "(need 1 <= nF <= (nUpper - nLower + 1) numk^2 - 1)");
+      return -2;
+    }
+  }
   int ndev = forge_device_count();
   if (ndev < 1) {
     set_error("forge_ctx_create: no HIP device");
@@ -430,13 +441,17 @@ static int hamiltonian_one(forge_ctx* c, forge_z* zH, const double* Potential, c
   }
   double* dph = (double*)c->part;  // scratch
   FORGE_HIP(hipMemcpyAsync(dph, ph.data(), sizeof(double) * nnc * 2, hipMemcpyHostToDevice, c->s));
+  // the potential goes to db and rho to rho_new: both are rebuilt by every
+  // SCF iteration before they are read, so a Hamiltonian call between two
+  // forge_scf_run calls leaves the SCF state (rho, da, pot, energies) intact
+  double* vpot = c->db;
   if (Potential)
-    FORGE_HIP(hipMemcpyAsync(c->da, Potential, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+    FORGE_HIP(hipMemcpyAsync(vpot, Potential, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
   else
-    FORGE_HIP(hipMemsetAsync(c->da, 0, sizeof(double) * n, c->s));
+    FORGE_HIP(hipMemsetAsync(vpot, 0, sizeof(double) * n, c->s));
   if (zFock)
     FORGE_HIP(hipMemcpyAsync(c->rho_new, zFock, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
-  CK(assemble(n, (int)nnc, 1, zFock ? c->rho_new : nullptr, c->V, c->T, c->da, c->bias,
+  CK(assemble(n, (int)nnc, 1, zFock ? c->rho_new : nullptr, c->V, c->T, vpot, c->bias,
               c->P.alpha, dph, c->H, c->ldH, (size_t)c->ldH * n, c->s));
   hipLaunchKernelGGL(k_zero_upper, dim3(cdiv(n * n, 256)), dim3(256), 0, c->s, n, c->ldH, c->H);
   FORGE_LAUNCH();

@@ -3,8 +3,9 @@
 ! Each subroutine keeps the name and the dummy-argument list of the reference
 ! routine it replaces, so a host written against the reference calls it
 ! unchanged; the body calls the MI355X engine through forge_c.  Arguments
-! the engine already holds (the geometry, uploaded once by forge_ctx_create)
-! are accepted and checked for size only.  numS = 1 or 2 (OptimalStep).
+! the engine already holds (the geometry and the model constants, uploaded
+! once by forge_ctx_create) are compared with the context's copies: sizes
+! always, alpha_, Delta_, Coords and tn when forge_attach was given them.  numS = 1 or 2 (OptimalStep).
 !
 !   LongRangeInteraction     HartreeFock.f90:14-53
 !   HamiltonianHartreeFock   HartreeFock.f90:57-127
@@ -35,18 +36,64 @@ module forge_hartreefock
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(dp), save :: ctx_ndim = -1, ctx_nnc = -1
+  ! the model the context was built with (forge_params and forge_ctx_create's
+  ! arrays): the shims compare the caller's alpha_, Delta_, Coords and tn
+  ! with these and stop on a difference, since the engine uses its own copies
+  logical, save :: ctx_model = .false.
+  real(dp), save :: ctx_alpha = 0.0_dp, ctx_delta = 0.0_dp, ctx_tn(2, 3) = 0.0_dp
+  real(dp), allocatable, save :: ctx_coords(:, :)
 
 
 contains
 
   ! The engine context the shims call (created by the host with forge_ctx_create).
-  subroutine forge_attach(c, ndim_, nnc_)
+  ! alpha_, Delta_, Coords and tn: the values the context was created with.
+  subroutine forge_attach(c, ndim_, nnc_, alpha_, Delta_, Coords, tn)
     type(c_ptr), intent(in) :: c
     integer(dp), intent(in) :: ndim_, nnc_
+    real(dp), intent(in), optional :: alpha_, Delta_, Coords(:, :), tn(2, 3)
     ctx = c
     ctx_ndim = ndim_
     ctx_nnc = nnc_
+    ctx_model = present(alpha_) .and. present(Delta_) .and. present(Coords) .and. present(tn)
+    if (ctx_model) then
+      ctx_alpha = alpha_
+      ctx_delta = Delta_
+      ctx_tn = tn
+      if (allocated(ctx_coords)) deallocate(ctx_coords)
+      allocate(ctx_coords(size(Coords, 1), size(Coords, 2)))
+      ctx_coords = Coords
+    end if
   end subroutine forge_attach
+
+  ! Stops when an argument differs from the value the context holds.
+  subroutine same_model(where, alpha_, Delta_, Coords, tn)
+    character(*), intent(in) :: where
+    real(dp), intent(in), optional :: alpha_, Delta_, Coords(:, :), tn(2, 3)
+    character(16) :: what
+    if (.not. ctx_model) return
+    what = ''
+    if (present(alpha_)) then
+      if (alpha_ /= ctx_alpha) what = 'alpha'
+    end if
+    if (present(Delta_)) then
+      if (Delta_ /= ctx_delta) what = 'Delta'
+    end if
+    if (present(tn)) then
+      if (any(tn /= ctx_tn)) what = 'tn'
+    end if
+    if (present(Coords)) then
+      if (any(shape(Coords) /= shape(ctx_coords))) then
+        what = 'Coords'
+      else if (any(Coords /= ctx_coords)) then
+        what = 'Coords'
+      end if
+    end if
+    if (what /= '') then
+      write(*, *) 'forge: ', where, ': ', trim(what), ' differs from the value the engine context holds'
+      error stop 2
+    end if
+  end subroutine same_model
 
   ! Stops with the engine's message on a negative return code.
   subroutine forge_check(rc, where)
@@ -76,6 +123,13 @@ contains
     real(dp), intent(in) :: t1(2), t2(2), Coords(ndim_, 3)
     real(dp), intent(inout) :: LongRange(ndim_, ndim_)
     call need(ndim_, -1_dp, 'LongRangeInteraction')
+    call same_model('LongRangeInteraction', Coords=Coords)
+    if (ctx_model) then
+      if (any(t1 /= ctx_tn(:, 1)) .or. any(t2 /= ctx_tn(:, 2))) then
+        write(*, *) 'forge: LongRangeInteraction: t1, t2 differ from the engine context'
+        error stop 2
+      end if
+    end if
     call forge_check(forge_longrange(ctx, LongRange), 'LongRangeInteraction')
   end subroutine LongRangeInteraction
 
@@ -88,6 +142,7 @@ contains
     complex(dp), intent(in) :: zFock(ndim_, ndim_, numNeighborCells)
     complex(dp), intent(out) :: zH(ndim_, ndim_)
     call need(ndim_, numNeighborCells, 'HamiltonianHartreeFock')
+    call same_model('HamiltonianHartreeFock', alpha_=alpha_, Delta_=Delta_, Coords=Coords, tn=tn)
     call forge_check(forge_hamiltonian_hf(ctx, zH, Potential, zFock, vk), 'HamiltonianHartreeFock')
   end subroutine HamiltonianHartreeFock
 
@@ -98,6 +153,7 @@ contains
     real(dp), intent(in) :: tn(2, 3), vk(2), Coords(ndim_, 3), Delta_
     complex(dp), intent(out) :: zH(ndim_, ndim_)
     call need(ndim_, numNeighborCells, 'TightBindingHamiltonian')
+    call same_model('TightBindingHamiltonian', Delta_=Delta_, Coords=Coords, tn=tn)
     call forge_check(forge_tb_hamiltonian(ctx, zH, vk), 'TightBindingHamiltonian')
   end subroutine TightBindingHamiltonian
 
@@ -130,6 +186,7 @@ contains
     complex(dp), intent(out) :: zFockBulk(ndim_, ndim_, numNeighborCells)
     integer(c_int) :: rc
     call need(ndim_, numNeighborCells, 'Solve')
+    call same_model('Solve', alpha_=alpha_, Coords=Coords, tn=tn)
     rc = forge_solve(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors)
     call forge_check(rc, 'Solve')
     if (rc > 0) write(*, *) 'zheev failed', rc
@@ -188,6 +245,7 @@ contains
     complex(dp), intent(out) :: zFockBulk(ndim_, ndim_, numNeighborCells)
     integer(c_int) :: rc
     call need(ndim_, numNeighborCells, 'Solve_C3')
+    call same_model('Solve_C3', alpha_=alpha_, Coords=Coords, tn=tn)
     rc = forge_solve_c3(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors, RotateLayers_, &
                         nC3pairs)
     call forge_check(rc, 'Solve_C3')

@@ -105,7 +105,7 @@ contains
     call forge_check(forge_ctx_create(setup_params(nnc), Coords, tn, MomentaValues, &
                                       nMomentaComponents, nUnitCell_1, nUnitCell_2, ctx), &
                      'forge_ctx_create')
-    call forge_attach(ctx, n, nnc)
+    call forge_attach(ctx, n, nnc, alpha, Delta, Coords, tn)
 
     allocate(zFock(n, n, nnc, numS), zFockIn(n, n, nnc, numS))
     allocate(Density(n, numS), DensityIn(n, numS), DensitySub(n), Potential(n, numS))

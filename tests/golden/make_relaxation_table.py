@@ -10,7 +10,9 @@ constants, rounded to binary32 before they are stored in real(dp); that
 rounding is reproduced here (correctly rounded from the decimal text), so
 each line holds the double the reference computes with.
 
-Runs only where the reference sources are mounted (/root/reference).
+Runs only where the reference sources are mounted (/root/reference); it sits
+beside the fixture generator because it reads the reference, which the
+product package never does.
 """
 from __future__ import annotations
 
@@ -22,7 +24,8 @@ from decimal import Decimal
 import numpy as np
 
 REF = "/root/reference/Geometry.f90"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "relaxation_fourier.txt")
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                   "forge_amd", "data", "relaxation_fourier.txt")
 
 
 def binary32(text: str) -> float:
@@ -62,7 +65,7 @@ def main() -> int:
             e[0 if comp == "x" else 1] = value(lit)
     with open(OUT, "w") as f:
         f.write("# ntheta n m uqx uqy: Fourier components of the relaxation profile\n")
-        f.write("# (binary64 values as the reference stores them), written by make_relaxation_table.py\n")
+        f.write("# (binary64 values as the reference stores them), written by tests/golden/make_relaxation_table.py\n")
         for t in sorted(table):
             for (n, mm) in sorted(table[t]):
                 ux, uy = table[t][(n, mm)]

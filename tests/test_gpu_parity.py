@@ -269,3 +269,26 @@ def test_mixed_precision_matches_fp64(case):
     assert d1 < 1e-6 and dEF < 1e-6
     assert nfx == 4 and dfx < 1e-6 and drho < 1e-3
     assert 0 < nmixed < len(tmx) and abs(tmx[-1, 8]) < 1e-8
+
+
+def test_hamiltonian_calls_leave_the_scf_state_intact():
+    """forge_tb_hamiltonian / forge_hamiltonian_hf between two forge_scf_run
+    calls: the loop continues from its own state (site density, potential,
+    energies), so scf_run(2) + Hamiltonians + scf_run(2) equals scf_run(4)."""
+    say("hamiltonian between scf_run calls")
+    e = engine_for("t2k3")
+    e.set_options(occ_mode=2, mix_mode=1, kT=0.001, beta=0.3)
+    start = seeded_rho(e.n, e.nnc, 1.0)
+    e.scf_init(start)
+    ref = e.scf_run(4)
+    rho_ref = e.density()
+    e.scf_init(start)
+    a = e.scf_run(2)
+    pot = np.linspace(-0.3, 0.2, e.n)
+    e.tb_hamiltonian(np.array([0.1, -0.2]))
+    e.hamiltonian_hf(np.array([0.05, 0.3]), pot, start)
+    b = e.scf_run(2)
+    e.set_options()
+    got = np.vstack([a, b])
+    np.testing.assert_array_equal(got[:, 1:9], ref[:, 1:9])
+    np.testing.assert_array_equal(e.density(), rho_ref)

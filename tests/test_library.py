@@ -48,6 +48,20 @@ def test_ctx_create_rejects_bad_params_before_any_device_work():
     assert rc < 0 and "fphase" in L.last_error()
 
 
+@pytest.mark.parametrize("nfilling", [40.0, -40.0])
+def test_ctx_create_rejects_a_filling_outside_the_band_window(nfilling):
+    # nF = nint(Nk (ndim/2 - nLower + 1) + Nk/2 nu) must leave a state above
+    # and below the Fermi level inside the window (Main.f90:400-401)
+    lib = L.load()
+    p = Setup(ntheta=1, numk=3, nfilling=nfilling).params(4)
+    ctx = ctypes.c_void_p()
+    z = np.zeros(64)
+    i = np.zeros(64, np.int64)
+    rc = lib.forge_ctx_create(ctypes.byref(p), z.ctypes.data, z.ctypes.data, z.ctypes.data,
+                              i.ctypes.data, i.ctypes.data, i.ctypes.data, ctypes.byref(ctx))
+    assert rc == -2 and "band window" in L.last_error()
+
+
 @pytest.mark.parametrize("case", ["t1k3", "t2k3", "t3k6", "tri1k3"])
 def test_product_geometry_is_bitwise_reference(case):
     f = load_case(case)
