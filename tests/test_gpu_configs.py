@@ -1,0 +1,179 @@
+"""The benchmark configurations at their full sizes on the GPU, through
+properties that hold at any size (the oracle does not finish at these n).
+
+  1.61 deg TBG, cell index 20, n = 5044, 4 x 4 k, nu = +1
+      FP64 window eigenvalues of one k against LAPACK (scipy zheevr range 'I',
+      the reference's own call, LapackRoutines.f90:414) within 1e-8 eV; the
+      mixed path against the FP64 path on identical inputs: window
+      eigenvalues of all 16 k within 5e-8 eV, the total energy of one SCF
+      iteration within 1e-9 eV per atom, the Fermi level within 1e-7 eV, the
+      density matrix within 1e-3 in Frobenius norm (of ~50).
+  twist sweep, cells m = 8..14 (n = 868..2524), 3 x 3 k each, fillings
+      seeded uniform in [-2, 2]: three iterations of the uneven-size batch
+      (63 eigensolves per iteration) against the systems solved one at a time
+      within 1e-8 eV in FP64 and 1e-6 eV in the mixed path.
+  1.05 deg TBG, cell index 31, n = 11908, 3 x 3 k, nu = +1
+      the learned start's first |dE| is far below the seeded start's, and one
+      mixed iteration from the learned state matches one FP64 iteration from
+      it within 1e-9 eV per atom (energy) and 1e-7 eV (Fermi level).
+  1.05 deg twisted trilayer, cell index 31, n = 17862, 3 x 3 k, nu = +2
+      one mixed and one FP64 iteration from the seeded start: the same bounds.
+
+Energies are compared per atom (the north star's unit); the per-cell
+difference is printed.  Each engine is released before the next is built.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from forge_amd.hf import Engine, Setup, Sweep
+from forge_amd.seed import seeded_rho
+
+pytestmark = pytest.mark.gpu
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "forge_amd",
+                     "data", "guess_mlp.npz")
+OPTS = dict(occ_mode=2, kT=0.001)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def full(H):
+    return np.tril(H) + np.tril(H, -1).conj().T
+
+
+def potential(e, rho, L):
+    """Hartree + Hubbard potential of the site densities (Main.f90:577-585)."""
+    d = np.real(np.diag(rho[:, :, 0])) - 0.5
+    return 2.0 * e.p.alphaH * (L @ d) + e.p.U * d
+
+
+def vk_of(e, k):
+    g = e.geom
+    i1, i2 = g.nMomentaComponents[k, 0], g.nMomentaComponents[k, 1]
+    return np.array([g.MomentaValues[i1, i2, 0], g.MomentaValues[i1, i2, 1]])
+
+
+def one_iteration(e, rho, precision, **opts):
+    e.set_options(**OPTS, **opts, precision=precision, switch_de=1e-14)
+    e.scf_init(rho)
+    r = e.scf_run(1)[0]
+    assert int(e.timings()[12]) == precision  # the iteration ran in the asked precision
+    return r
+
+
+def test_config_tbg_n5044_k4_fp64_and_mixed():
+    e = Engine(Setup(ntheta=20, numk=4, nfilling=1.0, xi_nm=40.0))
+    say("n5044 k4: n =", e.n, "Nk =", e.Nk)
+    assert e.n == 5044 and e.Nk == 16
+    e.set_options(**OPTS, mix_mode=0)
+    e.scf_init(seeded_rho(e.n, e.nnc, 1.0))
+    e.scf_run(3)
+    rho = e.density()
+    pot = potential(e, rho, e.longrange())
+    # FP64 against LAPACK on the same H(k)
+    H = full(e.hamiltonian_hf(vk_of(e, 0), pot, rho))
+    nL, nU = int(e.p.nLower), int(e.p.nUpper)
+    w = sla.eigh(H, eigvals_only=True, subset_by_index=[nL - 1, nU - 1], driver="evr")
+    del H
+    bands = {}
+    for prec in (0, 1):
+        e.set_options(**OPTS, mix_mode=1, beta=1.0, precision=prec, switch_de=1e-14)
+        _, b, _ = e.solve(rho, pot)
+        bands[prec] = b
+    d_lapack = np.abs(bands[0][:, 0] - w).max()
+    d_mixed = np.abs(bands[1] - bands[0]).max()
+    say(f"  FP64 window eigenvalues vs LAPACK (k = 0): {d_lapack:.2e} eV; mixed vs FP64 (16 k): "
+        f"{d_mixed:.2e} eV")
+    assert d_lapack < 1e-8
+    assert d_mixed < 5e-8
+    # one SCF iteration from the same state in both precisions
+    r64 = one_iteration(e, rho, 0, mix_mode=1, beta=1.0)
+    rho64 = e.density()
+    rmx = one_iteration(e, rho, 1, mix_mode=1, beta=1.0)
+    drho = np.linalg.norm(e.density() - rho64)
+    dE = (rmx[1] - rmx[8]) - (r64[1] - r64[8])
+    dEF = rmx[9] - r64[9]
+    say(f"  one iteration: dE_tot {dE:+.2e} eV per cell ({dE / e.n:+.2e} per atom), dE_F {dEF:+.2e} eV, "
+        f"|drho|_F {drho:.2e} of {np.linalg.norm(rho64):.1f}")
+    assert abs(dE) / e.n < 1e-9 and abs(dEF) < 1e-7 and drho < 1e-3
+    e.close()
+
+
+SWEEP = [8, 9, 10, 11, 12, 13, 14]
+
+
+def test_config_twist_sweep_m8_14_k3():
+    rng = np.random.default_rng(20261203)
+    fill = [float(x) for x in rng.uniform(-2.0, 2.0, len(SWEEP))]
+    engs = [Engine(Setup(ntheta=m, numk=3, nfilling=f, xi_nm=40.0)) for m, f in zip(SWEEP, fill)]
+    say("sweep m = 8..14:", [e.n for e in engs], "fillings", [round(f, 3) for f in fill])
+    assert [e.n for e in engs] == [868, 1084, 1324, 1588, 1876, 2188, 2524]
+    for prec in (0, 1):
+        single = []
+        for e, f in zip(engs, fill):
+            e.set_options(**OPTS, mix_mode=0, precision=prec, switch_de=1e-14)
+            e.scf_init(seeded_rho(e.n, e.nnc, f))
+            single.append(e.scf_run(3))
+        for e, f in zip(engs, fill):
+            e.set_options(**OPTS, mix_mode=0, precision=prec, switch_de=1e-14)
+            e.scf_init(seeded_rho(e.n, e.nnc, f))
+        sw = Sweep(engs)
+        batched = sw.run(3)
+        sw.close()
+        worst = 0.0
+        for e, a, b in zip(engs, single, batched):
+            assert len(a) == len(b) == 3
+            worst = max(worst, float(np.abs(b[:, 1] - a[:, 1]).max()),
+                        float(np.abs((b[:, 1] - b[:, 8]) - (a[:, 1] - a[:, 8])).max()),
+                        float(np.abs(b[:, 9] - a[:, 9]).max()))
+            np.testing.assert_array_equal(b[:, 10:12], a[:, 10:12])
+        say(f"  precision {prec}: max |sweep - single| over 3 iterations, 7 systems {worst:.2e} eV")
+        # FP64: the batched reduction rounds differently at the 1e-15 level;
+        # mixed: those differences can flip roundings of the FP32 copy
+        assert worst < (1e-8 if prec == 0 else 1e-6)
+    for e in engs:
+        e.close()
+
+
+def test_config_tbg_n11908_k3_learned_and_mixed():
+    from forge_amd.learned import learned_start
+    from forge_amd.mlp import Mlp
+    e = Engine(Setup(ntheta=31, numk=3, nfilling=1.0, xi_nm=40.0))
+    say("n11908 k3: n =", e.n)
+    assert e.n == 11908
+    e.set_options(**OPTS, mix_mode=0)
+    learned_start(e, Mlp.load(MODEL), 1.0)
+    rho = e.density()
+    r64 = one_iteration(e, rho, 0, mix_mode=0)
+    rmx = one_iteration(e, rho, 1, mix_mode=0)
+    del rho
+    rs = one_iteration(e, seeded_rho(e.n, e.nnc, 1.0), 0, mix_mode=0)
+    dE = (rmx[1] - rmx[8]) - (r64[1] - r64[8])
+    dEF = rmx[9] - r64[9]
+    say(f"  first |dE|: learned {abs(r64[8]):.2e} eV, seeded {abs(rs[8]):.2e} eV; mixed - FP64 from the "
+        f"learned state: dE_tot {dE:+.2e} eV per cell ({dE / e.n:+.2e} per atom), dE_F {dEF:+.2e} eV")
+    assert abs(r64[8]) < 1e-4 * abs(rs[8]) and abs(r64[8]) < 0.1
+    assert abs(dE) / e.n < 1e-9 and abs(dEF) < 1e-7
+    e.close()
+
+
+def test_config_ttg_n17862_k3_mixed_vs_fp64():
+    e = Engine(Setup(ntheta=31, nlayers=3, numk=3, nfilling=2.0, xi_nm=40.0))
+    say("trilayer n17862 k3: n =", e.n)
+    assert e.n == 17862
+    rho = seeded_rho(e.n, e.nnc, 2.0)
+    r64 = one_iteration(e, rho, 0, mix_mode=0)
+    rmx = one_iteration(e, rho, 1, mix_mode=0)
+    dE = (rmx[1] - rmx[8]) - (r64[1] - r64[8])
+    dEF = rmx[9] - r64[9]
+    say(f"  mixed - FP64, one iteration from the seeded start: dE_tot {dE:+.2e} eV per cell "
+        f"({dE / e.n:+.2e} per atom), dE_F {dEF:+.2e} eV, step {r64[7]:.6f} / {rmx[7]:.6f}")
+    assert abs(dE) / e.n < 1e-9 and abs(dEF) < 1e-7
+    e.close()
