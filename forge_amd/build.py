@@ -14,8 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT, "libforge_hip.so")
-SOURCES = ["operators.hip", "tridiag.hip", "dc.hip", "backtr.hip", "density.hip", "c3.hip",
-           "sym.hip", "valley.hip", "mlp.hip", "capi.hip"]
+SOURCES = ["operators.hip", "tridiag.hip", "dc.hip", "backtr.hip", "gemm.hip", "density.hip",
+           "c3.hip", "sym.hip", "valley.hip", "mlp.hip", "capi.hip"]
 FSRC = os.path.join(HERE, "fortran")
 # dependency order: the C-ABI binding, parameters, geometry, shims, driver
 FORTRAN = ["forge_c.f90", "forge_setup.f90", "forge_geometry.f90", "forge_hartreefock.f90",
@@ -43,8 +43,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs,
-           f"-L{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib", "-lrocblas"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

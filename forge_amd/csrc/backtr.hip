@@ -4,11 +4,11 @@
 // LapackRoutines.f90:414-426).  Reflectors are applied in blocks of BT in the
 // compact WY form  H_i0 ... H_{i0+b-1} = I - V T V^H  (T upper triangular,
 // LAPACK zlarft 'F','C'), last block first:
-//     Y = V^H C,  Y <- T Y,  C <- C - V Y          (rocBLAS gemm, batched over k)
+//     Y = V^H C,  Y <- T Y,  C <- C - V Y    (matrix-core GEMMs of gemm.hip, batched over k)
 // The clean V (unit diagonal, zeros above) and T are built by small kernels.
 //
 // FP32 variant (the mixed-precision phase of the SCF): the same blocks in
-// complex FP32 (cgemm on the FP32 matrix cores, V, T and C rounded to FP32),
+// complex FP32 (the FP32 matrix cores, V, T and C rounded to FP32),
 // then every column is scaled to unit 2-norm in FP64 and written to the FP64
 // output.  Eigenvector errors then enter the band energies only at second
 // order: for psi = psi* + d, <psi|H|psi> / <psi|psi> - eps = O(|d|^2), while
@@ -22,8 +22,6 @@
 // the promoted eigenvectors twice; the second run is timed with HIP events
 // and the width with the least time per reflector is kept for the shape for
 // the rest of the process.
-#include <rocblas/rocblas.h>
-
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -140,45 +138,22 @@ __global__ void __launch_bounds__(256) k_bt_normalise(i64 n, const float2* __res
   for (i64 r = threadIdx.x; r < n; r += 256) c64[r] = inv * widen(c32[r]);
 }
 
-#define RB(call)                                                              \
-  do {                                                                        \
-    rocblas_status _s = (call);                                               \
-    if (_s != rocblas_status_success) {                                       \
-      set_error(std::string("rocblas: ") + rocblas_status_to_string(_s) + " (" + #call + ")"); \
-      return -1;                                                              \
-    }                                                                         \
-  } while (0)
-
-rocblas_handle blas_handle(hipStream_t s);
-
 #define CK_BT(x)              \
   do {                        \
     int _r = (x);             \
     if (_r != 0) return _r;   \
   } while (0)
 
-// rocBLAS complex gemm for each storage type
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, i64 M,
-                              i64 N, i64 Kd, double alpha, const z2* A, i64 lda, size_t sA,
-                              const z2* B, i64 ldb, size_t sB, double beta, z2* C, i64 ldc,
-                              size_t sC, int batch) {
-  const rocblas_double_complex a(alpha, 0.0), b(beta, 0.0);
-  return rocblas_zgemm_strided_batched(
-      h, ta, tb, (rocblas_int)M, (rocblas_int)N, (rocblas_int)Kd, &a,
-      (const rocblas_double_complex*)A, (rocblas_int)lda, (rocblas_stride)sA,
-      (const rocblas_double_complex*)B, (rocblas_int)ldb, (rocblas_stride)sB, &b,
-      (rocblas_double_complex*)C, (rocblas_int)ldc, (rocblas_stride)sC, batch);
+// the matrix-core GEMM of each storage type (gemm.hip)
+static int gemm_sb(Op ta, Op tb, i64 M, i64 N, i64 Kd, double alpha, const z2* A, i64 lda,
+                   size_t sA, const z2* B, i64 ldb, size_t sB, double beta, z2* C, i64 ldc,
+                   size_t sC, int batch, hipStream_t s) {
+  return zgemm(ta, tb, M, N, Kd, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
 }
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, i64 M,
-                              i64 N, i64 Kd, double alpha, const float2* A, i64 lda, size_t sA,
-                              const float2* B, i64 ldb, size_t sB, double beta, float2* C,
-                              i64 ldc, size_t sC, int batch) {
-  const rocblas_float_complex a((float)alpha, 0.0f), b((float)beta, 0.0f);
-  return rocblas_cgemm_strided_batched(
-      h, ta, tb, (rocblas_int)M, (rocblas_int)N, (rocblas_int)Kd, &a,
-      (const rocblas_float_complex*)A, (rocblas_int)lda, (rocblas_stride)sA,
-      (const rocblas_float_complex*)B, (rocblas_int)ldb, (rocblas_stride)sB, &b,
-      (rocblas_float_complex*)C, (rocblas_int)ldc, (rocblas_stride)sC, batch);
+static int gemm_sb(Op ta, Op tb, i64 M, i64 N, i64 Kd, double alpha, const float2* A, i64 lda,
+                   size_t sA, const float2* B, i64 ldb, size_t sB, double beta, float2* C,
+                   i64 ldc, size_t sC, int batch, hipStream_t s) {
+  return cgemm(ta, tb, M, N, Kd, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
 }
 
 // C (ldC, stride sC per matrix) <- Q Z, blocks of BT reflectors last to first,
@@ -201,9 +176,7 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
   FORGE_LAUNCH();
   if (promoted) FORGE_HIP(hipEventRecord(promoted, s));
   if (n < 2) return 0;
-  rocblas_handle h = blas_handle(s);
-  if (!h) return -1;
-  const auto CT = rocblas_operation_conjugate_transpose, NO = rocblas_operation_none;
+  const Op CT = Op::H, NO = Op::N;
   const i64 nblocks = (n - 1 + BT - 1) / BT;
   for (i64 bq = first_only ? 0 : nblocks - 1; bq >= 0; --bq) {
     const i64 i0 = bq * BT;
@@ -213,15 +186,15 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
                        dim3(256), 0, s, n, i0, nb, t.A, t.sA, t.lda, V, sV);
     FORGE_LAUNCH();
     // G = V^H V
-    RB(gemm_sb(h, CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K));
+    CK_BT(gemm_sb(CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K, s));
     hipLaunchKernelGGL((k_bt_tbuild<E, BT>), dim3(nb, K), dim3(64), 0, s, nb, i0, n, G, t.tau, T);
     FORGE_LAUNCH();
     // Y = V^H C(i0+1:n, :)
-    RB(gemm_sb(h, CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K));
+    CK_BT(gemm_sb(CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K, s));
     // Y2 = T Y
-    RB(gemm_sb(h, NO, NO, nb, m, nb, 1.0, T, BT, sG, Y, BT, sY, 0.0, Y2, BT, sY, K));
+    CK_BT(gemm_sb(NO, NO, nb, m, nb, 1.0, T, BT, sG, Y, BT, sY, 0.0, Y2, BT, sY, K, s));
     // C(i0+1:n, :) -= V Y2
-    RB(gemm_sb(h, NO, NO, mt, m, nb, -1.0, V, n, sV, Y2, BT, sY, 1.0, C + i0 + 1, ldC, sC, K));
+    CK_BT(gemm_sb(NO, NO, mt, m, nb, -1.0, V, n, sV, Y2, BT, sY, 1.0, C + i0 + 1, ldC, sC, K, s));
   }
   return 0;
 }

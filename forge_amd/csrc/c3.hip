@@ -11,15 +11,12 @@
 // p is the site permutation of the +120 degree rotation within each layer and
 // sublattice half, and f, g are 1 off the corners.  Both kernels are gathers
 // bound by HBM, with one pass over their output.
-#include <rocblas/rocblas.h>
 
 #include <vector>
 
 #include "kernels.hpp"
 
 namespace forge {
-
-rocblas_handle blas_handle(hipStream_t s);
 
 // Corner factor of site i in factor set `set` (-1: none).  tab holds, per
 // set and layer, the factors of the first corner (last site of the first
@@ -85,7 +82,7 @@ __global__ void __launch_bounds__(256) k_c3_bulk(C3Geom g, int nnc, const z2* __
 }
 
 // Bulk density of K orbit representatives solved in one batch (columns of C,
-// ld ldC, stride sC; bands 1..nocc): per representative one rank-nocc zherk
+// ld ldC, stride sC; bands 1..nocc): per representative one rank-nocc HERK
 // into the scratch matrix after the nnc cells of rho, then its three images
 // into every cell.  accumulate = false overwrites at the first representative.
 int bulk_density_c3(const C3Geom& g, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
@@ -93,21 +90,12 @@ int bulk_density_c3(const C3Geom& g, int K, const z2* C, size_t ldC, size_t sC, 
   const i64 n = g.n;
   const size_t nn = (size_t)n * n;
   z2* R = rho + nnc * nn;
-  rocblas_handle h = blas_handle(s);
-  if (!h) return -1;
-  const double one = 1.0, zero = 0.0;
   if (n > 65535) { set_error("bulk_density_c3: n above the grid's y limit"); return -2; }
   dim3 grid((unsigned)cdiv(n, 256), (unsigned)n);
   for (int k = 0; k < K; ++k) {
     if (nocc > 0) {
-      rocblas_status st = rocblas_zherk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)n,
-                                        (rocblas_int)nocc, &one,
-                                        (const rocblas_double_complex*)(C + k * sC), (rocblas_int)ldC,
-                                        &zero, (rocblas_double_complex*)R, (rocblas_int)n);
-      if (st != rocblas_status_success) {
-        set_error(std::string("rocblas_zherk: ") + rocblas_status_to_string(st));
-        return -1;
-      }
+      const int r = herk_lower(n, nocc, C + k * sC, (i64)ldC, R, n, s);
+      if (r) return r;
     } else {
       FORGE_HIP(hipMemsetAsync(R, 0, sizeof(z2) * nn, s));
     }

@@ -1,7 +1,5 @@
 // C ABI of libforge_hip: context, reference-interface entry points and the
 // device-resident SCF loop (the call sequence of Main.f90:236-679, numS = 1).
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -17,25 +15,6 @@ namespace forge {
 
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
-
-static std::mutex g_blas_mu;
-static rocblas_handle g_blas[16] = {};
-rocblas_handle blas_handle(hipStream_t s) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) {
-    set_error("blas_handle: no device");
-    return nullptr;
-  }
-  std::lock_guard<std::mutex> lk(g_blas_mu);
-  if (!g_blas[dev]) {
-    if (rocblas_create_handle(&g_blas[dev]) != rocblas_status_success) {
-      set_error("rocblas_create_handle failed");
-      return nullptr;
-    }
-  }
-  rocblas_set_stream(g_blas[dev], s);
-  return g_blas[dev];
-}
 
 static const double PI = 3.14159265358979323846264338327950288;
 
@@ -571,6 +550,133 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   delete[] t.evs;
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+// ---------------------------------------------------------------- products of the LAPACK shim
+// The reference's product routines (LapackRoutines.f90:119-236) on the
+// matrix-core GEMMs of gemm.hip, for host arrays: each call copies its
+// operands to the device and the result back (stateless, like zdiagRed).
+
+// C += z R (or C = z R when overwrite), n elements
+__global__ void k_zaxpy(size_t n, z2 zf, const z2* __restrict__ R, z2* __restrict__ C, int overwrite) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const z2 v = zf * R[q];
+  C[q] = overwrite ? v : C[q] + v;
+}
+
+namespace {
+// device buffers and a stream of one host-array call, released on every path
+struct Scoped {
+  hipStream_t s = nullptr;
+  std::vector<void*> bufs;
+  ~Scoped() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (void* b : bufs) (void)hipFree(b);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  int init() {
+    FORGE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return 0;
+  }
+  template <typename T> int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    FORGE_HIP(hipMalloc(&q, sizeof(T) * (count ? count : 1)));
+    bufs.push_back(q);
+    *p = (T*)q;
+    return 0;
+  }
+};
+}  // namespace
+
+extern "C" int forge_z_vector_mul(forge_z* C, const forge_z* A, int64_t n, int64_t k,
+                                  const forge_z* z_factor) {
+  if (!C || !A || n < 1 || k < 1) { set_error("forge_z_vector_mul: invalid argument"); return -2; }
+  if (forge_device_count() < 1) { set_error("forge_z_vector_mul: no HIP device"); return -3; }
+  Scoped sc;
+  CK(sc.init());
+  z2 *dA, *dR, *dC;
+  const size_t nn = (size_t)n * n;
+  CK(sc.alloc(&dA, (size_t)n * k));
+  CK(sc.alloc(&dR, nn));
+  CK(sc.alloc(&dC, nn));
+  FORGE_HIP(hipMemcpyAsync(dA, A, sizeof(z2) * n * k, hipMemcpyHostToDevice, sc.s));
+  FORGE_HIP(hipMemcpyAsync(dC, C, sizeof(z2) * nn, hipMemcpyHostToDevice, sc.s));
+  CK(zgemm(Op::N, Op::H, n, n, k, 1.0, dA, n, 0, dA, n, 0, 0.0, dR, n, 0, 1, sc.s));
+  const z2 zf = z_factor ? mk(z_factor->re, z_factor->im) : mk(1.0, 0.0);
+  hipLaunchKernelGGL(k_zaxpy, dim3((unsigned)cdiv((i64)nn, 256)), dim3(256), 0, sc.s, nn, zf, dR, dC, 0);
+  FORGE_LAUNCH();
+  FORGE_HIP(hipMemcpyAsync(C, dC, sizeof(z2) * nn, hipMemcpyDeviceToHost, sc.s));
+  FORGE_HIP(hipStreamSynchronize(sc.s));
+  return 0;
+}
+
+extern "C" int forge_z_matrix_mul(forge_z* C, const forge_z* A, const forge_z* B, int64_t m,
+                                  int64_t k, int64_t n) {
+  if (!C || !A || !B || m < 1 || k < 1 || n < 1) {
+    set_error("forge_z_matrix_mul: invalid argument");
+    return -2;
+  }
+  if (forge_device_count() < 1) { set_error("forge_z_matrix_mul: no HIP device"); return -3; }
+  Scoped sc;
+  CK(sc.init());
+  z2 *dA, *dB, *dC;
+  CK(sc.alloc(&dA, (size_t)m * k));
+  CK(sc.alloc(&dB, (size_t)k * n));
+  CK(sc.alloc(&dC, (size_t)m * n));
+  FORGE_HIP(hipMemcpyAsync(dA, A, sizeof(z2) * m * k, hipMemcpyHostToDevice, sc.s));
+  FORGE_HIP(hipMemcpyAsync(dB, B, sizeof(z2) * k * n, hipMemcpyHostToDevice, sc.s));
+  CK(zgemm(Op::N, Op::N, m, n, k, 1.0, dA, m, 0, dB, k, 0, 0.0, dC, m, 0, 1, sc.s));
+  FORGE_HIP(hipMemcpyAsync(C, dC, sizeof(z2) * m * n, hipMemcpyDeviceToHost, sc.s));
+  FORGE_HIP(hipStreamSynchronize(sc.s));
+  return 0;
+}
+
+extern "C" int forge_z_matrix_overlap(forge_z* C, const forge_z* A, const forge_z* B, int64_t m1,
+                                      int64_t m0, const forge_z* z_factor) {
+  if (!C || !A || !B || m1 < 1 || m0 < 1) {
+    set_error("forge_z_matrix_overlap: invalid argument");
+    return -2;
+  }
+  if (forge_device_count() < 1) { set_error("forge_z_matrix_overlap: no HIP device"); return -3; }
+  Scoped sc;
+  CK(sc.init());
+  z2 *dA, *dB, *dR;
+  const size_t mm = (size_t)m0 * m0;
+  CK(sc.alloc(&dA, (size_t)m1 * m0));
+  CK(sc.alloc(&dB, (size_t)m1 * m0));
+  CK(sc.alloc(&dR, 2 * mm));
+  FORGE_HIP(hipMemcpyAsync(dA, A, sizeof(z2) * m1 * m0, hipMemcpyHostToDevice, sc.s));
+  FORGE_HIP(hipMemcpyAsync(dB, B, sizeof(z2) * m1 * m0, hipMemcpyHostToDevice, sc.s));
+  CK(zgemm(Op::H, Op::N, m0, m0, m1, 1.0, dA, m1, 0, dB, m1, 0, 0.0, dR, m0, 0, 1, sc.s));
+  const z2 zf = z_factor ? mk(z_factor->re, z_factor->im) : mk(1.0, 0.0);
+  hipLaunchKernelGGL(k_zaxpy, dim3((unsigned)cdiv((i64)mm, 256)), dim3(256), 0, sc.s, mm, zf, dR,
+                     dR + mm, 1);
+  FORGE_LAUNCH();
+  FORGE_HIP(hipMemcpyAsync(C, dR + mm, sizeof(z2) * mm, hipMemcpyDeviceToHost, sc.s));
+  FORGE_HIP(hipStreamSynchronize(sc.s));
+  return 0;
+}
+
+extern "C" int forge_c_matrix_mul(forge_c* C, const forge_c* A, const forge_c* B, int64_t m,
+                                  int64_t k, int64_t n) {
+  if (!C || !A || !B || m < 1 || k < 1 || n < 1) {
+    set_error("forge_c_matrix_mul: invalid argument");
+    return -2;
+  }
+  if (forge_device_count() < 1) { set_error("forge_c_matrix_mul: no HIP device"); return -3; }
+  Scoped sc;
+  CK(sc.init());
+  float2 *dA, *dB, *dC;
+  CK(sc.alloc(&dA, (size_t)m * k));
+  CK(sc.alloc(&dB, (size_t)k * n));
+  CK(sc.alloc(&dC, (size_t)m * n));
+  FORGE_HIP(hipMemcpyAsync(dA, A, sizeof(float2) * m * k, hipMemcpyHostToDevice, sc.s));
+  FORGE_HIP(hipMemcpyAsync(dB, B, sizeof(float2) * k * n, hipMemcpyHostToDevice, sc.s));
+  CK(cgemm(Op::N, Op::N, m, n, k, 1.0, dA, m, 0, dB, k, 0, 0.0, dC, m, 0, 1, sc.s));
+  FORGE_HIP(hipMemcpyAsync(C, dC, sizeof(float2) * m * n, hipMemcpyDeviceToHost, sc.s));
+  FORGE_HIP(hipStreamSynchronize(sc.s));
+  return 0;
 }
 
 // ---------------------------------------------------------------- C3-reduced k-set

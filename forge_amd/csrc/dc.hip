@@ -22,18 +22,15 @@
 #include <utility>
 #include <vector>
 
-#include <rocblas/rocblas.h>
-
 #include "kernels.hpp"
 
 namespace forge {
 
-rocblas_handle blas_handle(hipStream_t s);
-
 constexpr int LEAF = 32;
-// merges of at least this order multiply on rocBLAS dgemm (matrix cores);
-// smaller ones on k_dc_gemm, where one launch covers all merges of a level
-constexpr int DC_BLAS_MIN = 2048;
+// merges of at least this order multiply on the FP64 matrix cores (dgemm,
+// gemm.hip, batched over the K matrices, one call per merge half); smaller
+// ones on k_dc_gemm, where one launch covers all merges of a level
+constexpr int DC_GEMM_MIN = 2048;
 constexpr double EPSU = 1.1102230246251565e-16;  // unit roundoff, 2^-53
 
 struct MergeDesc {
@@ -745,28 +742,19 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
     // Q_out for the top merge is n x mout with ld n (caller buffer K x n x mout)
     if (top) {
       // the top merge is the bulk of the work: its two half products go to
-      // the FP64 matrix cores (rocBLAS), all K matrices in one batched call
+      // the FP64 matrix cores (dgemm), all K matrices in one batched call
       // each; caller layout: stride n * mout per matrix
       if (nm == 1) {
         const MergeDesc& M = all[levels[lv].first];
-        rocblas_handle h = blas_handle(s);
-        if (!h) return -1;
-        const double one = 1.0, zero = 0.0;
         for (int half = 0; half < 2; ++half) {
           const int rb = half ? M.n1 : 0, rn = half ? M.n2 : M.n1;
           if (rn == 0 || M.mout == 0) continue;
           const size_t a0 = (size_t)(M.o + rb) + (size_t)n * (M.o + rb);
           const size_t b0 = (size_t)(M.o + rb) + (size_t)n * M.o;
           const size_t c0 = (size_t)(M.o + rb) + (size_t)ldo * M.o;
-          const rocblas_status st = rocblas_dgemm_strided_batched(
-              h, rocblas_operation_none, rocblas_operation_none, rn, M.mout, rn, &one, P.Qa + a0,
-              (rocblas_int)n, (rocblas_stride)((size_t)n * n), P.Wm + b0, (rocblas_int)n,
-              (rocblas_stride)((size_t)n * n), &zero, Qout + c0, (rocblas_int)ldo,
-              (rocblas_stride)((size_t)n * mout), K);
-          if (st != rocblas_status_success) {
-            set_error(std::string("rocblas_dgemm_strided_batched: ") + rocblas_status_to_string(st));
-            return -1;
-          }
+          const int r = dgemm(Op::N, Op::N, rn, M.mout, rn, 1.0, P.Qa + a0, n, (size_t)n * n,
+                              P.Wm + b0, n, (size_t)n * n, 0.0, Qout + c0, ldo, (size_t)n * mout, K, s);
+          if (r) return r;
         }
       } else {
         unsigned tiles = (unsigned)(((lhalf + 63) / 64) * ((lmout + 63) / 64));
@@ -781,13 +769,10 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
                          evals);
       FORGE_LAUNCH();
     } else {
-      if (lm >= DC_BLAS_MIN) {
-        // large lower merges: their half products on rocBLAS dgemm as well,
+      if (lm >= DC_GEMM_MIN) {
+        // large lower merges: their half products on the matrix cores as well,
         // one call per merge and half, batched over the K matrices
-        rocblas_handle h = blas_handle(s);
-        if (!h) return -1;
-        const double one = 1.0, zero = 0.0;
-        const rocblas_stride snn = (rocblas_stride)((size_t)n * n);
+        const size_t snn = (size_t)n * n;
         for (size_t q = 0; q < nm; ++q) {
           const MergeDesc& M = all[levels[lv].first + q];
           for (int half = 0; half < 2; ++half) {
@@ -795,14 +780,9 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
             if (rn == 0 || M.mout == 0) continue;
             const size_t a0 = (size_t)(M.o + rb) + (size_t)n * (M.o + rb);
             const size_t b0 = (size_t)(M.o + rb) + (size_t)n * M.o;
-            const rocblas_status st = rocblas_dgemm_strided_batched(
-                h, rocblas_operation_none, rocblas_operation_none, rn, M.mout, rn, &one,
-                P.Qa + a0, (rocblas_int)n, snn, P.Wm + b0, (rocblas_int)n, snn, &zero, dst + b0,
-                (rocblas_int)ldo, snn, K);
-            if (st != rocblas_status_success) {
-              set_error(std::string("rocblas_dgemm_strided_batched: ") + rocblas_status_to_string(st));
-              return -1;
-            }
+            const int r = dgemm(Op::N, Op::N, rn, M.mout, rn, 1.0, P.Qa + a0, n, snn, P.Wm + b0, n,
+                                snn, 0.0, dst + b0, ldo, snn, K, s);
+            if (r) return r;
           }
         }
       } else {

@@ -3,82 +3,36 @@
 // Bulk (bands 1..nLower-1 of every k): the reference accumulates nLower-1
 // rank-1 zgemm updates per k, T(i,j) = sum_b conj(psi_b(i)) psi_b(j), and adds
 // T e^{-2 pi i (i1 n1 + i2 (n1 + n2)) / numk} to every stored cell.  Here
-// R_k = C_occ C_occ^H is one rank-nocc Hermitian product per k (rocBLAS zherk,
-// lower), T = conj(R_k), and k_bulk_accumulate adds the phased copies to all
-// cells in one pass (both triangles from the lower one).
+// R_k = C_occ C_occ^H is one rank-nocc Hermitian product per k on the FP64
+// matrix cores (gemm.hip:herk_cells), whose epilogue adds the phased
+// T = conj(R_k) to every cell directly, both triangles from the lower tiles,
+// so R_k never goes through memory.
 //
 // Window (sorted states s of the 20-band window): the reference adds
 // w_s e^{-i k_s.T_c} conj(psi_s) psi_s^T per state and cell with w_s = 1 for
 // occupied states and zdeg for the degenerate shell.  k_window_density forms
 // all cells for all states in one LDS-tiled pass and writes
 // rho_new = (bulk + window) / Nk (Main.f90:510-511).
-#include <rocblas/rocblas.h>
-
 #include "kernels.hpp"
 
 namespace forge {
 
-// rho_c(i,j) += ph_c * conj(R(i,j)), R Hermitian with the lower triangle valid.
-// ph: nnc complex phases for this k.
-__global__ void __launch_bounds__(256) k_bulk_accumulate(i64 n, int nnc, const z2* __restrict__ R,
-                                                         const double* __restrict__ ph,
-                                                         z2* __restrict__ rho, int first) {
-  const size_t nn = (size_t)n * n;
-  __shared__ z2 tr[32][33];
-  const i64 I = blockIdx.y, J = blockIdx.x;  // 32x32 tile (rows I, cols J)
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  // stage R(J-tile rows, I-tile cols) for the upper-triangle reflection
-  for (int q = 0; q < 4; ++q) {
-    int cc = ty * 4 + q;
-    i64 r = J * 32 + tx, c = I * 32 + cc;
-    tr[tx][cc] = (r < n && c < n) ? R[r + n * c] : mk(0.0, 0.0);
-  }
-  __syncthreads();
-  for (int q = 0; q < 4; ++q) {
-    int cc = ty * 4 + q;
-    i64 i = I * 32 + tx, j = J * 32 + cc;
-    if (i >= n || j >= n) continue;
-    z2 Rij = (i >= j) ? R[i + n * j] : conjz(tr[cc][tx]);  // R(i,j) = conj(R(j,i)) above
-    z2 T = conjz(Rij);
-    size_t o = i + n * j;
-    for (int c = 0; c < nnc; ++c) {
-      z2 p = mk(ph[2 * c], ph[2 * c + 1]);
-      z2 add = p * T;
-      rho[c * nn + o] = first ? add : rho[c * nn + o] + add;
-    }
-  }
-}
-
-rocblas_handle blas_handle(hipStream_t s);
-
-// rho (nnc + 1 matrices: the cells, then the R scratch) receives
-// sum_k ph_c(k) conj(R_k); accumulate = false overwrites at the first k.
-// The product stays FP64 in every phase: rounding of R enters the energies at
-// first order (E_kin and the Hartree terms are linear in rho).
+// rho (nnc cells) receives sum_k ph_c(k) conj(R_k); accumulate = false
+// overwrites at the first k.  The product stays FP64 in every phase: rounding
+// of R enters the energies at first order (E_kin and the Hartree terms are
+// linear in rho).
 int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
                  const double* phase_cells, z2* rho, bool accumulate, hipStream_t s) {
   const size_t nn = (size_t)n * n;
-  z2* R = rho + nnc * nn;
-  rocblas_handle h = blas_handle(s);
-  if (!h) return -1;
-  const double one = 1.0, zero = 0.0;
-  dim3 grid(cdiv(n, 32), cdiv(n, 32));
   for (int k = 0; k < K; ++k) {
+    const bool first = k == 0 && !accumulate;
     if (nocc > 0) {
-      rocblas_status st = rocblas_zherk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)n,
-                                        (rocblas_int)nocc, &one,
-                                        (const rocblas_double_complex*)(C + k * sC), (rocblas_int)ldC,
-                                        &zero, (rocblas_double_complex*)R, (rocblas_int)n);
-      if (st != rocblas_status_success) {
-        set_error(std::string("rocblas_zherk: ") + rocblas_status_to_string(st));
-        return -1;
-      }
-    } else {
-      FORGE_HIP(hipMemsetAsync(R, 0, sizeof(z2) * nn, s));
+      const int r = herk_cells(n, nocc, C + k * sC, (i64)ldC, nnc, phase_cells + (size_t)k * nnc * 2,
+                               rho, first, s);
+      if (r) return r;
+    } else if (first) {
+      FORGE_HIP(hipMemsetAsync(rho, 0, sizeof(z2) * nnc * nn, s));
     }
-    hipLaunchKernelGGL(k_bulk_accumulate, grid, dim3(256), 0, s, n, nnc, R,
-                       phase_cells + (size_t)k * nnc * 2, rho, (k == 0 && !accumulate) ? 1 : 0);
-    FORGE_LAUNCH();
   }
   return 0;
 }

@@ -91,6 +91,25 @@ int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, 
                   size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32 = nullptr,
                   int* width_used = nullptr);
 
+// gemm.hip: matrix-core GEMMs (C = alpha op(A) op(B) + beta C, batched with
+// strides; Op::H = conjugate transpose, plain transpose for real data)
+enum class Op { N, H };
+int zgemm(Op ta, Op tb, i64 M, i64 N, i64 K, double alpha, const z2* A, i64 lda, size_t sA,
+          const z2* B, i64 ldb, size_t sB, double beta, z2* C, i64 ldc, size_t sC, int batch,
+          hipStream_t s);
+int cgemm(Op ta, Op tb, i64 M, i64 N, i64 K, double alpha, const float2* A, i64 lda, size_t sA,
+          const float2* B, i64 ldb, size_t sB, double beta, float2* C, i64 ldc, size_t sC,
+          int batch, hipStream_t s);
+int dgemm(Op ta, Op tb, i64 M, i64 N, i64 K, double alpha, const double* A, i64 lda, size_t sA,
+          const double* B, i64 ldb, size_t sB, double beta, double* C, i64 ldc, size_t sC,
+          int batch, hipStream_t s);
+// R = A A^H (n x n, rank k) on the lower 128 x 128 tiles (diagonal tiles whole)
+int herk_lower(i64 n, i64 k, const z2* A, i64 lda, z2* R, i64 ldr, hipStream_t s);
+// rho_c(i,j) (+)= ph_c conj(R(i,j)) for all i, j and the nnc cells (stride n^2),
+// R = A A^H; first: overwrite instead of accumulate
+int herk_cells(i64 n, i64 k, const z2* A, i64 lda, int nnc, const double* ph, z2* rho,
+               bool first, hipStream_t s);
+
 // density.hip
 int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
                  const double* phase_cells, z2* rho, bool accumulate, hipStream_t s);

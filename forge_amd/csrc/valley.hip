@@ -10,13 +10,10 @@
 // (reduction, divide and conquer, back-transform at order nb), and
 // C <- C U sign(Lambda) U^H.  V is built one k at a time into an n x n
 // scratch matrix.
-#include <rocblas/rocblas.h>
 
 #include "kernels.hpp"
 
 namespace forge {
-
-rocblas_handle blas_handle(hipStream_t s);
 
 #define CK(x)               \
   do {                      \
@@ -120,27 +117,13 @@ int valley_transform(const ValleyGeom& g, const double* vk, int K, i64 nb, z2* C
   z2* D = (z2*)(b + L.D);
   z2* Hs = (z2*)(b + L.Hs);
   const i64 lds = (nb + 15) & ~(i64)15;
-  rocblas_handle h = blas_handle(s);
-  if (!h) return -1;
-  const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
   for (int k = 0; k < K; ++k) {
     CK(valley_phase(g, vk[2 * k], vk[2 * k + 1], V, s));
     const z2* Ck = C + (size_t)k * n * nb;
-    rocblas_status st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
-                                      (rocblas_int)nb, (rocblas_int)n, &one,
-                                      (const rocblas_double_complex*)V, (rocblas_int)n,
-                                      (const rocblas_double_complex*)Ck, (rocblas_int)n, &zero,
-                                      (rocblas_double_complex*)D, (rocblas_int)n);
-    if (st == rocblas_status_success)
-      st = rocblas_zgemm(h, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                         (rocblas_int)nb, (rocblas_int)nb, (rocblas_int)n, &one,
-                         (const rocblas_double_complex*)Ck, (rocblas_int)n,
-                         (const rocblas_double_complex*)D, (rocblas_int)n, &zero,
-                         (rocblas_double_complex*)(Hs + (size_t)k * lds * nb), (rocblas_int)lds);
-    if (st != rocblas_status_success) {
-      set_error(std::string("valley_transform: rocblas_zgemm: ") + rocblas_status_to_string(st));
-      return -1;
-    }
+    // D = V C_k, then P_k = C_k^H D (lower triangle read by the eigensolver)
+    CK(zgemm(Op::N, Op::N, n, nb, n, 1.0, V, n, 0, Ck, n, 0, 0.0, D, n, 0, 1, s));
+    CK(zgemm(Op::H, Op::N, nb, nb, n, 1.0, Ck, n, 0, D, n, 0, 0.0, Hs + (size_t)k * lds * nb, lds, 0,
+             1, s));
   }
   // P_k = U_k Lambda_k U_k^H for all k
   TridiagWork t{};
@@ -164,17 +147,8 @@ int valley_transform(const ValleyGeom& g, const double* vk, int K, i64 nb, z2* C
   FORGE_LAUNCH();
   // C_k <- C_k S_k
   z2* T = (z2*)(b + L.T);
-  const rocblas_status st = rocblas_zgemm_strided_batched(
-      h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nb,
-      (rocblas_int)nb, &one, (const rocblas_double_complex*)C, (rocblas_int)n,
-      (rocblas_stride)(n * nb), (const rocblas_double_complex*)S, (rocblas_int)nb,
-      (rocblas_stride)(nb * nb), &zero, (rocblas_double_complex*)T, (rocblas_int)n,
-      (rocblas_stride)(n * nb), K);
-  if (st != rocblas_status_success) {
-    set_error(std::string("valley_transform: rocblas_zgemm_strided_batched: ") +
-              rocblas_status_to_string(st));
-    return -1;
-  }
+  CK(zgemm(Op::N, Op::N, n, nb, nb, 1.0, C, n, (size_t)n * nb, S, nb, (size_t)nb * nb, 0.0, T, n,
+           (size_t)n * nb, K, s));
   FORGE_HIP(hipMemcpyAsync(C, T, sizeof(z2) * n * nb * K, hipMemcpyDeviceToDevice, s));
   return 0;
 }

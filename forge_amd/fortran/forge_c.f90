@@ -20,6 +20,7 @@ module forge_c
   public :: forge_scf_init_guess, forge_scf_init_guess_update, forge_scf_init_update
   public :: forge_mlp_param_count, forge_mlp_loss_grad, forge_mlp_train, forge_mlp_infer
   public :: forge_valley_phase, forge_valley_transform, forge_scf_set_valley
+  public :: forge_z_vector_mul, forge_z_matrix_mul, forge_z_matrix_overlap, forge_c_matrix_mul
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -99,6 +100,43 @@ module forge_c
       integer(c_int32_t), intent(out) :: info
       integer(c_int) :: r
     end function forge_zdiagred
+
+    ! products of the LAPACK shim (LapackRoutines.f90:119-236); z_factor may be
+    ! c_null_ptr (1)
+    function forge_z_vector_mul(C, A, n, k, z_factor) bind(C, name="forge_z_vector_mul") result(r)
+      import :: c_int, c_int64_t, c_double_complex, c_ptr
+      complex(c_double_complex), intent(inout) :: C(*)
+      complex(c_double_complex), intent(in) :: A(*)
+      integer(c_int64_t), value :: n, k
+      type(c_ptr), value :: z_factor
+      integer(c_int) :: r
+    end function forge_z_vector_mul
+
+    function forge_z_matrix_mul(C, A, B, m, k, n) bind(C, name="forge_z_matrix_mul") result(r)
+      import :: c_int, c_int64_t, c_double_complex
+      complex(c_double_complex), intent(out) :: C(*)
+      complex(c_double_complex), intent(in) :: A(*), B(*)
+      integer(c_int64_t), value :: m, k, n
+      integer(c_int) :: r
+    end function forge_z_matrix_mul
+
+    function forge_z_matrix_overlap(C, A, B, m1, m0, z_factor) &
+        bind(C, name="forge_z_matrix_overlap") result(r)
+      import :: c_int, c_int64_t, c_double_complex, c_ptr
+      complex(c_double_complex), intent(out) :: C(*)
+      complex(c_double_complex), intent(in) :: A(*), B(*)
+      integer(c_int64_t), value :: m1, m0
+      type(c_ptr), value :: z_factor
+      integer(c_int) :: r
+    end function forge_z_matrix_overlap
+
+    function forge_c_matrix_mul(C, A, B, m, k, n) bind(C, name="forge_c_matrix_mul") result(r)
+      import :: c_int, c_int64_t, c_float_complex
+      complex(c_float_complex), intent(out) :: C(*)
+      complex(c_float_complex), intent(in) :: A(*), B(*)
+      integer(c_int64_t), value :: m, k, n
+      integer(c_int) :: r
+    end function forge_c_matrix_mul
 
     function forge_solve(ctx, zFockBulk, zFock, Potential, Bands, zEigenvectors) &
         bind(C, name="forge_solve") result(r)

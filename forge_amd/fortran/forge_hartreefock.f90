@@ -18,6 +18,9 @@
 !   OptimalStep              HartreeFock.f90:1148-1227
 !   TightBindingHamiltonian  TightBinding.f90:13-55
 !   zdiagRed                 LapackRoutines.f90:381-433
+!   z_vector_mul             LapackRoutines.f90:168-183   (A may hold k > 1 columns)
+!   z_matrix_mul, c_matrix_mul  LapackRoutines.f90:208-236
+!   z_matrix_overlap         LapackRoutines.f90:119-139
 !   C3_RelatedPoints         Geometry.f90:1172-1223    (host)
 !   C2_RelatedPoints         Geometry.f90:1140-1167    (host)
 !   Solve_C3                 HartreeFock.f90:131-398
@@ -33,6 +36,19 @@ module forge_hartreefock
   public :: C3_RelatedPoints, Solve_C3, GetFock_C3, SortEnergies_2Spins, C2_RelatedPoints
   public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
   public :: OptimalStep, TightBindingHamiltonian, zdiagRed
+  public :: vector_mul, z_vector_mul, matrix_mul, z_matrix_mul, c_matrix_mul
+  public :: matrix_overlap, z_matrix_overlap
+
+  ! generics of the reference's LAPACK shim (LapackRoutines.f90:21-37)
+  interface vector_mul
+    module procedure z_vector_mul
+  end interface
+  interface matrix_mul
+    module procedure z_matrix_mul, c_matrix_mul
+  end interface
+  interface matrix_overlap
+    module procedure z_matrix_overlap
+  end interface
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(dp), save :: ctx_ndim = -1, ctx_nnc = -1
@@ -171,6 +187,56 @@ contains
     call forge_check(rc, 'zdiagRed')
     if (info /= 0) write(*, *) 'zheev failed', info
   end subroutine zdiagRed
+
+  ! C = C + z_factor A A^H (the reference's rank-1 update with one column of
+  ! A; every column of A here)
+  subroutine z_vector_mul(A, C, z_factor)
+    complex(8), intent(in) :: A(:, :)
+    complex(8), intent(inout), target :: C(:, :)
+    complex(8), intent(in), optional, target :: z_factor
+    type(c_ptr) :: zp
+    complex(8), target :: zloc
+    zp = c_null_ptr
+    if (present(z_factor)) then
+      zloc = z_factor
+      zp = c_loc(zloc)
+    end if
+    call forge_check(forge_z_vector_mul(C, A, int(size(A, 1), c_int64_t), int(size(A, 2), c_int64_t), &
+                                        zp), 'z_vector_mul')
+  end subroutine z_vector_mul
+
+  ! C = A B
+  subroutine z_matrix_mul(A, B, C)
+    complex(8), intent(in) :: A(:, :), B(:, :)
+    complex(8), intent(inout) :: C(:, :)
+    call forge_check(forge_z_matrix_mul(C, A, B, int(size(A, 1), c_int64_t), &
+                                        int(size(A, 2), c_int64_t), int(size(B, 2), c_int64_t)), &
+                     'z_matrix_mul')
+  end subroutine z_matrix_mul
+
+  subroutine c_matrix_mul(A, B, C)
+    complex(4), intent(in) :: A(:, :), B(:, :)
+    complex(4), intent(inout) :: C(:, :)
+    call forge_check(forge_c_matrix_mul(C, A, B, int(size(A, 1), c_int64_t), &
+                                        int(size(A, 2), c_int64_t), int(size(B, 2), c_int64_t)), &
+                     'c_matrix_mul')
+  end subroutine c_matrix_mul
+
+  ! C = z_factor A^H B
+  subroutine z_matrix_overlap(A, B, C, z_factor)
+    complex(8), intent(in) :: A(:, :), B(:, :)
+    complex(8), intent(out) :: C(:, :)
+    complex(8), intent(in), optional :: z_factor
+    type(c_ptr) :: zp
+    complex(8), target :: zloc
+    zp = c_null_ptr
+    if (present(z_factor)) then
+      zloc = z_factor
+      zp = c_loc(zloc)
+    end if
+    call forge_check(forge_z_matrix_overlap(C, A, B, int(size(A, 1), c_int64_t), &
+                                            int(size(A, 2), c_int64_t), zp), 'z_matrix_overlap')
+  end subroutine z_matrix_overlap
 
   subroutine Solve(zFockBulk, zFock, Potential, alpha_, Bands, zEigenvectors, Coords, MomentaValues, &
                    nMomentaComponents, nLower_, nUpper_, ndim_, numk_, Nk_, numNeighborCells, &

@@ -380,3 +380,49 @@ def zdiagred(A, il, iu):
     L.check(lib.forge_zdiagred(_p(a), n, _p(w), int(il), int(iu), C.byref(info)), "forge_zdiagred")
     m = iu - il + 1
     return w[:m], a[:, :m], info.value
+
+
+def z_vector_mul(A, C, z_factor=None):
+    """z_vector_mul(A, C, z_factor) (LapackRoutines.f90:168-183): C + z A A^H,
+    A n x k (the reference passes one column)."""
+    lib = L.load()
+    A = _f(np.atleast_2d(np.asarray(A).T).T, np.complex128)
+    out = _f(C, np.complex128).copy(order="F")
+    n, k = A.shape
+    z = None if z_factor is None else np.array([z_factor], np.complex128)
+    L.check(lib.forge_z_vector_mul(_p(out), _p(A), n, k, _p(z)), "forge_z_vector_mul")
+    return out
+
+
+def z_matrix_mul(A, B):
+    """z_matrix_mul(A, B, C) (LapackRoutines.f90:208-220): A B."""
+    lib = L.load()
+    A, B = _f(A, np.complex128), _f(B, np.complex128)
+    m, k = A.shape
+    n = B.shape[1]
+    out = np.zeros((m, n), np.complex128, order="F")
+    L.check(lib.forge_z_matrix_mul(_p(out), _p(A), _p(B), m, k, n), "forge_z_matrix_mul")
+    return out
+
+
+def z_matrix_overlap(A, B, z_factor=None):
+    """z_matrix_overlap(A, B, C, z_factor) (LapackRoutines.f90:119-139): z A^H B."""
+    lib = L.load()
+    A, B = _f(A, np.complex128), _f(B, np.complex128)
+    m1, m0 = A.shape
+    out = np.zeros((m0, m0), np.complex128, order="F")
+    z = None if z_factor is None else np.array([z_factor], np.complex128)
+    L.check(lib.forge_z_matrix_overlap(_p(out), _p(A), _p(B), m1, m0, _p(z)),
+            "forge_z_matrix_overlap")
+    return out
+
+
+def c_matrix_mul(A, B):
+    """c_matrix_mul(A, B, C) (LapackRoutines.f90:224-236): complex(4) A B."""
+    lib = L.load()
+    A, B = _f(A, np.complex64), _f(B, np.complex64)
+    m, k = A.shape
+    n = B.shape[1]
+    out = np.zeros((m, n), np.complex64, order="F")
+    L.check(lib.forge_c_matrix_mul(_p(out), _p(A), _p(B), m, k, n), "forge_c_matrix_mul")
+    return out

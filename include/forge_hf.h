@@ -25,6 +25,7 @@ extern "C" {
 #endif
 
 typedef struct { double re, im; } forge_z;
+typedef struct { float re, im; } forge_c; /* complex(4) */
 
 /* Runtime form of the Setup.f90 parameters (Setup.f90:6-74). */
 typedef struct {
@@ -91,6 +92,24 @@ int forge_hamiltonian_hf(forge_ctx* ctx, forge_z* zH, const double* Potential,
  * and eigenvalues(1:iu-il+1) the eigenvalues.  *info as zheevr's. */
 int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_t il, int64_t iu,
                    int32_t* info);
+
+/* Products of the reference's LAPACK shim on the matrix-core GEMMs (host
+ * arrays in and out; no context).
+ * z_vector_mul(A, C, z_factor), LapackRoutines.f90:168-183: C (n x n) +=
+ *   z A A^H with A n x k; the reference passes k = 1 (one rank-1 zgemm per
+ *   band), k > 1 is the same update for k bands at once.  z_factor may be
+ *   null (1).
+ * z_matrix_mul(A, B, C), LapackRoutines.f90:208-220: C (m x n) = A (m x k) B.
+ * z_matrix_overlap(A, B, C, z_factor), LapackRoutines.f90:119-139:
+ *   C (m0 x m0) = z A^H B with A, B m1 x m0; z_factor may be null (1).
+ * c_matrix_mul(A, B, C), LapackRoutines.f90:224-236: complex(4) C = A B. */
+int forge_z_vector_mul(forge_z* C, const forge_z* A, int64_t n, int64_t k, const forge_z* z_factor);
+int forge_z_matrix_mul(forge_z* C, const forge_z* A, const forge_z* B, int64_t m, int64_t k,
+                       int64_t n);
+int forge_z_matrix_overlap(forge_z* C, const forge_z* A, const forge_z* B, int64_t m1, int64_t m0,
+                           const forge_z* z_factor);
+int forge_c_matrix_mul(forge_c* C, const forge_c* A, const forge_c* B, int64_t m, int64_t k,
+                       int64_t n);
 
 /* Solve(zFockBulk, zFock, Potential, alpha, Bands, zEigenvectors, ...),
  * HartreeFock.f90:402-483: every k of the grid is assembled, diagonalised and
