@@ -1246,10 +1246,6 @@ extern "C" int forge_scf_set_valley(forge_ctx* c, const int64_t* RotateLayers) {
     c->valley = false;
     return 0;
   }
-  if (c->c3) {
-    set_error("forge_scf_set_valley: the valley projection with the C3-reduced k-set is not built");
-    return -2;
-  }
   CK(valley_configure(c, RotateLayers, "forge_scf_set_valley"));
   c->valley = true;
   return 0;
@@ -1314,10 +1310,6 @@ extern "C" int forge_scf_set_c3(forge_ctx* c, const int64_t* RotateLayers, const
   if (!RotateLayers && !nC3pairs) {
     c->c3 = false;
     return 0;
-  }
-  if (c->valley) {
-    set_error("forge_scf_set_c3: the C3-reduced k-set with the valley projection is not built");
-    return -2;
   }
   CK(c3_configure(c, RotateLayers, nC3pairs, "forge_scf_set_c3"));
   c->c3 = true;
@@ -1467,11 +1459,23 @@ static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation
   if (c->valley && in_loop) {
     // Main.f90:469-494: GetFock of the states, ValleyTransform, GetFock of
     // the transformed states (same energies, so the same sorted columns),
-    // rho_new = (bulk + (win + win_valley) / 2) / Nk
-    if (c->c3) { set_error("window_step: the valley projection with the C3-reduced k-set is not built"); return -2; }
+    // rho_new = (bulk + (win + win_valley) / 2) / Nk; with the C3-reduced
+    // k-set both window densities are GetFock_C3's (Main.f90:473-476,
+    // 487-490), on the window states of every momentum (the partners' are
+    // the permuted copies Solve_C3 wrote)
     const i64 n = c->n, S = (i64)occ.idx.size();
     std::vector<double> half(occ.wph);
     for (double& x : half) x *= 0.5;
+    if (c->c3) {
+      std::vector<i64> cols(S), kof(S);
+      for (i64 s = 0; s < S; ++s) {
+        cols[s] = occ.idx[s];
+        kof[s] = occ.idx[s] / c->nb;
+      }
+      CK(c3_window(c, S, cols, kof, half.data(), c->Cwin, 1.0));
+      CK(valley_transform_cwin(c));
+      return c3_window(c, S, cols, kof, half.data(), c->Cwin, 1.0 / (double)c->Nk);
+    }
     if (S > 0) {
       FORGE_HIP(hipMemcpyAsync(c->gidx, occ.idx.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->s));
       FORGE_HIP(hipMemcpyAsync(c->wph, half.data(), sizeof(double) * half.size(), hipMemcpyHostToDevice, c->s));

@@ -12,6 +12,7 @@ module forge_geometry
   implicit none
   private
   public :: NumberNeighborCells, OrderNeighborCells, SampleBZ, build_cell, lattice_relaxation
+  public :: rotation_matrix
 
 contains
 
@@ -127,6 +128,14 @@ contains
       end do
     end do
   end subroutine add_sublattice
+
+  ! RotMatrix of Main.f90:133: the rotation by half the twist angle.
+  function rotation_matrix() result(R)
+    real(dp) :: R(2, 2), aM, h
+    aM = 3.0_dp * ntheta * ntheta + 3.0_dp * ntheta + 1.0_dp
+    h = 0.5_dp * acos(1.0_dp - 1.0_dp / (2.0_dp * aM))
+    R = reshape([cos(h), -sin(h), sin(h), cos(h)], [2, 2])
+  end function rotation_matrix
 
   ! Coordinates of the moire cell (ndim x 3, rotated frame), lattice vectors
   ! t1, t2, t3 = t2 - t1 and reciprocal vectors g1, g12 of the rotated frame.

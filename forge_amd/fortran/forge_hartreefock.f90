@@ -26,6 +26,7 @@
 !   Solve_C3                 HartreeFock.f90:131-398
 !   GetFock_C3               HartreeFock.f90:564-881
 !   SortEnergies_2Spins      HartreeFock.f90:929-1004  (host)
+!   ValleyTransform          TightBinding.f90:170-211
 module forge_hartreefock
   use iso_c_binding
   use forge_c
@@ -37,7 +38,7 @@ module forge_hartreefock
   public :: SortEnergies_1Spin, GetKineticEnergy, GetHartreeHubbardEnergy, GetFockEnergy
   public :: OptimalStep, TightBindingHamiltonian, zdiagRed
   public :: vector_mul, z_vector_mul, matrix_mul, z_matrix_mul, c_matrix_mul
-  public :: matrix_overlap, z_matrix_overlap
+  public :: matrix_overlap, z_matrix_overlap, ValleyTransform
 
   ! generics of the reference's LAPACK shim (LapackRoutines.f90:21-37)
   interface vector_mul
@@ -187,6 +188,33 @@ contains
     call forge_check(rc, 'zdiagRed')
     if (info /= 0) write(*, *) 'zheev failed', info
   end subroutine zdiagRed
+
+  ! Every k's window eigenvectors <- C sign(C^H V C) with V the valley
+  ! operator at k.  The engine builds V from its own geometry, the stored
+  ! cells and RotateLayers (forge_setup); RotMatrix is checked against the
+  ! rotation the engine uses (by half the twist angle of the cell,
+  ! Main.f90:133).
+  subroutine ValleyTransform(zEigenvectors, Coords, MomentaValues, nMomentaComponents, numb_, ndim_, &
+                             numk_, Nk_, numNeighborCells, nUnitCell_1, nUnitCell_2, tn, RotMatrix)
+    integer(dp), intent(in) :: ndim_, numk_, Nk_, numb_, numNeighborCells
+    integer(dp), intent(in) :: nMomentaComponents(Nk_, 2), nUnitCell_1(numNeighborCells)
+    integer(dp), intent(in) :: nUnitCell_2(numNeighborCells)
+    real(dp), intent(in) :: MomentaValues(numk_, numk_, 2), Coords(ndim_, 3), tn(2, 3), RotMatrix(2, 2)
+    complex(dp), intent(inout) :: zEigenvectors(ndim_, numb_, Nk_)
+    integer(c_int64_t), allocatable, target :: rot(:)
+    real(dp) :: aM, h
+    call need(ndim_, numNeighborCells, 'ValleyTransform')
+    call same_model('ValleyTransform', Coords=Coords, tn=tn)
+    aM = real(ndim_ / (2 * nlayers), dp)
+    h = 0.5_dp * acos(1.0_dp - 1.0_dp / (2.0_dp * aM))
+    if (maxval(abs(RotMatrix - reshape([cos(h), -sin(h), sin(h), cos(h)], [2, 2]))) > 1e-12_dp) then
+      write(*, *) 'forge: ValleyTransform: RotMatrix differs from the rotation of the engine''s cell'
+      error stop 2
+    end if
+    allocate(rot(nlayers))
+    rot = RotateLayers
+    call forge_check(forge_valley_transform(ctx, zEigenvectors, rot), 'ValleyTransform')
+  end subroutine ValleyTransform
 
   ! C = C + z_factor A A^H (the reference's rank-1 update with one column of
   ! A; every column of A here)

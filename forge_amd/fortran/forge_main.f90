@@ -27,7 +27,8 @@
 ! Main.f90:156-160; relaxtable= overrides forge_amd/data/relaxation_fourier.txt)
 ! T C2 C2T (nenforceT / nenforceC2 /
 ! nenforceC2T, Setup.f90:28-30: 1 = symmetrise each new density as
-! Main.f90:514-534 does).
+! Main.f90:514-534 does) V (nenforceValley, Setup.f90:31: 1 = the valley
+! projection of Main.f90:469-494, also together with c3=1; numS = 1).
 program forge_main
   use iso_c_binding
   use forge_c
@@ -40,8 +41,8 @@ program forge_main
   character(len=512) :: start = 'first', start2 = '', outfile = 'forge_traj.txt', refout = ''
   character(len=512) :: relaxtable = ''
   integer(c_int32_t) :: occ = 0, mix = 0, prec = 0
-  integer(dp) :: nenforceC3 = 0
-  integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:)
+  integer(dp) :: nenforceC3 = 0, nenforceValley = 0
+  integer(c_int64_t), allocatable, target :: nC3pairs(:), rotC3(:), rotV(:)
   integer(c_int64_t), allocatable :: nC2pairs(:)
   integer(c_int32_t) :: symflags = 0
   real(dp) :: kT = 0.001_dp, beta = 0.3_dp, etol = 0.0_dp, switch_de = 0.0_dp
@@ -54,7 +55,7 @@ program forge_main
   complex(dp), allocatable, target :: zFock(:, :, :, :)
   complex(dp), allocatable :: zFockIn(:, :, :, :), zFockBulk(:, :, :, :)
   complex(dp), allocatable :: zEigenvectors(:, :, :, :), zSorted(:, :, :)
-  real(dp) :: t1(2), t2(2), t3(2), g1(2), g12(2), tn(2, 3)
+  real(dp) :: t1(2), t2(2), t3(2), g1(2), g12(2), tn(2, 3), RotMatrix(2, 2)
   real(dp) :: KineticEnergy(2), FockEnergy(2), HartreeEnergy, HubbardEnergy, TotalEnergy
   real(dp) :: KineticEnergyIn(2), FockEnergyIn(2), HartreeEnergyIn, HubbardEnergyIn
   real(dp) :: TotalEnergyIn, FermiEnergy, DegFactor, Step, DensityConvergence
@@ -79,6 +80,7 @@ program forge_main
   if (nrelax == 1 .and. nlayers == 2) call lattice_relaxation(Coords, g1, g12, trim(relaxtable))
   call SampleBZ(nMomentaComponents, MomentaValues, numk, g1, g12)
   tn = reshape([t1, t2, t3], [2, 3])
+  RotMatrix = rotation_matrix()
   ! C2_RelatedPoints (Main.f90:162)
   allocate(nC2pairs(n))
   nC2pairs = 0
@@ -116,6 +118,10 @@ contains
       write(*, *) 'forge_main: numS = 2 runs in mode=reference with occ=0 mix=0'
       error stop 1
     end if
+    if (numS == 2 .and. nenforceValley == 1) then
+      write(*, *) 'forge_main: the valley projection runs with numS = 1'
+      error stop 1
+    end if
     if (trim(start) /= 'first') call read_fock(trim(start), 1_dp)
     if (numS == 2 .and. trim(start) /= 'first') then
       if (len_trim(start2) == 0) then
@@ -137,6 +143,11 @@ contains
         call forge_check(forge_scf_set_c3(ctx, c_loc(rotC3), c_loc(nC3pairs)), 'forge_scf_set_c3')
       if (symflags /= 0) &
         call forge_check(forge_scf_set_symmetry(ctx, symflags, nC2pairs), 'forge_scf_set_symmetry')
+      if (nenforceValley == 1) then
+        allocate(rotV(nlayers))
+        rotV = RotateLayers
+        call forge_check(forge_scf_set_valley(ctx, c_loc(rotV)), 'forge_scf_set_valley')
+      end if
       if (trim(start) == 'first') then
         call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
       else
@@ -209,7 +220,20 @@ contains
       call fermi_level(SortedEnergies(:, 1), numS)
       zFock(:, :, :, 1) = (0.0_dp, 0.0_dp)
       call getfock_k(1_dp, nOcc, nPart)
-      zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      if (nenforceValley == 1) then
+        ! Main.f90:469-494: GetFock of the states and of their valley
+        ! transform (same energies, so the same sort), averaged
+        call ValleyTransform(zEigenvectors(:, :, :, 1), Coords, MomentaValues, nMomentaComponents, &
+                             numb, n, numk, Nk, nnc, nUnitCell_1, nUnitCell_2, tn, RotMatrix)
+        call SortEnergies_1Spin(n, numk, numb, Nk, nMomentaComponents, zSorted(:, :, 1), &
+                                SortedEnergies(:, 1), nSortedMomenta(:, :, 1), &
+                                zEigenvectors(:, :, :, 1), Bands(:, :, 1))
+        call getfock_k(1_dp, nOcc, nPart)
+        zFock(:, :, :, 1) = 0.5_dp * zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)
+        zFock(:, :, :, 1) = zFock(:, :, :, 1) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      else
+        zFock(:, :, :, 1) = (zFock(:, :, :, 1) + zFockBulk(:, :, :, 1)) / cmplx(real(Nk, dp), 0.0_dp, dp)
+      end if
       call symmetrize_spin(1_dp)
 
       HartreeEnergyIn = HartreeEnergy
@@ -612,6 +636,7 @@ contains
       case ('C2'); read(val, *) i; if (i == 1) symflags = ior(symflags, 2_c_int32_t)
       case ('C2T'); read(val, *) i; if (i == 1) symflags = ior(symflags, 4_c_int32_t)
       case ('c3'); read(val, *) nenforceC3
+      case ('V'); read(val, *) nenforceValley
       case default
         write(*, *) 'forge_main: unknown key ', trim(key)
         error stop 1

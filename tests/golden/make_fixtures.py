@@ -240,18 +240,22 @@ COMBO_CASES = [
     ("t1k3s_s2c3", 1, 3, 1.0, 8, "split", dict(numS=2, nenforceC3=1)),
     ("t1k3_V", 1, 3, 1.0, 6, "reference", dict(numS=1, nenforceValley=1)),
     ("t1k3s_V", 1, 3, 1.0, 8, "split", dict(numS=1, nenforceValley=1)),
+    ("t1k3_c3V", 1, 3, 1.0, 6, "reference", dict(numS=1, nenforceC3=1, nenforceValley=1)),
+    ("t1k3s_c3V", 1, 3, 1.0, 8, "split", dict(numS=1, nenforceC3=1, nenforceValley=1)),
 ]
 
 
-def combo_fixtures():
+def combo_fixtures(only=None):
     """Trajectories of the reference program built with several of the
     settings at once (C3 with T; two spins with C2 or with C3), and with the
-    valley projection: tests/golden/<case>.traj.txt.  The header lists every
-    setting."""
+    valley projection, alone and with C3: tests/golden/<case>.traj.txt.  The
+    header lists every setting.  only: the case names to (re)write."""
     keys = ("numS", "nenforceC3", "nenforceT", "nenforceC2", "nenforceC2T", "nenforceValley")
     env_of = dict(numS="NUMS", nenforceC3="C3", nenforceT="ENFT", nenforceC2="ENFC2",
                   nenforceC2T="ENFC2T", nenforceValley="ENFV")
     for (name, ntheta, numk, nfill, itmax, start, st) in COMBO_CASES:
+        if only and name not in only:
+            continue
         full = {k: st.get(k, 1 if k == "numS" else 0) for k in keys}
         e = dict(os.environ, **{env_of[k]: str(v) for k, v in full.items()})
         subprocess.run([REF_BUILD, name, str(ntheta), str(numk), f"{nfill:.1f}", str(itmax),
@@ -338,6 +342,8 @@ def bands_fixture():
 
 if __name__ == "__main__":
     args = sys.argv[1:]
+    combo_only = [a.split("=", 1)[1] for a in args if a.startswith("combo=")]
+    args = ["combo" if a.startswith("combo=") else a for a in args]
     cases = [a for a in args if a not in ("ckpt", "bands", "c3", "spin", "sym", "combo", "relax")]
     if not args or cases:
         main(cases)
@@ -352,6 +358,6 @@ if __name__ == "__main__":
     if not args or "sym" in args:
         sym_fixtures()
     if not args or "combo" in args:
-        combo_fixtures()
+        combo_fixtures(combo_only or None)
     if not args or "relax" in args:
         relax_fixtures()

@@ -11,7 +11,8 @@ the reference's own Energy/Convergence files (refout=); c3=1 (nenforceC3)
 reproduces the reference built with Solve_C3 / GetFock_C3; numS=2 the
 reference built with two spins (first diagonalisation and spin-split
 checkpoints); T= C2= C2T= the reference built with nenforceT / nenforceC2 /
-nenforceC2T.
+nenforceC2T; V= the reference built with nenforceValley, alone and with
+nenforceC3.
 """
 import os
 import subprocess
@@ -374,3 +375,36 @@ def test_fortran_relaxed_geometry_is_bitwise_reference(ntheta, tmp_path):
     np.testing.assert_array_equal(g["Coords"].reshape(-1, 3), f["CoordsRelaxed"])
     par = [l.split()[2] for l in open(out) if l.startswith("# parameters ")][0]
     assert "-relax1-" in par
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,mode", [("t1k3_V", "reference"), ("t1k3s_V", "device"),
+                                       ("t1k3_c3V", "reference"), ("t1k3s_c3V", "reference"),
+                                       ("t1k3s_c3V", "device")])
+def test_fortran_host_valley_trajectory_matches_reference(case, mode, tmp_path):
+    """V=1 (nenforceValley): in mode=reference the host calls GetFock,
+    ValleyTransform, SortEnergies_1Spin and GetFock again as Main.f90:469-494
+    does (GetFock_C3 with c3=1); in mode=device the loop does the same on the
+    device.  The valley-projected SCF amplifies rounding (test_oracle), so the
+    first three iterations are held to 1e-8 and the rest to 1e-5."""
+    from forge_amd.checkpoint import write_checkpoint
+    from forge_amd.geometry import build_geometry
+    from forge_amd.seed import seeded_rho
+    print("fortran host valley", case, mode, flush=True)
+    m = traj_meta(case)
+    args = [FORTRAN_EXE, f"mode={mode}", f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}",
+            f"numk={m['numk']}", f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}",
+            "V=1", f"c3={m['nenforceC3']}", f"itmax={m['itmax']}", f"out={tmp_path / 'traj.txt'}"]
+    if m["start"] == "split":
+        g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
+        write_checkpoint(str(tmp_path / "s.ckpt"), seeded_rho(g.ndim, g.nnc, float(m["nfilling"])))
+        args.append(f"start={tmp_path / 's.ckpt'}")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    E, Cv = _traj(tmp_path / "traj.txt")
+    Eref, Cref = load_traj(case)
+    assert E.shape == Eref.shape
+    np.testing.assert_allclose(E[:3], Eref[:3], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(E, Eref, rtol=0, atol=1e-5)
+    np.testing.assert_allclose(Cv[:, 2], Cref[:, 2], rtol=0, atol=1e-3)

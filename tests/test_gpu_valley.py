@@ -43,12 +43,16 @@ def test_valley_phase_and_transform_match_oracle():
     e.close()
 
 
-@pytest.mark.parametrize("case", ["t1k3_V", "t1k3s_V"])
+@pytest.mark.parametrize("case", ["t1k3_V", "t1k3s_V", "t1k3_c3V", "t1k3s_c3V"])
 def test_scf_valley_trajectory_matches_reference(case):
+    """Alone and with the C3-reduced k-set (the reference built with
+    nenforceValley = 1 and nenforceC3 = 1: GetFock_C3 in both passes)."""
     say("scf valley", case)
     m = traj_meta(case)
     e = Engine(Setup(ntheta=1, numk=3, nfilling=1.0, xi_nm=40.0))
     e.set_options()
+    if m["nenforceC3"] == "1":
+        e.set_c3(True)
     e.set_valley(True)
     start = seeded_rho(e.n, e.nnc, 1.0) if m["start"] == "split" else None
     e.scf_init(start)

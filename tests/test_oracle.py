@@ -324,13 +324,15 @@ def test_combined_settings_match_reference(case):
     np.testing.assert_allclose(traj[:, 8], Cv[:, 3], rtol=0, atol=2e-9)
 
 
-VALLEY_CASES = ["t1k3_V", "t1k3s_V"]
+VALLEY_CASES = ["t1k3_V", "t1k3s_V", "t1k3_c3V", "t1k3s_c3V"]
 
 
 @pytest.mark.parametrize("case", VALLEY_CASES)
 def test_valley_projection_matches_reference(case):
     """nenforceValley = 1 (ValleyTransform, TightBinding.f90:170-211, and the
-    doubled GetFock of Main.f90:469-494) against the reference built with it.
+    doubled GetFock of Main.f90:469-494) against the reference built with it,
+    alone and with the C3-reduced k-set (GetFock_C3 in both passes,
+    Main.f90:473-476, 487-490).
     The oracle runs on one thread, so its summation order is fixed.  The first
     five iterations agree to the printed 1e-9.  Later iterations are held to
     1e-6: this SCF oscillates and amplifies rounding (the next test measures
@@ -339,7 +341,8 @@ def test_valley_projection_matches_reference(case):
     m = traj_meta(case)
     s = System(int(m["ntheta"]), int(m["numk"]), float(m["nfilling"]))
     start = seeded_rho(s.n, s.nnc, float(m["nfilling"])) if m["start"] == "split" else None
-    its, traj, _, _ = s.scf(int(m["itmax"]), rho_start=start, sym=8, nthreads=1)
+    its, traj, _, _ = s.scf(int(m["itmax"]), rho_start=start, sym=8, nthreads=1,
+                            c3=m["nenforceC3"] == "1")
     E, Cv = load_traj(case)
     assert its == int(m["itmax"]) == len(E)
     np.testing.assert_allclose(traj[:5, 1:6], E[:5, 1:6], rtol=0, atol=2e-9)
