@@ -256,8 +256,15 @@ static int tuned_width(const TridiagWork& t, const double* Z, size_t ldZ, size_t
 }
 
 int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
-                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32, int* width_used) {
+                  size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32, int* width_used,
+                  int fixed_width) {
   int bt = 128;
+  if (fixed_width) {
+    // no tuning (it times with events of its own): the caller's width
+    if (C32) { set_error("backtransform: a fixed width applies to the FP64 products only"); return -2; }
+    if (width_used) *width_used = fixed_width;
+    return apply_q_bt<z2>(fixed_width, t, Z, ldZ, sZ, m, C, ldC, sC, work, s, false);
+  }
   if (!C32) {
     CK_BT(tuned_width<z2>(t, Z, ldZ, sZ, m, C, ldC, sC, work, s, &bt));
     if (width_used) *width_used = bt;

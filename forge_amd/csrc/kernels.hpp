@@ -85,11 +85,12 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
 // backtr.hip: C(n x m, complex) = Q_house * Z(n x m, real), Q from tridiag.
 // C32 non-null: the products run in complex FP32 in C32 (n x m per matrix,
 // ld n), and C receives the columns normalised in FP64.  The block width of
-// the compact WY products is tuned per shape on first use (*width_used).
+// the compact WY products is tuned per shape on first use (*width_used);
+// fixed_width (64, 128 or 256; FP64 only) skips the tuning.
 size_t backtr_workspace_bytes(i64 n, int K, i64 m);
 int backtransform(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, z2* C,
                   size_t ldC, size_t sC, void* work, hipStream_t s, float2* C32 = nullptr,
-                  int* width_used = nullptr);
+                  int* width_used = nullptr, int fixed_width = 0);
 
 // gemm.hip: matrix-core GEMMs (C = alpha op(A) op(B) + beta C, batched with
 // strides; Op::H = conjugate transpose, plain transpose for real data)

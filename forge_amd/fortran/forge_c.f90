@@ -21,6 +21,7 @@ module forge_c
   public :: forge_mlp_param_count, forge_mlp_loss_grad, forge_mlp_train, forge_mlp_infer
   public :: forge_valley_phase, forge_valley_transform, forge_scf_set_valley
   public :: forge_z_vector_mul, forge_z_matrix_mul, forge_z_matrix_overlap, forge_c_matrix_mul
+  public :: forge_zdiagred_noalloc
 
   ! struct forge_params (Setup.f90:6-74 at run time)
   type, bind(C) :: forge_params_t
@@ -100,6 +101,17 @@ module forge_c
       integer(c_int32_t), intent(out) :: info
       integer(c_int) :: r
     end function forge_zdiagred
+
+    ! forge_zdiagred without device allocation (n <= 512; no reference counterpart)
+    function forge_zdiagred_noalloc(A, n, eigenvalues, il, iu, info) &
+        bind(C, name="forge_zdiagred_noalloc") result(r)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex
+      complex(c_double_complex), intent(inout) :: A(*)
+      integer(c_int64_t), value :: n, il, iu
+      real(c_double), intent(out) :: eigenvalues(*)
+      integer(c_int32_t), intent(out) :: info
+      integer(c_int) :: r
+    end function forge_zdiagred_noalloc
 
     ! products of the LAPACK shim (LapackRoutines.f90:119-236); z_factor may be
     ! c_null_ptr (1)

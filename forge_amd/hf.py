@@ -382,6 +382,19 @@ def zdiagred(A, il, iu):
     return w[:m], a[:, :m], info.value
 
 
+def zdiagred_noalloc(A, il, iu):
+    """zdiagred without device allocation (forge_zdiagred_noalloc, n <= 512)."""
+    lib = L.load()
+    n = A.shape[0]
+    a = np.array(A, dtype=np.complex128, order="F", copy=True)
+    w = np.zeros(n)
+    info = C.c_int32(0)
+    L.check(lib.forge_zdiagred_noalloc(_p(a), n, _p(w), int(il), int(iu), C.byref(info)),
+            "forge_zdiagred_noalloc")
+    m = iu - il + 1
+    return w[:m], a[:, :m], info.value
+
+
 def z_vector_mul(A, C, z_factor=None):
     """z_vector_mul(A, C, z_factor) (LapackRoutines.f90:168-183): C + z A A^H,
     A n x k (the reference passes one column)."""

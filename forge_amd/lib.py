@@ -45,6 +45,7 @@ _SIGS = {
     "forge_tb_hamiltonian": (C.c_int, [vp, vp, vp]),
     "forge_hamiltonian_hf": (C.c_int, [vp, vp, vp, vp, vp]),
     "forge_zdiagred": (C.c_int, [vp, i64, vp, i64, i64, C.POINTER(i32)]),
+    "forge_zdiagred_noalloc": (C.c_int, [vp, i64, vp, i64, i64, C.POINTER(i32)]),
     "forge_z_vector_mul": (C.c_int, [vp, vp, i64, i64, vp]),
     "forge_z_matrix_mul": (C.c_int, [vp, vp, vp, i64, i64, i64]),
     "forge_z_matrix_overlap": (C.c_int, [vp, vp, vp, i64, i64, vp]),

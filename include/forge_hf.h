@@ -111,6 +111,16 @@ int forge_z_matrix_overlap(forge_z* C, const forge_z* A, const forge_z* B, int64
 int forge_c_matrix_mul(forge_c* C, const forge_c* A, const forge_c* B, int64_t m, int64_t k,
                        int64_t n);
 
+/* No counterpart in the reference: forge_zdiagred for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same arguments and results; every
+ * device buffer comes from a preallocated 32 MiB device array per GPU, so
+ * n <= 512 (larger n returns -2).  Runs on the null stream.  Calls on one
+ * GPU share that array: host threads are serialised by a lock; overlapping
+ * calls by other means race. */
+int forge_zdiagred_noalloc(forge_z* A, int64_t n, double* eigenvalues, int64_t il, int64_t iu,
+                           int32_t* info);
+
 /* Solve(zFockBulk, zFock, Potential, alpha, Bands, zEigenvectors, ...),
  * HartreeFock.f90:402-483: every k of the grid is assembled, diagonalised and
  * its bulk density accumulated; zFockBulk (ndim x ndim x nnc) is the
