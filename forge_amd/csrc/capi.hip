@@ -179,10 +179,21 @@ struct forge_ctx {
   ValleyGeom vg{};
   void* valley_ws = nullptr;
   std::vector<double> h_vk;  // [Nk][2]
+  // the device-resident iteration (options.graph 2: eager, 3: one HIP graph
+  // per phase, captured on first use and replayed)
+  double* dband = nullptr;  // nb * Nk window energies, band-major
+  i64* dS = nullptr;        // weighted window states
+  double* docc = nullptr;   // E_F, nOcc, nPart, DegFactor
+  double* dst = nullptr;    // [0..3] energies of the state, [4] E_tot, [5] step
+  double* drow = nullptr;   // trajectory row of the iteration
+  bool dev_iter = false;    // eig_batch: reduction without graph or syncs, deferred D&C
+  bool dev_state = false;   // dst holds the state's energies
+  bool dev_warm[2] = {false, false};  // a host-path iteration of the phase ran
+  hipGraphExec_t it_graph[2] = {nullptr, nullptr};  // mixed phase, FP64
 };
 
 static double total_energy(const forge_params& P, const double* e) {
-  return P.alpha * e[1] * 2.0 + e[0] * 2.0 + P.U * e[3] + P.alphaH * e[2];
+  return scf_total_energy(P.alpha, P.alphaH, P.U, e);
 }
 
 extern "C" const char* forge_last_error(void) { return g_err.c_str(); }
@@ -379,7 +390,10 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
                   c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
-                  c->c3_Psi, c->c3_wph, c->c2_perm, c->valley_ws};
+                  c->c3_Psi, c->c3_wph, c->c2_perm, c->valley_ws, c->dband, c->dS, c->docc,
+                  c->dst, c->drow};
+  for (auto& g : c->it_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   tridiag_release(c->trd);
@@ -466,15 +480,18 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   c->trd.K = K;
   const bool mixed = mixed_phase(c);
   c->trd.S = mixed ? c->S : nullptr;
-  c->trd.use_graph = c->opt.graph;
+  c->trd.use_graph = c->opt.graph == 1;
   c->trd.sS = (size_t)c->ldH * n;
   double hemv_ms = 0, hemv_bytes = 0;
   long hemv_n = 0;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[2], c->s));
-  CK(tridiag(c->trd, c->s, timed ? &hemv_ms : nullptr, timed ? &hemv_n : nullptr,
-             timed ? &hemv_bytes : nullptr));
+  if (c->dev_iter) CK(tridiag_enqueue(c->trd, c->s));
+  else
+    CK(tridiag(c->trd, c->s, timed ? &hemv_ms : nullptr, timed ? &hemv_n : nullptr,
+               timed ? &hemv_bytes : nullptr));
   if (timed) FORGE_HIP(hipEventRecord(c->ev[3], c->s));
-  int fails = dc_solve(c->trd.d, c->trd.e, n, K, mout, c->evals, c->Zq, c->dc_ws, c->s);
+  // device iteration: the failure count stays on the device (read after it)
+  int fails = dc_solve(c->trd.d, c->trd.e, n, K, mout, c->evals, c->Zq, c->dc_ws, c->s, c->dev_iter);
   if (fails < 0) return fails;
   if (timed) FORGE_HIP(hipEventRecord(c->ev[4], c->s));
   // mixed phase: the FP32 copy is free once the reduction is done and holds
@@ -928,6 +945,11 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
       FORGE_HIP(hipEventElapsedTime(&a, c->ev[6], c->ev[7]));
       c->tm[4] += a;
     }
+    if (c->dev_iter) {
+      // the window energies stay on the device for k_occupations
+      CK(band_gather(n, nb, c->nL - 1, K, k0, Nk, c->evals, c->dband, c->s));
+      continue;
+    }
     FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
     FORGE_HIP(hipStreamSynchronize(c->s));
     for (int kb = 0; kb < K; ++kb)
@@ -1349,13 +1371,7 @@ static int energy_pieces(forge_ctx* c, const z2* a, const z2* b, double* pieces)
 }
 
 // energies (Ekin, EF, EH, EU) of (1-l) a + l b from the pieces
-static void energies_at(const double* pc, double l, double* e) {
-  double u = 1.0 - l;
-  e[0] = u * (pc[0] + pc[11]) + l * (pc[1] + pc[12]);
-  e[1] = u * u * pc[2] + 2.0 * u * l * pc[3] + l * l * pc[4];
-  e[2] = 2.0 * (u * u * pc[5] + 2.0 * u * l * pc[6] + l * l * pc[7]);
-  e[3] = u * u * pc[8] + 2.0 * u * l * pc[9] + l * l * pc[10];
-}
+static void energies_at(const double* pc, double l, double* e) { scf_energies_at(pc, l, e); }
 
 extern "C" int forge_energies(forge_ctx* c, const forge_z* zFock, const double* dens, double* out4) {
   if (!c || !zFock || !dens || !out4) { set_error("forge_energies: null argument"); return -2; }
@@ -1401,6 +1417,23 @@ extern "C" int forge_scf_set_options(forge_ctx* c, const forge_scf_options* opt)
     set_error("forge_scf_set_options: precision must be 0 or 1 and switch_de >= 0");
     return -2;
   }
+  if (opt->graph < 0 || opt->graph > 3 || (opt->graph >= 2 && opt->timing)) {
+    set_error("forge_scf_set_options: graph must be 0..3, and 2 or 3 without timing");
+    return -2;
+  }
+  if (opt->graph >= 2 && !c->dband) {
+    CK(dalloc((void**)&c->dband, sizeof(double) * c->nb * c->Nk));
+    CK(dalloc((void**)&c->dS, sizeof(i64)));
+    CK(dalloc((void**)&c->docc, sizeof(double) * 4));
+    CK(dalloc((void**)&c->dst, sizeof(double) * 8));
+    CK(dalloc((void**)&c->drow, sizeof(double) * 12));
+  }
+  // the iteration graphs hold the options as kernel arguments
+  for (auto& g : c->it_graph)
+    if (g) {
+      FORGE_HIP(hipGraphExecDestroy(g));
+      g = nullptr;
+    }
   if (opt->precision == 1 && !c->S) {
     const size_t bytes = sizeof(float2) * (size_t)c->ldH * c->n * c->Kb;
     size_t fr = 0, tot = 0;
@@ -1556,6 +1589,10 @@ static int init_state_of_rho(forge_ctx* c) {
   FORGE_HIP(hipStreamSynchronize(c->s));
   c->it = 0;
   c->scf_ready = true;
+  c->dev_state = false;
+  // every solve starts each phase on the host path, so that runs with and
+  // without the iteration graph take the same arithmetic path
+  c->dev_warm[0] = c->dev_warm[1] = false;
   return 0;
 }
 
@@ -1698,11 +1735,7 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   energies_at(pc, 0.5, eh);
   if (c->opt.mix_mode == 0) {
     // OptimalStep, HartreeFock.f90:1162-1208
-    double c0 = EtotIn, E1 = total_energy(c->P, e1), Eh = total_energy(c->P, eh);
-    double c1 = 4.0 * Eh - E1 - 3.0 * c0, c2 = E1 - c0 - c1;
-    if (c2 > 0 && std::fabs(-c1 / 2.0 / c2 - 0.5) < 0.5) step = -c1 / 2.0 / c2;
-    else if (E1 < c0) step = 1.0;
-    else step = c->P.step_alt;
+    step = scf_oda_step(EtotIn, total_energy(c->P, e1), total_energy(c->P, eh), c->P.step_alt);
   } else {
     step = c->opt.beta;
   }
@@ -1748,6 +1781,146 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   return fails;
 }
 
+// ---------------------------------------------------------------- device-resident iteration
+// options.graph 2 / 3: the iteration of scf_step with the sort, Fermi level,
+// occupations, OptimalStep and convergence measure on the device
+// (scf_device.hip), so that it is one launch sequence; with 3 it is captured
+// into a HIP graph per phase (mixed, FP64) on its first use and replayed.
+// The first iteration of each phase runs on the host path: it tunes the
+// back-transformation's block width and leaves the divide-and-conquer's
+// merge descriptors on the device, which the deferred solve then reuses.
+// Applies to the plain loop (no C3 set, valley projection or symmetry
+// enforcement) with all k-points in one eigensolver batch.
+static bool dev_eligible(const forge_ctx* c) {
+  return !c->c3 && !c->valley && c->sym == 0 && c->Kb == c->Nk && c->nnc == 4 &&
+         c->nb * c->Nk <= OCC_MAX;
+}
+
+static int enqueue_device_iteration(forge_ctx* c) {
+  const i64 n = c->n, nnc = c->nnc, Nk = c->Nk;
+  const size_t nn = (size_t)n * n;
+  std::vector<double> unused;
+  c->dev_iter = true;
+  const int f = solve_device(c, c->rho, c->pot, unused, false);
+  c->dev_iter = false;
+  if (f < 0) return f;
+  OccArgs oa{};
+  oa.N = (int)(c->nb * Nk);
+  oa.Nk = (int)Nk;
+  oa.nb = (int)c->nb;
+  oa.nnc = (int)nnc;
+  oa.numk = (int)c->numk;
+  oa.occ_mode = c->opt.occ_mode;
+  {
+    const double x = (double)(Nk * ((n / 2 - c->nL) + 1)) + (double)Nk / 2.0 * c->P.nfilling;
+    oa.nF = (i64)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5));
+  }
+  oa.energy_tol = c->P.energy_tol;
+  oa.kT = c->opt.kT > 0 ? c->opt.kT : 1e-3;
+  oa.band = c->dband;
+  oa.ph_cell = c->ph_cell;
+  oa.idx = c->gidx;
+  oa.wph = c->wph;
+  oa.S = c->dS;
+  oa.occ = c->docc;
+  CK(device_occupations(oa, c->s));
+  CK(gather_cols_dev(n, c->nb * Nk, c->dS, c->gidx, c->Cwin, c->Psi, c->s));
+  CK(window_density(n, (int)nnc, 0, c->Psi, c->wph, c->rho_new, 1.0 / (double)Nk, c->s, c->dS));
+  CK(site_density(n, c->rho_new, c->db, c->s));
+  CK(energy_pass(nn, (int)nnc, c->rho, c->rho_new, c->V, c->T, c->part, 1024, c->e5, c->s));
+  CK(hartree_matvec(n, c->L, c->da, c->db, c->ya, c->yb, c->s));
+  hipLaunchKernelGGL(k_dots, dim3(1), dim3(256), 0, c->s, n, c->da, c->db, c->ya, c->yb, c->dots);
+  FORGE_LAUNCH();
+  hipLaunchKernelGGL(k_bias_dot, dim3(1), dim3(256), 0, c->s, n, c->da, c->db, c->bias, c->dots + 8);
+  FORGE_LAUNCH();
+  StepArgs sa{};
+  sa.mix_mode = c->opt.mix_mode;
+  sa.beta = c->opt.beta;
+  sa.step_alt = c->P.step_alt;
+  sa.alpha = c->P.alpha;
+  sa.alphaH = c->P.alphaH;
+  sa.U = c->P.U;
+  sa.e5 = c->e5;
+  sa.dots = c->dots;
+  sa.occ = c->docc;
+  sa.st = c->dst;
+  sa.row = c->drow;
+  CK(scf_step_update(sa, c->s));
+  CK(density_convergence(n, c->da, c->db, c->dst, c->drow, c->s));
+  CK(mix_dev(nnc * nn, c->rho, c->rho_new, c->dst, c->s));
+  CK(potential_dev(n, c->ya, c->yb, c->da, c->db, c->dst, c->P.alphaH, c->P.U, c->pot, c->s));
+  return 0;
+}
+
+static int scf_step_dev(forge_ctx* c, double* traj_row, bool* stop) {
+  const bool mixed = c->opt.precision == 1 && !c->mixed_done;
+  const int ph = mixed ? 0 : 1;
+  if (!c->dev_warm[ph]) {
+    const int f = scf_step(c, traj_row, false, false, stop);
+    if (f >= 0) c->dev_warm[ph] = true;
+    c->dev_state = false;
+    return f;
+  }
+  *stop = false;
+  if (mixed) c->tm[12] += 1;
+  c->trd.S = mixed ? c->S : nullptr;
+  c->trd.sS = (size_t)c->ldH * c->n;
+  c->trd.K = c->Kb;
+  if (!c->dev_state) {
+    double st[8] = {c->e[0], c->e[1], c->e[2], c->e[3], c->Etot, 0, 0, 0};
+    FORGE_HIP(hipMemcpyAsync(c->dst, st, sizeof(st), hipMemcpyHostToDevice, c->s));
+    FORGE_HIP(hipStreamSynchronize(c->s));
+    c->dev_state = true;
+  }
+  if (c->opt.graph == 3) {
+    if (!c->it_graph[ph]) {
+      hipGraph_t g = nullptr;
+      FORGE_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed));
+      const int rc = enqueue_device_iteration(c);
+      const hipError_t ce = hipStreamEndCapture(c->s, &g);
+      if (rc != 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ce != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        return -1;
+      }
+      const hipError_t ie = hipGraphInstantiate(&c->it_graph[ph], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        c->it_graph[ph] = nullptr;
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        return -1;
+      }
+    }
+    FORGE_HIP(hipGraphLaunch(c->it_graph[ph], c->s));
+  } else {
+    CK(enqueue_device_iteration(c));
+  }
+  double row[12], st[8];
+  int fails = 0;
+  FORGE_HIP(hipMemcpyAsync(row, c->drow, sizeof(row), hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(st, c->dst, sizeof(st), hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(&fails, dc_fail_ptr(c->dc_ws, c->n, c->Kb), sizeof(int),
+                           hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipStreamSynchronize(c->s));
+  for (int q = 0; q < 4; ++q) c->e[q] = st[q];
+  c->Etot = st[4];
+  ++c->it;
+  row[0] = (double)c->it;
+  if (traj_row)
+    for (int q = 0; q < 12; ++q) traj_row[q] = row[q];
+  const double dE = std::fabs(row[8]);
+  if (mixed) {
+    const double sw = c->opt.switch_de > 0 ? c->opt.switch_de : 1e-4;
+    if (dE < sw || (c->opt.etol > 0 && dE < c->opt.etol)) c->mixed_done = true;
+    return fails;
+  }
+  if (c->opt.etol > 0 && dE < c->opt.etol) *stop = true;
+  return fails;
+}
+
 extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64_t* iters_done) {
   if (!c || max_iter < 0) { set_error("forge_scf_run: invalid argument"); return -2; }
   if (!c->scf_ready) { set_error("forge_scf_run: call forge_scf_init first"); return -5; }
@@ -1756,9 +1929,16 @@ extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64
   int fails = 0;
   i64 done = 0;
   FORGE_HIP(hipEventRecord(c->ev[8], c->s));
+  const bool dev = c->opt.graph >= 2;
+  if (dev && !dev_eligible(c)) {
+    set_error("forge_scf_run: options.graph 2 and 3 apply to the plain loop (no C3 set, valley "
+              "projection or symmetry enforcement) with every k-point in one eigensolver batch");
+    return -2;
+  }
   for (i64 itn = 0; itn < max_iter; ++itn) {
     bool stop = false;
-    int f = scf_step(c, traj ? traj + 12 * itn : nullptr, timed, false, &stop);
+    int f = dev ? scf_step_dev(c, traj ? traj + 12 * itn : nullptr, &stop)
+                : scf_step(c, traj ? traj + 12 * itn : nullptr, timed, false, &stop);
     if (f < 0) return f;
     fails += f;
     ++done;

@@ -652,8 +652,18 @@ __global__ void k_dc_copyout(i64 n, i64 mout, const double* Qa, const double* Da
   if (q < mout) Do[(size_t)k * n + q] = Da[(size_t)k * n + q];
 }
 
+__global__ void k_dc_fill(size_t count, double* __restrict__ p, int* __restrict__ flag) {
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < count; q += (size_t)gridDim.x * 256)
+    p[q] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+}
+
+const int* dc_fail_ptr(void* work, i64 n, int K) {
+  return (const int*)((char*)work + dc_layout(n, K).total + 200 * 1024);
+}
+
 int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* evals, double* Qout,
-             void* work, hipStream_t s) {
+             void* work, hipStream_t s, bool deferred) {
   DCLayout L = dc_layout(n, K);
   char* b = (char*)work;
   DCPtr P;
@@ -669,8 +679,14 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
   P.mrot = (int*)(b + L.mrot); P.mrho = (double*)(b + L.mrho);
   MergeDesc* dmd = (MergeDesc*)(b + L.total);
   int* dfail = (int*)(b + L.total + 200 * 1024);
-  FORGE_HIP(hipMemsetAsync(dfail, 0, sizeof(int), s));
-  FORGE_HIP(hipMemsetAsync(P.Qa, 0, sizeof(double) * n * n * K, s));
+  if (deferred) {
+    // kernels, not memset nodes, inside a captured graph
+    hipLaunchKernelGGL(k_dc_fill, dim3(2048), dim3(256), 0, s, (size_t)n * n * K, P.Qa, dfail);
+    FORGE_LAUNCH();
+  } else {
+    FORGE_HIP(hipMemsetAsync(dfail, 0, sizeof(int), s));
+    FORGE_HIP(hipMemsetAsync(P.Qa, 0, sizeof(double) * n * n * K, s));
+  }
   // leaves
   const int nleaf = (int)((n + LEAF - 1) / LEAF);
   hipLaunchKernelGGL(k_dc_leaf, dim3(nleaf, K), dim3(64), 0, s, n, d, e, P.Qa, P.Da, dfail);
@@ -709,7 +725,11 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
   }
   if (!all.empty()) {
     if (all.size() * sizeof(MergeDesc) > 200 * 1024) { set_error("dc: too many merges"); return -2; }
-    FORGE_HIP(hipMemcpyAsync(dmd, all.data(), all.size() * sizeof(MergeDesc), hipMemcpyHostToDevice, s));
+    // deferred calls reuse the descriptors a non-deferred call on the same
+    // workspace and shape uploaded (they depend on n and mout only): a copy
+    // from this host vector could not be replayed from a graph
+    if (!deferred)
+      FORGE_HIP(hipMemcpyAsync(dmd, all.data(), all.size() * sizeof(MergeDesc), hipMemcpyHostToDevice, s));
   }
   int maxm = 0;
   for (auto& M : all) maxm = std::max(maxm, M.n1 + M.n2);
@@ -808,6 +828,7 @@ int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* e
                        P.Da, Qout, evals);
     FORGE_LAUNCH();
   }
+  if (deferred) return 0;
   int hfail = 0;
   FORGE_HIP(hipMemcpyAsync(&hfail, dfail, sizeof(int), hipMemcpyDeviceToHost, s));
   FORGE_HIP(hipStreamSynchronize(s));

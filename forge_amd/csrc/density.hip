@@ -40,11 +40,14 @@ int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int
 // rho_out_c(i,j) = scale * (bulk_c(i,j) + sum_s wph[s][c] conj(Psi(i,s)) Psi(j,s)).
 // Psi: n x S (ld n); wph: [S][nnc][2] complex weights (occupation x cell phase).
 // 32x32 output tiles, states staged 16 at a time.  bulk and out may alias.
+// Sdev non-null: the state count is read from device memory (it is computed
+// on the device in the graph-captured loop).
 template <int NNC>
-__global__ void __launch_bounds__(256) k_window_density(i64 n, i64 S,
+__global__ void __launch_bounds__(256) k_window_density(i64 n, i64 S, const i64* Sdev,
                                                         const z2* __restrict__ Psi,
                                                         const double* __restrict__ wph,
                                                         const z2* bulk, z2* out, double scale) {
+  if (Sdev) S = *Sdev;
   const size_t nn = (size_t)n * n;
   const i64 I = blockIdx.y, J = blockIdx.x;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -96,10 +99,11 @@ __global__ void __launch_bounds__(256) k_window_density(i64 n, i64 S,
 }
 
 int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
-                   double scale, hipStream_t s) {
+                   double scale, hipStream_t s, const i64* Sdev) {
   dim3 grid(cdiv(n, 32), cdiv(n, 32));
   if (nnc != 4) { set_error("window_density: 4 stored cells (numI = 1) expected"); return -2; }
-  hipLaunchKernelGGL(k_window_density<4>, grid, dim3(256), 0, s, n, S, Psi, wph, rho, rho, scale);
+  hipLaunchKernelGGL(k_window_density<4>, grid, dim3(256), 0, s, n, S, Sdev, Psi, wph, rho, rho,
+                     scale);
   FORGE_LAUNCH();
   return 0;
 }

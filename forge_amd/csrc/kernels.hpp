@@ -18,6 +18,31 @@ struct DeviceGeom {
   CellShifts cells;
 };
 
+// Energy model of OptimalStep (HartreeFock.f90:1148-1227) from the pieces of
+// one energy pass over a = rho_in and b = rho_new: pc[0..4] = K_a, K_b, F_aa,
+// F_ab, F_bb; [5..7] = H_aa, H_ab, H_bb (d.L.d); [8..10] = U_aa, U_ab, U_bb;
+// [11..12] bias terms of a, b.  Energies (Ekin, EF, EH, EU) of (1-l) a + l b
+// -- the energy is quadratic in rho, so these are exact for every l -- and
+// the total energy (Main.f90:312,556), shared by the host and device loops.
+__host__ __device__ inline void scf_energies_at(const double* pc, double l, double* e) {
+  const double u = 1.0 - l;
+  e[0] = u * (pc[0] + pc[11]) + l * (pc[1] + pc[12]);
+  e[1] = u * u * pc[2] + 2.0 * u * l * pc[3] + l * l * pc[4];
+  e[2] = 2.0 * (u * u * pc[5] + 2.0 * u * l * pc[6] + l * l * pc[7]);
+  e[3] = u * u * pc[8] + 2.0 * u * l * pc[9] + l * l * pc[10];
+}
+__host__ __device__ inline double scf_total_energy(double alpha, double alphaH, double U,
+                                                   const double* e) {
+  return alpha * e[1] * 2.0 + e[0] * 2.0 + U * e[3] + alphaH * e[2];
+}
+// OptimalStep's choice (HartreeFock.f90:1162-1208) from E(0) = c0, E(1), E(1/2)
+__host__ __device__ inline double scf_oda_step(double c0, double E1, double Eh, double step_alt) {
+  const double c1 = 4.0 * Eh - E1 - 3.0 * c0, c2 = E1 - c0 - c1;
+  if (c2 > 0 && fabs(-c1 / 2.0 / c2 - 0.5) < 0.5) return -c1 / 2.0 / c2;
+  if (E1 < c0) return 1.0;
+  return step_alt;
+}
+
 // operators.hip
 int pair_tables(const DeviceGeom& g, const PairParams& pp, double* V, double* T, hipStream_t s);
 int longrange(const DeviceGeom& g, const PairParams& pp, int numC, double* L, hipStream_t s);
@@ -74,13 +99,21 @@ size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes);
+// the reduction's launches alone, for an enclosing stream capture (no host
+// synchronisation, no graph of its own)
+int tridiag_enqueue(TridiagWork& w, hipStream_t s);
 void tridiag_release(TridiagWork& w);
 
 // dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
 struct DCWork;
 size_t dc_workspace_bytes(i64 n, int K, i64 mout);
+// deferred: no host synchronisation (graph capture): workspace clears by
+// kernel, the failure count stays on the device (dc_fail_ptr) instead of
+// being returned, and the merge descriptors are the ones a non-deferred call
+// with the same workspace, n, K and mout left there
 int dc_solve(const double* d, const double* e, i64 n, int K, i64 mout, double* evals,
-             double* Q, void* work, hipStream_t s);
+             double* Q, void* work, hipStream_t s, bool deferred = false);
+const int* dc_fail_ptr(void* work, i64 n, int K);
 
 // backtr.hip: C(n x m, complex) = Q_house * Z(n x m, real), Q from tridiag.
 // C32 non-null: the products run in complex FP32 in C32 (n x m per matrix,
@@ -115,7 +148,44 @@ int herk_cells(i64 n, i64 k, const z2* A, i64 lda, int nnc, const double* ph, z2
 int bulk_density(i64 n, int K, const z2* C, size_t ldC, size_t sC, i64 nocc, int nnc,
                  const double* phase_cells, z2* rho, bool accumulate, hipStream_t s);
 int window_density(i64 n, int nnc, i64 S, const z2* Psi, const double* wph, z2* rho,
-                   double scale, hipStream_t s);
+                   double scale, hipStream_t s, const i64* Sdev = nullptr);
+
+// scf_device.hip: the steps of one SCF iteration that the host loop does on
+// the host (sort, Fermi level, occupations, OptimalStep, convergence
+// measure), as kernels, so that a whole iteration is one launch sequence.
+constexpr int OCC_MAX = 4096;  // window states (nb x Nk) the device sort holds
+struct OccArgs {
+  int N, Nk, nb, nnc, numk, occ_mode;
+  i64 nF;
+  double energy_tol, kT;
+  const double* band;     // [N] window energies, flat band-major b * Nk + k
+  const double* ph_cell;  // [Nk][nnc][2] cell phases of each k
+  int* idx;               // [N] window column (b + nb k) of each weighted state
+  double* wph;            // [N][nnc][2] occupation x cell phase
+  i64* S;                 // number of weighted states
+  double* occ;            // E_F, nOcc, nPart, DegFactor
+};
+int device_occupations(const OccArgs& a, hipStream_t s);
+int band_gather(i64 n, i64 nb, i64 nL1, int K, i64 k0, i64 Nk, const double* evals, double* band,
+                hipStream_t s);
+int gather_cols_dev(i64 n, i64 Smax, const i64* S, const int* idx, const z2* Cwin, z2* Psi,
+                    hipStream_t s);
+struct StepArgs {
+  int mix_mode;
+  double beta, step_alt, alpha, alphaH, U;
+  const double* e5;    // energy-pass sums (K_a, K_b, F_aa, F_ab, F_bb)
+  const double* dots;  // k_dots [0..5], k_bias_dot [8..9]
+  const double* occ;   // E_F, nOcc, nPart
+  double* st;          // state: [0..3] energies, [4] E_tot, [5] step
+  double* row;         // trajectory row (12)
+};
+int scf_step_update(const StepArgs& a, hipStream_t s);
+// row[6] = max_i |((1-s) da_i + s db_i) - da_i|, s = st[5]
+int density_convergence(i64 n, const double* da, const double* db, const double* st, double* row,
+                        hipStream_t s);
+int mix_dev(size_t tot, z2* a, const z2* b, const double* st, hipStream_t s);
+int potential_dev(i64 n, const double* ya, const double* yb, double* da, const double* db,
+                  const double* st, double alphaH, double U, double* pot, hipStream_t s);
 
 // c3.hip: the C3-reduced k-set (nenforceC3 = 1).
 struct C3Geom {

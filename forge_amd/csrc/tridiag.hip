@@ -839,6 +839,10 @@ void tridiag_release(TridiagWork& w) {
   w.graph = nullptr;
 }
 
+int tridiag_enqueue(TridiagWork& w, hipStream_t s) {
+  return enqueue_reduction(w, s, nullptr, nullptr, nullptr, true);
+}
+
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes) {
   if (!w.use_graph || hemv_ms) return enqueue_reduction(w, s, hemv_ms, hemv_launches, hemv_bytes, false);

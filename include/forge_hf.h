@@ -51,9 +51,17 @@ typedef struct {
                          FP64) until |E_in - E_new| < switch_de, then FP64 to convergence */
   int32_t timing;     /* nonzero: per-stage HIP-event timings (forge_scf_timings) */
   double switch_de;   /* eV, precision 1 (0: 1e-4) */
-  int32_t graph;      /* nonzero: the Householder reduction's launches (3 per column and one
+  int32_t graph;      /* 1: the Householder reduction's launches (3 per column and one
                          trailing update per panel) are captured into a HIP graph on first
-                         use and replayed while the batch shape is unchanged; not with timing */
+                         use and replayed while the batch shape is unchanged; not with timing.
+                         2: the whole SCF iteration runs on the device (sort, Fermi level,
+                         occupations, OptimalStep / linear mixing and the convergence measure
+                         as kernels; one host synchronisation per iteration); 3: as 2, each
+                         phase's iteration (mixed, FP64) captured into one HIP graph on its
+                         first use and replayed.  2 and 3 apply to the plain loop (no C3 set,
+                         valley projection or symmetry enforcement, every k-point in one
+                         eigensolver batch, nb * numk^2 <= 4096) without timing; the first
+                         iteration of each phase of a solve runs on the host path. */
   int32_t reserved;   /* 0 */
 } forge_scf_options;
 
