@@ -1012,46 +1012,28 @@ struct Occupation {
   std::vector<double> wph;    // [S][nnc][2]
 };
 
-// SortEnergies_1Spin (HartreeFock.f90:885-925) + the Fermi block
-// (Main.f90:400-412) + GetFock weights (HartreeFock.f90:506-555).
-static void occupations(const forge_ctx* c, const std::vector<double>& band, Occupation& o) {
-  const i64 Nk = c->Nk, nb = c->nb, N = nb * Nk, nnc = c->nnc;
-  std::vector<double> key(N), val(N);
+// SortEnergies_1Spin (HartreeFock.f90:885-925): the window energies in
+// ascending order (key) and the 1-based flat source of each (val).
+static void sort_window(const forge_ctx* c, const std::vector<double>& band, std::vector<double>& key,
+                        std::vector<double>& val) {
+  const i64 Nk = c->Nk, nb = c->nb, N = nb * Nk;
+  key.resize(N);
+  val.resize(N);
   for (i64 ic = 1; ic <= N; ++ic) {
     i64 indev = (ic - 1) / Nk + 1;
     key[ic - 1] = band[(indev - 1) + nb * (ic - (indev - 1) * Nk - 1)];
     val[ic - 1] = (double)ic;
   }
   heap_sort2(key, val);
-  const i64 NP = (c->n / 2 - c->nL) + 1;
-  double x = (double)(Nk * NP) + (double)Nk / 2.0 * c->P.nfilling;
-  o.nF = (i64)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5));
-  o.EF = (key[o.nF - 1] + key[o.nF]) / 2.0;
-  o.nOcc = o.nPart = 0;
-  for (i64 q = 0; q < N; ++q) {
-    if ((key[q] - o.EF) < -c->P.energy_tol) ++o.nOcc;
-    else if (std::fabs(key[q] - o.EF) < c->P.energy_tol) { ++o.nOcc; ++o.nPart; }
-  }
-  o.Deg = 1.0 / (double)o.nPart * (double)(o.nF + o.nPart - o.nOcc);
-  std::vector<double> w(N, 0.0), wi(N, 0.0);
-  if (c->opt.occ_mode == 2) {
-    double kT = c->opt.kT > 0 ? c->opt.kT : 1e-3;
-    double lo = key[0] - 50 * kT - 1.0, hi = key[N - 1] + 50 * kT + 1.0, mu = 0;
-    for (int it = 0; it < 200; ++it) {
-      mu = 0.5 * (lo + hi);
-      double s = 0;
-      for (i64 q = 0; q < N; ++q) s += 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
-      if (s > (double)o.nF) hi = mu; else lo = mu;
-    }
-    o.EF = mu;
-    for (i64 q = 0; q < N; ++q) w[q] = 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
-  } else {
-    for (i64 q = 0; q < o.nOcc && q < N; ++q) {
-      if (q < o.nOcc - o.nPart) w[q] = 1.0;
-      else if (c->opt.occ_mode == 0) { w[q] = (double)(float)o.Deg; wi[q] = 8.0; }
-      else w[q] = o.Deg;
-    }
-  }
+}
+
+// The weighted states of GetFock (HartreeFock.f90:506-555): for each sorted
+// position q with a weight w[q] + i wi[q] != 0, its Cwin column and its nnc
+// complex weights (weight x cell phase of its momentum).
+static void weighted_states(const forge_ctx* c, const std::vector<double>& val,
+                            const std::vector<double>& w, const std::vector<double>& wi,
+                            Occupation& o) {
+  const i64 Nk = c->Nk, nb = c->nb, N = nb * Nk, nnc = c->nnc;
   o.idx.clear();
   o.wph.clear();
   for (i64 q = 0; q < N; ++q) {
@@ -1067,6 +1049,55 @@ static void occupations(const forge_ctx* c, const std::vector<double>& band, Occ
       o.wph.push_back(z.imag());
     }
   }
+}
+
+// T = 0 weights of the first nOcc sorted states (GetFock,
+// HartreeFock.f90:506-555): 1, and DegFactor for the last nPart of them
+// (occ_mode 0: the reference's zdeg = (float(DegFactor), 8); 1: real).
+static void zero_temperature_weights(int occ_mode, i64 N, i64 nOcc, i64 nPart, double Deg,
+                                     std::vector<double>& w, std::vector<double>& wi) {
+  w.assign(N, 0.0);
+  wi.assign(N, 0.0);
+  for (i64 q = 0; q < nOcc && q < N; ++q) {
+    if (q < nOcc - nPart) w[q] = 1.0;
+    else if (occ_mode == 0) { w[q] = (double)(float)Deg; wi[q] = 8.0; }
+    else w[q] = Deg;
+  }
+}
+
+// SortEnergies_1Spin + the Fermi block (Main.f90:400-412) + GetFock weights.
+static void occupations(const forge_ctx* c, const std::vector<double>& band, Occupation& o) {
+  const i64 Nk = c->Nk, nb = c->nb, N = nb * Nk;
+  std::vector<double> key, val;
+  sort_window(c, band, key, val);
+  const i64 NP = (c->n / 2 - c->nL) + 1;
+  double x = (double)(Nk * NP) + (double)Nk / 2.0 * c->P.nfilling;
+  o.nF = (i64)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5));
+  o.EF = (key[o.nF - 1] + key[o.nF]) / 2.0;
+  o.nOcc = o.nPart = 0;
+  for (i64 q = 0; q < N; ++q) {
+    if ((key[q] - o.EF) < -c->P.energy_tol) ++o.nOcc;
+    else if (std::fabs(key[q] - o.EF) < c->P.energy_tol) { ++o.nOcc; ++o.nPart; }
+  }
+  o.Deg = 1.0 / (double)o.nPart * (double)(o.nF + o.nPart - o.nOcc);
+  std::vector<double> w, wi;
+  if (c->opt.occ_mode == 2) {
+    w.assign(N, 0.0);
+    wi.assign(N, 0.0);
+    double kT = c->opt.kT > 0 ? c->opt.kT : 1e-3;
+    double lo = key[0] - 50 * kT - 1.0, hi = key[N - 1] + 50 * kT + 1.0, mu = 0;
+    for (int it = 0; it < 200; ++it) {
+      mu = 0.5 * (lo + hi);
+      double s = 0;
+      for (i64 q = 0; q < N; ++q) s += 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
+      if (s > (double)o.nF) hi = mu; else lo = mu;
+    }
+    o.EF = mu;
+    for (i64 q = 0; q < N; ++q) w[q] = 1.0 / (1.0 + std::exp((key[q] - mu) / kT));
+  } else {
+    zero_temperature_weights(c->opt.occ_mode, N, o.nOcc, o.nPart, o.Deg, w, wi);
+  }
+  weighted_states(c, val, w, wi, o);
 }
 
 extern "C" int forge_getfock(forge_ctx* c, forge_z* zFock, const forge_z* zSorted,
@@ -1486,9 +1517,16 @@ static int valley_transform_cwin(forge_ctx* c);
 // window density into c->rho_new (which holds the bulk): rho_new = (bulk + win) / Nk.
 // in_loop: an SCF iteration (the valley projection applies only there; the
 // initial states of Main.f90:238-288 and of the guesses use the plain states)
+static int window_of(forge_ctx* c, const Occupation& occ, bool in_loop);
+
 static int window_step(forge_ctx* c, const std::vector<double>& band, Occupation& occ,
                        bool in_loop) {
   occupations(c, band, occ);
+  return window_of(c, occ, in_loop);
+}
+
+// the window density of the weighted states occ (see window_step)
+static int window_of(forge_ctx* c, const Occupation& occ, bool in_loop) {
   if (c->valley && in_loop) {
     // Main.f90:469-494: GetFock of the states, ValleyTransform, GetFock of
     // the transformed states (same energies, so the same sorted columns),
@@ -1963,6 +2001,401 @@ extern "C" int forge_scf_get_density(forge_ctx* c, forge_z* rho) {
 extern "C" int forge_scf_timings(forge_ctx* c, double* out16) {
   if (!c || !out16) { set_error("forge_scf_timings: null argument"); return -2; }
   for (int q = 0; q < 16; ++q) out16[q] = c->tm[q];
+  return 0;
+}
+
+// ---------------------------------------------------------------- two spins (numS = 2)
+// The SCF loop of Main.f90:236-679 with numS = 2 on the device: two contexts
+// of the same system, one per spin, each holding its spin's density, bulk
+// and window states, potential and eigensolver batch.  Per iteration:
+// Solve (or Solve_C3) for each spin with its own potential, SortEnergies_1Spin
+// per spin, SortEnergies_2Spins and the two-spin Fermi block
+// (Main.f90:416-458), GetFock per spin with its own counts and the shared
+// DegFactor, OptimalStep on the two-spin energy (HartreeFock.f90:1148-1227:
+// kinetic and Fock energies summed over the spins, the Hartree energy of the
+// total density and the Hubbard energy D1.D2, factor 3 - numS = 1), mixing
+// of both spins with the one step, and the potentials
+// V_1 = alphaH L (d_1 + d_2) + U d_2, V_2 = alphaH L (d_1 + d_2) + U d_1
+// (Main.f90:586-595).  Everything stays on the device; the host reads the
+// window energies and a few scalars.
+
+namespace forge {
+
+// With A = a0 + a1, B = b0 + b1 and yX = L X: out[0] = A.yA, out[1] = A.yB,
+// out[2] = B.yB, out[3] = a0.a1, out[4] = a0.b1 + b0.a1, out[5] = b0.b1
+// (single block, fixed order; b*, yb* may be null: terms read 0).
+__global__ void k_dots2(i64 n, const double* a0, const double* b0, const double* ya0,
+                        const double* yb0, const double* a1, const double* b1, const double* ya1,
+                        const double* yb1, double* out) {
+  __shared__ double sh[6][256];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (i64 i = threadIdx.x; i < n; i += 256) {
+    const double x0 = a0[i], x1 = a1[i];
+    const double z0 = b0 ? b0[i] : 0.0, z1 = b1 ? b1[i] : 0.0;
+    const double A = x0 + x1, B = z0 + z1;
+    const double yA = ya0[i] + ya1[i], yB = (yb0 ? yb0[i] : 0.0) + (yb1 ? yb1[i] : 0.0);
+    acc[0] += A * yA;
+    acc[1] += A * yB;
+    acc[2] += B * yB;
+    acc[3] += x0 * x1;
+    acc[4] += x0 * z1 + z0 * x1;
+    acc[5] += z0 * z1;
+  }
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) out[q] = sh[q][0];
+}
+
+// d_s = (1-l) da_s + l db_s, y = (1-l)(ya0 + ya1) + l (yb0 + yb1):
+// pot0 = alphaH y + U d1, pot1 = alphaH y + U d0; da_s <- d_s (db*, yb* may
+// be null when l = 0)
+__global__ void k_potential2(i64 n, double* da0, const double* db0, const double* ya0,
+                             const double* yb0, double* da1, const double* db1, const double* ya1,
+                             const double* yb1, double l, double alphaH, double U, double* pot0,
+                             double* pot1) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double u = 1.0 - l;
+  double y = ya0[i] + ya1[i], d0 = da0[i], d1 = da1[i];
+  if (db0) {
+    y = u * y + l * (yb0[i] + yb1[i]);
+    d0 = u * d0 + l * db0[i];
+    d1 = u * d1 + l * db1[i];
+  }
+  pot0[i] = alphaH * y + U * d1;
+  pot1[i] = alphaH * y + U * d0;
+  da0[i] = d0;
+  da1[i] = d1;
+}
+
+// Energies (Ekin_1 + Ekin_2, EF_1 + EF_2, EH, EU) of (1-l) a + l b from the
+// per-spin pieces of energy_pieces and the cross-spin dots of k_dots2.
+static void energies2_at(const double* pc0, const double* pc1, const double* x, double l, double* e) {
+  const double u = 1.0 - l;
+  e[0] = 0.0;
+  e[1] = 0.0;
+  for (const double* pc : {pc0, pc1}) {
+    e[0] += u * (pc[0] + pc[11]) + l * (pc[1] + pc[12]);
+    e[1] += u * u * pc[2] + 2.0 * u * l * pc[3] + l * l * pc[4];
+  }
+  e[2] = 0.5 * (u * u * x[0] + 2.0 * u * l * x[1] + l * l * x[2]);
+  e[3] = u * u * x[3] + u * l * x[4] + l * l * x[5];
+}
+
+static double total_energy2(const forge_params& P, const double* e) {
+  return P.alpha * e[1] + e[0] + P.U * e[3] + P.alphaH * e[2];
+}
+
+}  // namespace forge
+
+struct forge_scf2 {
+  forge_ctx* c[2] = {nullptr, nullptr};
+  double* x = nullptr;  // device: the 6 cross-spin dots
+  bool ready = false, mixed_done = false;
+  double e[4] = {0, 0, 0, 0}, Etot = 0;
+  int it = 0;
+};
+
+extern "C" int forge_scf2_create(forge_ctx* spin1, forge_ctx* spin2, forge_scf2** out) {
+  if (!spin1 || !spin2 || !out || spin1 == spin2) {
+    set_error("forge_scf2_create: two distinct contexts expected");
+    return -2;
+  }
+  const forge_params &a = spin1->P, &b = spin2->P;
+  if (a.ndim != b.ndim || a.nnc != b.nnc || a.numk != b.numk || a.nLower != b.nLower ||
+      a.nUpper != b.nUpper || a.nlayers != b.nlayers || a.nfilling != b.nfilling || a.U != b.U ||
+      a.alpha != b.alpha || a.alphaH != b.alphaH || a.Delta != b.Delta ||
+      a.energy_tol != b.energy_tol || a.step_alt != b.step_alt || spin1->Coords != spin2->Coords) {
+    set_error("forge_scf2_create: the two contexts must describe the same system");
+    return -2;
+  }
+  auto* p = new forge_scf2();
+  p->c[0] = spin1;
+  p->c[1] = spin2;
+  if (dalloc((void**)&p->x, sizeof(double) * 8)) {
+    delete p;
+    return -4;
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" int forge_scf2_destroy(forge_scf2* p) {
+  if (!p) return 0;
+  if (p->x) (void)hipFree(p->x);
+  delete p;
+  return 0;
+}
+
+// Options, C3 set, symmetry and valley flags must agree between the spins;
+// the two-spin loop follows the reference's T = 0 rule and OptimalStep.
+static int scf2_check(forge_scf2* p, const char* who) {
+  forge_ctx *c0 = p->c[0], *c1 = p->c[1];
+  const forge_scf_options &o0 = c0->opt, &o1 = c1->opt;
+  if (o0.occ_mode != o1.occ_mode || o0.mix_mode != o1.mix_mode || o0.precision != o1.precision ||
+      o0.etol != o1.etol || o0.switch_de != o1.switch_de || c0->c3 != c1->c3 ||
+      c0->valley != c1->valley || c0->sym != c1->sym) {
+    set_error(std::string(who) + ": the two spins' options, C3 set, valley and symmetry flags differ");
+    return -2;
+  }
+  if (o0.occ_mode == 2 || o0.mix_mode != 0) {
+    set_error(std::string(who) + ": two spins follow the reference's T = 0 rule (occ_mode 0 or 1) "
+              "and OptimalStep (mix_mode 0)");
+    return -2;
+  }
+  if (c0->valley) {
+    set_error(std::string(who) + ": the valley projection runs with one spin");
+    return -2;
+  }
+  if (o0.graph >= 2 || o1.graph >= 2 || o0.timing || o1.timing) {
+    set_error(std::string(who) + ": two spins run with graph 0 or 1 and without timing");
+    return -2;
+  }
+  return 0;
+}
+
+static int scf2_solve(forge_ctx* c, const z2* rho, std::vector<double>& band) {
+  return c->c3 ? solve_device_c3(c, rho, c->pot, band, false)
+               : solve_device(c, rho, c->pot, band, false);
+}
+
+// energies and potentials of the spins' states c[s]->rho (Main.f90:290-346)
+static int scf2_init_state(forge_scf2* p) {
+  forge_ctx *c0 = p->c[0], *c1 = p->c[1];
+  const i64 n = c0->n;
+  double pc[2][13];
+  for (int s = 0; s < 2; ++s) {
+    CK(site_density(n, p->c[s]->rho, p->c[s]->da, p->c[s]->s));
+    CK(energy_pieces(p->c[s], p->c[s]->rho, nullptr, pc[s]));  // synchronises c[s]->s
+  }
+  hipLaunchKernelGGL(k_dots2, dim3(1), dim3(256), 0, c0->s, n, c0->da, nullptr, c0->ya, nullptr,
+                     c1->da, nullptr, c1->ya, nullptr, p->x);
+  FORGE_LAUNCH();
+  hipLaunchKernelGGL(k_potential2, dim3(cdiv(n, 256)), dim3(256), 0, c0->s, n, c0->da, nullptr,
+                     c0->ya, nullptr, c1->da, nullptr, c1->ya, nullptr, 0.0, c0->P.alphaH, c0->P.U,
+                     c0->pot, c1->pot);
+  FORGE_LAUNCH();
+  double x[6];
+  FORGE_HIP(hipMemcpyAsync(x, p->x, sizeof(x), hipMemcpyDeviceToHost, c0->s));
+  FORGE_HIP(hipStreamSynchronize(c0->s));
+  energies2_at(pc[0], pc[1], x, 0.0, p->e);
+  p->Etot = total_energy2(c0->P, p->e);
+  p->it = 0;
+  p->mixed_done = false;
+  p->ready = true;
+  for (forge_ctx* c : p->c) {
+    c->scf_ready = false;  // the contexts' own loop does not continue the pair's state
+    c->mixed_done = false;
+    std::fill(c->tm, c->tm + 16, 0.0);
+  }
+  return 0;
+}
+
+extern "C" int forge_scf2_init(forge_scf2* p, const forge_z* rho_start) {
+  if (!p) { set_error("forge_scf2_init: null pair"); return -2; }
+  CK(scf2_check(p, "forge_scf2_init"));
+  forge_ctx* c0 = p->c[0];
+  const i64 n = c0->n, nnc = c0->nnc;
+  const size_t tot = (size_t)nnc * n * n;
+  int fails = 0;
+  if (rho_start) {
+    for (int s = 0; s < 2; ++s)
+      FORGE_HIP(hipMemcpyAsync(p->c[s]->rho, (const z2*)rho_start + tot * s, sizeof(z2) * tot,
+                               hipMemcpyHostToDevice, p->c[s]->s));
+    for (int s = 0; s < 2; ++s) FORGE_HIP(hipStreamSynchronize(p->c[s]->s));
+  } else {
+    // first diagonalisation of spin 1 with rho = 0 and Potential = 0, its
+    // density copied to spin 2 (Main.f90:238-288)
+    FORGE_HIP(hipMemsetAsync(c0->pot, 0, sizeof(double) * n, c0->s));
+    std::vector<double> band;
+    fails = scf2_solve(c0, nullptr, band);
+    if (fails < 0) return fails;
+    Occupation occ;
+    CK(window_step(c0, band, occ, false));
+    FORGE_HIP(hipMemcpyAsync(c0->rho, c0->rho_new, sizeof(z2) * tot, hipMemcpyDeviceToDevice, c0->s));
+    FORGE_HIP(hipMemcpyAsync(p->c[1]->rho, c0->rho_new, sizeof(z2) * tot, hipMemcpyDeviceToDevice, c0->s));
+    FORGE_HIP(hipStreamSynchronize(c0->s));
+  }
+  CK(scf2_init_state(p));
+  return fails;
+}
+
+// SortEnergies_2Spins (HartreeFock.f90:929-1004) over the two spins' sorted
+// window energies key[0], key[1] (N each), with the window
+// W = [[NP-1, NP-1], [NP+3, NP+3]] of Main.f90:418-421: lab[q] = 0 or 1.
+// The reference fills its spin-2 lead-in from row ncount1 of spin 2 while
+// advancing ncount2 (HartreeFock.f90:951-956); that is kept.  Its merge
+// loops test the row bound after reading the row; the bound is tested first
+// here (it decides either way).
+static void sort_two_spins(const std::vector<double>* key, i64 N, i64 Nk, i64 NP,
+                           std::vector<double>& E, std::vector<int>& lab) {
+  const i64 w1 = NP - 1, w2 = NP + 3;
+  E.clear();
+  lab.clear();
+  i64 c1 = 1, c2 = 1;
+  auto put = [&](double v, int l) { E.push_back(v); lab.push_back(l); };
+  auto e1 = [&](i64 q) { return key[0][q - 1]; };
+  auto e2 = [&](i64 q) { return key[1][q - 1]; };
+  for (i64 q = 1; q <= (w1 - 1) * Nk; ++q) { put(e1(c1), 0); ++c1; }
+  for (i64 q = 1; q <= (w1 - 1) * Nk; ++q) { put(e2(c1), 1); ++c2; }
+  while (c1 <= Nk * (w2 - 1) && c2 <= Nk * (w2 - 1)) {
+    if (e1(c1) <= e2(c2)) {
+      put(e1(c1), 0); ++c1;
+      while (c1 <= N && e1(c1) <= e2(c2)) { put(e1(c1), 0); ++c1; }
+      put(e2(c2), 1); ++c2;
+    } else {
+      put(e2(c2), 1); ++c2;
+      while (c2 <= N && e2(c2) <= e1(c1)) { put(e2(c2), 1); ++c2; }
+      put(e1(c1), 0); ++c1;
+    }
+  }
+  while (c1 <= N) { put(e1(c1), 0); ++c1; }
+  while (c2 <= N) { put(e2(c2), 1); ++c2; }
+}
+
+// One two-spin SCF iteration; traj_row (12 values) may be null.
+static int scf2_step(forge_scf2* p, double* traj_row, bool* stop) {
+  forge_ctx *c0 = p->c[0], *c1 = p->c[1];
+  const i64 n = c0->n, nnc = c0->nnc, Nk = c0->Nk, N = c0->nb * Nk;
+  const size_t tot = (size_t)nnc * n * n;
+  *stop = false;
+  const bool mixed = c0->opt.precision == 1 && !p->mixed_done;
+  for (forge_ctx* c : p->c) {
+    c->mixed_done = p->mixed_done;
+    if (mixed) c->tm[12] += 1;
+  }
+  int fails = 0;
+  std::vector<double> band[2], key[2], val[2];
+  for (int s = 0; s < 2; ++s) {
+    const int f = scf2_solve(p->c[s], p->c[s]->rho, band[s]);
+    if (f < 0) return f;
+    fails += f;
+    sort_window(p->c[s], band[s], key[s], val[s]);
+  }
+  // two-spin Fermi block (Main.f90:416-458)
+  const i64 NP = (n / 2 - c0->nL) + 1;
+  std::vector<double> E;
+  std::vector<int> lab;
+  sort_two_spins(key, N, Nk, NP, E, lab);
+  const double xf = (double)(Nk * NP) + (double)Nk / 2.0 * c0->P.nfilling;
+  const i64 nF = 2 * (i64)(xf >= 0 ? std::floor(xf + 0.5) : -std::floor(-xf + 0.5));
+  if (nF < 1 || nF >= 2 * N) {
+    set_error("forge_scf2_run: the Fermi level lies outside the two spins' window");
+    return -2;
+  }
+  const double EF = (E[nF - 1] + E[nF]) / 2.0;
+  i64 nOcc[2] = {0, 0}, nPart[2] = {0, 0};
+  for (i64 q = 0; q < 2 * N; ++q) {
+    const int sp = lab[q];
+    if ((E[q] - EF) < -c0->P.energy_tol) ++nOcc[sp];
+    else if (std::fabs(E[q] - EF) < c0->P.energy_tol) { ++nPart[sp]; ++nOcc[sp]; }
+  }
+  const double Deg = (nPart[0] != 0 || nPart[1] != 0)
+                         ? 1.0 / (double)(nPart[0] + nPart[1]) *
+                               (double)(nF + nPart[0] + nPart[1] - nOcc[0] - nOcc[1])
+                         : 0.0;
+  // GetFock per spin with its own counts and the shared DegFactor, then the
+  // symmetry enforcement of each spin's new density (Main.f90:514-534)
+  for (int s = 0; s < 2; ++s) {
+    forge_ctx* c = p->c[s];
+    std::vector<double> w, wi;
+    zero_temperature_weights(c->opt.occ_mode, N, nOcc[s], nPart[s], Deg, w, wi);
+    Occupation occ;
+    weighted_states(c, val[s], w, wi, occ);
+    CK(window_of(c, occ, true));
+    if (c->sym) CK(symmetrize(n, (int)nnc, c->rho_new, c->rho_new + tot, c->sym, c->c2_perm, c->s));
+  }
+  // OptimalStep on the two-spin energy
+  double pc[2][13];
+  for (int s = 0; s < 2; ++s) {
+    CK(site_density(n, p->c[s]->rho_new, p->c[s]->db, p->c[s]->s));
+    CK(energy_pieces(p->c[s], p->c[s]->rho, p->c[s]->rho_new, pc[s]));  // synchronises c[s]->s
+  }
+  hipLaunchKernelGGL(k_dots2, dim3(1), dim3(256), 0, c0->s, n, c0->da, c0->db, c0->ya, c0->yb,
+                     c1->da, c1->db, c1->ya, c1->yb, p->x);
+  FORGE_LAUNCH();
+  double x[6];
+  FORGE_HIP(hipMemcpyAsync(x, p->x, sizeof(x), hipMemcpyDeviceToHost, c0->s));
+  std::vector<double> hda[2], hdb[2];
+  for (int s = 0; s < 2; ++s) {
+    hda[s].resize(n);
+    hdb[s].resize(n);
+    FORGE_HIP(hipMemcpyAsync(hda[s].data(), p->c[s]->da, sizeof(double) * n, hipMemcpyDeviceToHost, c0->s));
+    FORGE_HIP(hipMemcpyAsync(hdb[s].data(), p->c[s]->db, sizeof(double) * n, hipMemcpyDeviceToHost, c0->s));
+  }
+  FORGE_HIP(hipStreamSynchronize(c0->s));
+  const double ein[4] = {p->e[0], p->e[1], p->e[2], p->e[3]};
+  const double EtotIn = p->Etot;
+  double e1[4], eh[4];
+  energies2_at(pc[0], pc[1], x, 1.0, e1);
+  energies2_at(pc[0], pc[1], x, 0.5, eh);
+  const double step = scf_oda_step(EtotIn, total_energy2(c0->P, e1), total_energy2(c0->P, eh),
+                                   c0->P.step_alt);
+  energies2_at(pc[0], pc[1], x, step, p->e);
+  p->Etot = total_energy2(c0->P, p->e);
+  // mixing of both spins with the one step, then the potentials
+  for (int s = 0; s < 2; ++s) CK(mix(tot, p->c[s]->rho, p->c[s]->rho_new, step, p->c[s]->s));
+  hipLaunchKernelGGL(k_potential2, dim3(cdiv(n, 256)), dim3(256), 0, c0->s, n, c0->da, c0->db,
+                     c0->ya, c0->yb, c1->da, c1->db, c1->ya, c1->yb, step, c0->P.alphaH, c0->P.U,
+                     c0->pot, c1->pot);
+  FORGE_LAUNCH();
+  for (int s = 0; s < 2; ++s) FORGE_HIP(hipStreamSynchronize(p->c[s]->s));
+  // DensityConvergence: max over spins and sites |D_mixed - D_in| (Main.f90:564-571)
+  double dconv = 0;
+  for (int s = 0; s < 2; ++s)
+    for (i64 q = 0; q < n; ++q) {
+      const double dmix = (1.0 - step) * hda[s][q] + step * hdb[s][q];
+      dconv = std::max(dconv, std::fabs(dmix - hda[s][q]));
+    }
+  ++p->it;
+  if (traj_row) {
+    double* r = traj_row;
+    r[0] = (double)p->it; r[1] = EtotIn; r[2] = ein[0]; r[3] = ein[1]; r[4] = ein[2];
+    r[5] = ein[3]; r[6] = dconv; r[7] = step; r[8] = EtotIn - p->Etot; r[9] = EF;
+    r[10] = (double)(nOcc[0] + nOcc[1]); r[11] = (double)(nPart[0] + nPart[1]);
+  }
+  const double dE = std::fabs(EtotIn - p->Etot);
+  if (mixed) {
+    const double sw = c0->opt.switch_de > 0 ? c0->opt.switch_de : 1e-4;
+    if (dE < sw || (c0->opt.etol > 0 && dE < c0->opt.etol)) p->mixed_done = true;
+    return fails;
+  }
+  if (c0->opt.etol > 0 && dE < c0->opt.etol) *stop = true;
+  return fails;
+}
+
+extern "C" int forge_scf2_run(forge_scf2* p, int64_t max_iter, double* traj, int64_t* iters_done) {
+  if (!p || max_iter < 0) { set_error("forge_scf2_run: invalid argument"); return -2; }
+  if (!p->ready) { set_error("forge_scf2_run: call forge_scf2_init first"); return -5; }
+  CK(scf2_check(p, "forge_scf2_run"));
+  int fails = 0;
+  i64 done = 0;
+  for (i64 itn = 0; itn < max_iter; ++itn) {
+    bool stop = false;
+    const int f = scf2_step(p, traj ? traj + 12 * itn : nullptr, &stop);
+    if (f < 0) return f;
+    fails += f;
+    ++done;
+    if (stop) break;
+  }
+  if (iters_done) *iters_done = done;
+  return fails;
+}
+
+extern "C" int forge_scf2_get_density(forge_scf2* p, forge_z* rho) {
+  if (!p || !rho) { set_error("forge_scf2_get_density: null argument"); return -2; }
+  const size_t tot = (size_t)p->c[0]->nnc * p->c[0]->n * p->c[0]->n;
+  for (int s = 0; s < 2; ++s) {
+    FORGE_HIP(hipMemcpyAsync((z2*)rho + tot * s, p->c[s]->rho, sizeof(z2) * tot, hipMemcpyDeviceToHost,
+                             p->c[s]->s));
+    FORGE_HIP(hipStreamSynchronize(p->c[s]->s));
+  }
   return 0;
 }
 

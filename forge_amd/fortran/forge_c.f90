@@ -15,6 +15,7 @@ module forge_c
   public :: forge_getfock, forge_energies, forge_scf_set_options, forge_scf_init
   public :: forge_scf_run, forge_scf_get_density, forge_scf_timings
   public :: forge_sweep_create, forge_sweep_run, forge_sweep_destroy
+  public :: forge_scf2_create, forge_scf2_init, forge_scf2_run, forge_scf2_get_density, forge_scf2_destroy
   public :: forge_c3_pairs, forge_solve_c3, forge_getfock_c3, forge_scf_set_c3
   public :: forge_c2_pairs, forge_symmetrize, forge_scf_set_symmetry, forge_hartree_hubbard
   public :: forge_scf_init_guess, forge_scf_init_guess_update, forge_scf_init_update
@@ -357,6 +358,42 @@ module forge_c
       integer(c_int64_t), intent(out) :: iters_done
       integer(c_int) :: r
     end function forge_scf_run
+
+    ! two spins (numS = 2) on the device: a pair of contexts, one per spin
+    function forge_scf2_create(spin1, spin2, pair) bind(C, name="forge_scf2_create") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: spin1, spin2
+      type(c_ptr), intent(out) :: pair
+      integer(c_int) :: r
+    end function forge_scf2_create
+
+    function forge_scf2_init(pair, rho_start) bind(C, name="forge_scf2_init") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: pair, rho_start
+      integer(c_int) :: r
+    end function forge_scf2_init
+
+    function forge_scf2_run(pair, max_iter, traj, iters_done) bind(C, name="forge_scf2_run") result(r)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: pair
+      integer(c_int64_t), value :: max_iter
+      real(c_double), intent(out) :: traj(*)
+      integer(c_int64_t), intent(out) :: iters_done
+      integer(c_int) :: r
+    end function forge_scf2_run
+
+    function forge_scf2_get_density(pair, rho) bind(C, name="forge_scf2_get_density") result(r)
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: pair
+      complex(c_double_complex), intent(out) :: rho(*)
+      integer(c_int) :: r
+    end function forge_scf2_get_density
+
+    function forge_scf2_destroy(pair) bind(C, name="forge_scf2_destroy") result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: pair
+      integer(c_int) :: r
+    end function forge_scf2_destroy
 
     ! twist-angle sweeps: contexts advanced together, one uneven-size batch
     function forge_sweep_create(ctxs, nsys, sweep) bind(C, name="forge_sweep_create") result(r)

@@ -21,8 +21,9 @@
 ! receiving the reference's own Energy<parameters> and
 ! Convergence<parameters> files, Main.f90:352-358, 616-631) c3 (nenforceC3,
 ! Setup.f90:27: 1 = Solve_C3 / GetFock_C3 where Main.f90 switches on it)
-! numS (1, or 2 for independent up and down spins in mode=reference with the
-! reference's weights and optimal damping) start2 (spin 2's checkpoint when
+! numS (1, or 2 for independent up and down spins with the reference's
+! weights and optimal damping: mode=reference calls the per-spin shims,
+! mode=device runs the two-spin loop forge_scf2_run) start2 (spin 2's checkpoint when
 ! numS = 2; start= is then spin 1's) nrelax (1: relaxed bilayer lattice,
 ! Main.f90:156-160; relaxtable= overrides forge_amd/data/relaxation_fourier.txt)
 ! T C2 C2T (nenforceT / nenforceC2 /
@@ -59,7 +60,7 @@ program forge_main
   real(dp) :: KineticEnergy(2), FockEnergy(2), HartreeEnergy, HubbardEnergy, TotalEnergy
   real(dp) :: KineticEnergyIn(2), FockEnergyIn(2), HartreeEnergyIn, HubbardEnergyIn
   real(dp) :: TotalEnergyIn, FermiEnergy, DegFactor, Step, DensityConvergence
-  type(c_ptr) :: ctx
+  type(c_ptr) :: ctx, ctx2 = c_null_ptr, pair = c_null_ptr
   type(forge_scf_options_t) :: opt
   integer(c_int64_t) :: done
   integer :: fout
@@ -114,8 +115,8 @@ contains
     DensitySub = 0.5_dp
     Potential = 0.0_dp
     zFock = (0.0_dp, 0.0_dp)
-    if (numS == 2 .and. (trim(mode) /= 'reference' .or. occ /= 0 .or. mix /= 0)) then
-      write(*, *) 'forge_main: numS = 2 runs in mode=reference with occ=0 mix=0'
+    if (numS == 2 .and. (occ > 1 .or. mix /= 0)) then
+      write(*, *) 'forge_main: numS = 2 runs with the T = 0 rule (occ=0 or 1) and mix=0'
       error stop 1
     end if
     if (numS == 2 .and. nenforceValley == 1) then
@@ -138,23 +139,33 @@ contains
     if (trim(mode) == 'device') then
       opt%occ_mode = occ; opt%mix_mode = mix; opt%kT = kT; opt%beta = beta; opt%etol = etol
       opt%precision = prec; opt%switch_de = switch_de
-      call forge_check(forge_scf_set_options(ctx, opt), 'forge_scf_set_options')
-      if (nenforceC3 == 1) &
-        call forge_check(forge_scf_set_c3(ctx, c_loc(rotC3), c_loc(nC3pairs)), 'forge_scf_set_c3')
-      if (symflags /= 0) &
-        call forge_check(forge_scf_set_symmetry(ctx, symflags, nC2pairs), 'forge_scf_set_symmetry')
-      if (nenforceValley == 1) then
-        allocate(rotV(nlayers))
-        rotV = RotateLayers
-        call forge_check(forge_scf_set_valley(ctx, c_loc(rotV)), 'forge_scf_set_valley')
+      if (numS == 2) then
+        ! spin 2's context: the same system; the pair keeps both spins on the device
+        call forge_check(forge_ctx_create(setup_params(nnc), Coords, tn, MomentaValues, &
+                                          nMomentaComponents, nUnitCell_1, nUnitCell_2, ctx2), &
+                         'forge_ctx_create')
       end if
-      if (trim(start) == 'first') then
-        call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
-      else
-        call forge_check(forge_scf_init(ctx, c_loc(zFock)), 'forge_scf_init')
-      end if
+      call device_flags(ctx)
+      if (numS == 2) call device_flags(ctx2)
       allocate(traj(12, itmax))
-      call forge_check(forge_scf_run(ctx, int(itmax, c_int64_t), traj, done), 'forge_scf_run')
+      if (numS == 2) then
+        call forge_check(forge_scf2_create(ctx, ctx2, pair), 'forge_scf2_create')
+        if (trim(start) == 'first') then
+          call forge_check(forge_scf2_init(pair, c_null_ptr), 'forge_scf2_init')
+        else
+          call forge_check(forge_scf2_init(pair, c_loc(zFock)), 'forge_scf2_init')
+        end if
+        call forge_check(forge_scf2_run(pair, int(itmax, c_int64_t), traj, done), 'forge_scf2_run')
+        call forge_check(forge_scf2_destroy(pair), 'forge_scf2_destroy')
+        call forge_check(forge_ctx_destroy(ctx2), 'forge_ctx_destroy')
+      else
+        if (trim(start) == 'first') then
+          call forge_check(forge_scf_init(ctx, c_null_ptr), 'forge_scf_init')
+        else
+          call forge_check(forge_scf_init(ctx, c_loc(zFock)), 'forge_scf_init')
+        end if
+        call forge_check(forge_scf_run(ctx, int(itmax, c_int64_t), traj, done), 'forge_scf_run')
+      end if
       write(fout, '(A)') '# Energy: it Etot 2Ekin 2EF EH EU'
       do i = 1, done
         write(fout, '(F5.1,3X,5(F24.9,3X))') traj(1, i), traj(2:6, i)
@@ -176,6 +187,23 @@ contains
     close(fout)
     call forge_check(forge_ctx_destroy(ctx), 'forge_ctx_destroy')
   end subroutine run_engine
+
+  ! options, C3 set, symmetry and valley flags of a device-resident context
+  subroutine device_flags(c)
+    type(c_ptr), intent(in) :: c
+    call forge_check(forge_scf_set_options(c, opt), 'forge_scf_set_options')
+    if (nenforceC3 == 1) &
+      call forge_check(forge_scf_set_c3(c, c_loc(rotC3), c_loc(nC3pairs)), 'forge_scf_set_c3')
+    if (symflags /= 0) &
+      call forge_check(forge_scf_set_symmetry(c, symflags, nC2pairs), 'forge_scf_set_symmetry')
+    if (nenforceValley == 1) then
+      if (.not. allocated(rotV)) then
+        allocate(rotV(nlayers))
+        rotV = RotateLayers
+      end if
+      call forge_check(forge_scf_set_valley(c, c_loc(rotV)), 'forge_scf_set_valley')
+    end if
+  end subroutine device_flags
 
   subroutine reference_sequence()
     real(dp), allocatable :: erow(:, :), crow(:, :)

@@ -331,6 +331,57 @@ class Engine:
         return out
 
 
+class SpinPair:
+    """The numS = 2 SCF on the device (forge_scf2_*): two engines of the same
+    system, spin 1 and spin 2, owned by the caller.  Set the same options on
+    both (occ_mode 0 or 1, mix_mode 0) before init()."""
+
+    def __init__(self, spin1: Engine, spin2: Engine):
+        self.engines = (spin1, spin2)
+        self.lib = L.load()
+        pr = C.c_void_p()
+        L.check(self.lib.forge_scf2_create(spin1.ctx, spin2.ctx, C.byref(pr)), "forge_scf2_create")
+        self.pr = pr
+
+    def init(self, rho_start=None):
+        """rho_start: both spins' densities as a sequence of two (n, n, nnc)
+        arrays or one (n, n, nnc, 2) array; None: the first diagonalisation."""
+        if rho_start is None:
+            return L.check(self.lib.forge_scf2_init(self.pr, None), "forge_scf2_init")
+        if isinstance(rho_start, (list, tuple)):
+            rho_start = np.stack([np.asarray(r) for r in rho_start], axis=-1)
+        rho = _f(rho_start, np.complex128)
+        return L.check(self.lib.forge_scf2_init(self.pr, _p(rho)), "forge_scf2_init")
+
+    def run(self, max_iter):
+        """-> trajectory rows (it, E_in, Ekin1+Ekin2, EF1+EF2, EH, EU, max|dDensity|,
+        Step, E_in - E_new, E_Fermi, nOcc, nPart)."""
+        traj = np.zeros((max(max_iter, 1), 12))
+        done = C.c_int64(0)
+        rc = L.check(self.lib.forge_scf2_run(self.pr, int(max_iter), _p(traj), C.byref(done)),
+                     "forge_scf2_run")
+        if rc > 0:
+            print("zheev failed", rc)
+        return traj[:done.value]
+
+    def density(self):
+        e = self.engines[0]
+        rho = np.zeros((e.n, e.n, e.nnc, 2), np.complex128, order="F")
+        L.check(self.lib.forge_scf2_get_density(self.pr, _p(rho)), "forge_scf2_get_density")
+        return rho
+
+    def close(self):
+        if self.pr:
+            self.lib.forge_scf2_destroy(self.pr)
+            self.pr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Sweep:
     """SCF of several systems of different orders advanced together, their
     eigensolves batched into one uneven-size reduction per iteration

@@ -72,6 +72,11 @@ _SIGS = {
     "forge_scf_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),
     "forge_scf_get_density": (C.c_int, [vp, vp]),
     "forge_scf_timings": (C.c_int, [vp, vp]),
+    "forge_scf2_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "forge_scf2_destroy": (C.c_int, [vp]),
+    "forge_scf2_init": (C.c_int, [vp, vp]),
+    "forge_scf2_run": (C.c_int, [vp, i64, vp, C.POINTER(i64)]),
+    "forge_scf2_get_density": (C.c_int, [vp, vp]),
     "forge_sweep_create": (C.c_int, [vp, i64, C.POINTER(vp)]),
     "forge_sweep_destroy": (C.c_int, [vp]),
     "forge_sweep_run": (C.c_int, [vp, i64, vp, vp]),

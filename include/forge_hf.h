@@ -275,6 +275,37 @@ int forge_scf_run(forge_ctx* ctx, int64_t max_iter, double* traj, int64_t* iters
 /* Copies the current density matrix (ndim x ndim x nnc) to the host. */
 int forge_scf_get_density(forge_ctx* ctx, forge_z* rho);
 
+/* ---- Two spins (numS = 2) on the device ----
+ * The SCF loop of Main.f90:236-679 with numS = 2 (the per-spin Solve and
+ * SortEnergies_1Spin, SortEnergies_2Spins and the two-spin Fermi block of
+ * Main.f90:416-458, GetFock per spin, OptimalStep on the two-spin energy of
+ * HartreeFock.f90:1148-1227, the potentials of Main.f90:586-595), with both
+ * spins' densities, states and potentials kept on the device.  Replaces the
+ * per-spin host-array calls of the drop-in host's numS = 2 path.  spin1 and
+ * spin2 are two contexts of the same system (same parameters and
+ * coordinates), owned by the caller; each carries its spin's density, bulk
+ * and window states and eigensolver batch.  Their forge_scf_set_options must
+ * agree, with occ_mode 0 or 1 (the reference's T = 0 rule), mix_mode 0
+ * (OptimalStep), graph 0 or 1 and no timing; forge_scf_set_c3 and
+ * forge_scf_set_symmetry, if used, on both (the valley projection is
+ * refused: it runs with one spin). */
+typedef struct forge_scf2 forge_scf2;
+int forge_scf2_create(forge_ctx* spin1, forge_ctx* spin2, forge_scf2** out);
+int forge_scf2_destroy(forge_scf2* pair);
+/* rho_start: ndim x ndim x nnc x 2 (both spins, the reference's two
+ * checkpoint files, Main.f90:201-235), or NULL: the first diagonalisation of
+ * spin 1 with zero density and potential, copied to spin 2 (Main.f90:238-288).
+ * Returns eigensolver failures (> 0) or an error (< 0). */
+int forge_scf2_init(forge_scf2* pair, const forge_z* rho_start);
+/* Up to max_iter iterations.  traj rows (12 doubles): it, E_in, Ekin1+Ekin2,
+ * EF1+EF2, EH, EU (input state; the reference's numS = 2 Energy file,
+ * Main.f90:632-638), max over spins of max|dDensity|, Step, E_in - E_new,
+ * E_Fermi, nOcc1+nOcc2, nPart1+nPart2.  Stops once |E_in - E_new| < etol
+ * (etol > 0); with precision 1 the mixed phase runs until switch_de. */
+int forge_scf2_run(forge_scf2* pair, int64_t max_iter, double* traj, int64_t* iters_done);
+/* Both spins' densities, ndim x ndim x nnc x 2. */
+int forge_scf2_get_density(forge_scf2* pair, forge_z* rho);
+
 /* ---- Twist-angle sweeps (no reference counterpart: the reference runs one
  * system per process, Main.f90:34-685) ----
  * A sweep advances the SCF of nsys systems of different orders together (e.g.

@@ -198,16 +198,19 @@ def test_fortran_two_spin_file_names(case, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["reference", "device"])
 @pytest.mark.parametrize("case", ["t1k3_s2", "t1k3s_s2", "t2k3s_s2"])
-def test_fortran_host_two_spin_trajectory_matches_reference(case, tmp_path):
+def test_fortran_host_two_spin_trajectory_matches_reference(case, mode, tmp_path):
+    """mode=reference: the per-spin shims with host arrays; mode=device: the
+    two-spin loop on the device (forge_scf2_run)."""
     from forge_amd.checkpoint import write_checkpoint
     from forge_amd.geometry import build_geometry
     from forge_amd.seed import seeded_rho_2spins
-    print("fortran host numS=2", case, flush=True)
+    print("fortran host numS=2", case, mode, flush=True)
     args, m = _s2_args(case)
     refdir = tmp_path / "output"
     refdir.mkdir()
-    args = [FORTRAN_EXE, "mode=reference", *args, f"itmax={m['itmax']}",
+    args = [FORTRAN_EXE, f"mode={mode}", *args, f"itmax={m['itmax']}",
             f"out={tmp_path / 'traj.txt'}", f"refout={refdir}/"]
     if m["start"] == "split":
         g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
@@ -273,18 +276,19 @@ def test_fortran_host_symmetrised_trajectory_matches_reference(case, mode, tmp_p
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["reference", "device"])
 @pytest.mark.parametrize("case", ["t1k3s_s2C2", "t1k3s_s2c3"])
-def test_fortran_host_two_spins_with_symmetry_matches_reference(case, tmp_path):
+def test_fortran_host_two_spins_with_symmetry_matches_reference(case, mode, tmp_path):
     """Two spins with C2 enforcement or with the C3-reduced k-set."""
     from forge_amd.checkpoint import write_checkpoint
     from forge_amd.geometry import build_geometry
     from forge_amd.seed import seeded_rho_2spins
-    print("fortran host combination", case, flush=True)
+    print("fortran host combination", case, mode, flush=True)
     m = traj_meta(case)
     g = build_geometry(int(m["ntheta"]), int(m["nlayers"]), int(m["numk"]))
     for sp, rho in enumerate(seeded_rho_2spins(g.ndim, g.nnc, float(m["nfilling"])), start=1):
         write_checkpoint(str(tmp_path / f"s{sp}.ckpt"), rho)
-    args = [FORTRAN_EXE, "mode=reference", f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}",
+    args = [FORTRAN_EXE, f"mode={mode}", f"ntheta={m['ntheta']}", f"nlayers={m['nlayers']}",
             f"numk={m['numk']}", f"nfilling={m['nfilling']}", f"xi_nm={m['xi'].rstrip('nm')}",
             "numS=2", f"c3={m['nenforceC3']}", f"T={m['nenforceT']}", f"C2={m['nenforceC2']}",
             f"C2T={m['nenforceC2T']}", f"itmax={m['itmax']}", f"out={tmp_path / 'traj.txt'}",
