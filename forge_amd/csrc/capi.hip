@@ -500,6 +500,14 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
                    c->s, mixed ? c->S : nullptr, &width));
   c->tm[13] = width;
+  {
+    // the hemv variants of the last reduction, one decimal digit per band of
+    // trailing orders (lowest band last)
+    int code[4];
+    i64 lo[4];
+    (void)tridiag_hemv_plan(c->trd, code, lo);
+    c->tm[14] = 1000.0 * code[3] + 100.0 * code[2] + 10.0 * code[1] + code[0];
+  }
   if (timed) {
     FORGE_HIP(hipEventRecord(c->ev[5], c->s));
     FORGE_HIP(hipEventSynchronize(c->ev[5]));

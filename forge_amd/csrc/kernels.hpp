@@ -91,12 +91,16 @@ struct TridiagWork {
 };
 constexpr int TRD_NB = 32;  // panel width (a multiple of 8, at most 128; 64 measured slower)
 constexpr int TRD_TS = 32;     // hemv column-strip width
-constexpr int TRD_SEGR = 1024;   // hemv rows per block streaming the FP64 matrix
-constexpr int TRD_SEGR32 = 2048; // ... streaming the FP32 copy (measured: 1024 best for FP64,
-                                 // 2048 for the FP32 copy, whose blocks carry half the bytes)
+constexpr int TRD_SEGR = 1024;   // the smallest hemv segment height (sizes the partial arrays);
+                                 // the height per trailing order is tuned (1024 or 2048)
 constexpr int TRD_PB = 4;      // hemv panel-product blocks per row segment
 size_t tridiag_workspace_bytes(i64 n, int K);
 int tridiag_bind(TridiagWork& w, void* base, i64 n, int K);
+// the hemv variants of w's shape per band of trailing orders (bands start at
+// band_lo[0..3]): code = segment height / 1024 + 2 x lane layout (1, 2: 16
+// lanes per column chunk, 1024 or 2048 rows; 3, 4: 8 lanes with 16-byte
+// row-pair loads); returns 1 when tuned, 0 for the defaults
+int tridiag_hemv_plan(const TridiagWork& w, int* code_by_band, i64* band_lo);
 int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
             double* hemv_bytes);
 // the reduction's launches alone, for an enclosing stream capture (no host

@@ -35,6 +35,11 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "kernels.hpp"
@@ -50,10 +55,6 @@ __host__ __device__ static inline i64 nt_of(i64 m) { return (m + TS - 1) / TS; }
 // row segments of strip J (rows 32J .. m-1) of height segr
 __host__ __device__ static inline i64 nseg_of(i64 m, i64 J, i64 segr = SEGR) {
   return (m - TS * J + segr - 1) / segr;
-}
-// segment height of the hemv streaming elements of type TM
-template <typename TM> __host__ __device__ constexpr int segr_of() {
-  return sizeof(TM) == 8 ? TRD_SEGR32 : TRD_SEGR;
 }
 __host__ __device__ static inline i64 nblk_of(i64 m) { return (m + 255) / 256; }
 // rows per k_trd_fin block and its block count
@@ -249,102 +250,83 @@ __global__ void __launch_bounds__(256) k_trd_col(i64 n, i64 j, i64 i0, int jj, z
 }
 
 // ---------------------------------------------------------------- K2: hemv
-// grid (PB + nt, nseg(m, 0), K).  Strip blocks (x = PB + J) stream column strip J
-// (32 columns, c0 = base + 32 J) over rows [32 J + s SG, min(m, 32 J + (s+1) SG)), SG = 1024 (FP64) or 2048 (FP32 copy)
-// of the trailing matrix.  Lane l of a wave covers row (l mod 16) and the 8
-// columns 8 (l / 16) .. + 7 of a 16-row step, so one load instruction reads
-// four 256-byte column segments, a thread holds only 8 column partials, and
-// the 4 waves take interleaved 16-row steps:
+// grid (PB + nt, nseg(m, 0, SG), K).  Strip blocks (x = PB + J) stream column
+// strip J (32 columns, c0 = base + 32 J) over rows
+// [32 J + s SG, min(m, 32 J + (s+1) SG)) of the trailing matrix:
 //   P[J][r]      = sum_c A(r, c) v_c                (c <= r inside the diagonal tile)
 //   P2[J][s][c]  = sum_r conj(A(r, c)) v_r          (r >  c inside the diagonal tile)
 // Blocks x < PB form the W^H v and V^H v partials of chunk x of row segment s.
-// TM: element type of the streamed matrix M (z2: M = A; float2: the FP32 copy)
-template <typename TM>
-// 3 waves per SIMD (at most 168 VGPRs, no spills): the loads in flight, not
-// the arithmetic, bound this kernel (4 waves need spills and run slower)
-__global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int jj, const z2* A_,
-                                                  size_t sA, i64 lda, const TM* M_, size_t sM,
-                                                  const z2* W_, const double* npart_, z2* P_,
-                                                  z2* P2_, z2* tpart_, z2* scal_, i64 ntmax,
-                                                  i64 msmax, i64 nbmax) {
-  constexpr int SG = segr_of<TM>();  // rows per segment
-  const int k = blockIdx.z;
-  const i64 s = blockIdx.y;
-  const i64 base = j + 1, m = n - base, nt = nt_of(m);
-  const z2* A = A_ + k * sA;
+// TM: element type of the streamed matrix (z2: A itself; float2: the FP32
+// copy).  SG: rows per segment, chosen per trailing order by the tuner below
+// (k_trd_fin sums the partials of the same segmentation).
+
+// W^H v and V^H v over rows [rs, re) of the trailing part: chunk pb of
+// segment s.  These blocks come first in the grid and are small, so they
+// finish under the strip blocks instead of forming the tail.
+template <int SG>
+__device__ __forceinline__ void hemv_panel_block(i64 n, i64 j, i64 i0, int jj, const z2* A, i64 lda,
+                                                 const z2* W_, const double* npart, z2* scal_,
+                                                 z2* tpart_, i64 msmax, int k, i64 s) {
+  const i64 base = j + 1, m = n - base;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (blockIdx.x < TRD_PB) {
-    // ---- W^H v and V^H v over rows [rs, re) of the trailing part: chunk pb
-    // of segment s.  These blocks come first in the grid and are small, so
-    // they finish under the strip blocks instead of forming the tail.
-    const int pb = blockIdx.x;
-    const i64 rs = s * SG + pb * (SG / TRD_PB);
-    const i64 re = rs + SG / TRD_PB < m ? rs + SG / TRD_PB : m;
-    const bool publish = s == 0 && pb == 0;
-    Reflector R{};
-    if (publish || (jj > 0 && rs < m))  // uniform over the block
-      R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
-    if (publish && threadIdx.x == 0) {  // published for k_trd_fin, which rewrites A(j+1, j)
-      scal_[k * 4 + 0] = R.tau;
-      scal_[k * 4 + 1] = R.scale;
-      scal_[k * 4 + 2] = mk(R.beta, 0.0);
-    }
-    const z2* W = W_ + (size_t)k * n * NB;
-    constexpr int QW = NB / 4;  // panel columns per wave, lanes over rows
-    const int qb = wv * QW;
-    // two panel columns per pass keep this branch's registers below the
-    // strip branch's, which sets the kernel's occupancy
-    z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
-    for (int q0 = 0; q0 < QW; q0 += 2) {
-      z2 t1[2] = {mk(0.0, 0.0), mk(0.0, 0.0)}, t2[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
-      if (qb + q0 < jj) {
-        for (i64 rr = rs + lane; rr < re; rr += 64) {
-          const i64 r = base + rr;
-          const z2 vr = (rr == 0) ? mk(1.0, 0.0) : A[r + lda * j] * R.scale;
+  const int pb = blockIdx.x;
+  const i64 rs = s * SG + pb * (SG / TRD_PB);
+  const i64 re = rs + SG / TRD_PB < m ? rs + SG / TRD_PB : m;
+  const bool publish = s == 0 && pb == 0;
+  Reflector R{};
+  if (publish || (jj > 0 && rs < m))  // uniform over the block
+    R = reflector_of(A, lda, n, j, npart);
+  if (publish && threadIdx.x == 0) {  // published for k_trd_fin, which rewrites A(j+1, j)
+    scal_[k * 4 + 0] = R.tau;
+    scal_[k * 4 + 1] = R.scale;
+    scal_[k * 4 + 2] = mk(R.beta, 0.0);
+  }
+  const z2* W = W_ + (size_t)k * n * NB;
+  constexpr int QW = NB / 4;  // panel columns per wave, lanes over rows
+  const int qb = wv * QW;
+  // two panel columns per pass keep this branch's registers below the
+  // strip branch's, which sets the kernel's occupancy
+  z2* tp = tpart_ + (((size_t)k * msmax + s) * TRD_PB + pb) * 2 * NB;
+  for (int q0 = 0; q0 < QW; q0 += 2) {
+    z2 t1[2] = {mk(0.0, 0.0), mk(0.0, 0.0)}, t2[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
+    if (qb + q0 < jj) {
+      for (i64 rr = rs + lane; rr < re; rr += 64) {
+        const i64 r = base + rr;
+        const z2 vr = (rr == 0) ? mk(1.0, 0.0) : A[r + lda * j] * R.scale;
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            if (qb + q0 + q < jj) {
-              t1[q] = t1[q] + cmul(W[r + n * (qb + q0 + q)], vr);
-              t2[q] = t2[q] + cmul(A[r + lda * (i0 + qb + q0 + q)], vr);
-            }
+        for (int q = 0; q < 2; ++q) {
+          if (qb + q0 + q < jj) {
+            t1[q] = t1[q] + cmul(W[r + n * (qb + q0 + q)], vr);
+            t2[q] = t2[q] + cmul(A[r + lda * (i0 + qb + q0 + q)], vr);
           }
         }
       }
-      // written also for empty chunks: k_trd_fin sums every chunk of the
-      // segments that hold rows
+    }
+    // written also for empty chunks: k_trd_fin sums every chunk of the
+    // segments that hold rows
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
+    for (int q = 0; q < 2; ++q) {
+      double a = t1[q].x, b = t1[q].y, c = t2[q].x, d = t2[q].y;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          a += __shfl_xor(a, off, 64);
-          b += __shfl_xor(b, off, 64);
-          c += __shfl_xor(c, off, 64);
-          d += __shfl_xor(d, off, 64);
-        }
-        if (lane == 0) {
-          tp[qb + q0 + q] = mk(a, b);
-          tp[NB + qb + q0 + q] = mk(c, d);
-        }
+      for (int off = 1; off < 64; off <<= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+        d += __shfl_xor(d, off, 64);
+      }
+      if (lane == 0) {
+        tp[qb + q0 + q] = mk(a, b);
+        tp[NB + qb + q0 + q] = mk(c, d);
       }
     }
-    return;
   }
-  const i64 J = (i64)blockIdx.x - TRD_PB;  // column strip
-  const i64 rs = TS * J + s * SG;  // first row (relative to base) of this block
-  if (J >= nt || rs >= m) return;
-  const i64 re = rs + SG < m ? rs + SG : m;
-  const Reflector R = reflector_of(A, lda, n, j, npart_ + (size_t)k * nbmax);
-  const z2 scale = R.scale;
-  const i64 c0 = base + J * TS;
-  const int ncol = (int)((n - c0) < TS ? (n - c0) : TS);
-  __shared__ z2 vJ[TS];
-  __shared__ z2 red[4][TS];
-  // v of this block's rows (relative rows ra0 + q; zero outside [rs, re)),
-  // staged once so that the row steps read it from LDS instead of waiting
-  // on a load per step
-  __shared__ z2 vrow[SG + 32];
-  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+}
+
+// v of strip J's columns and of the block's rows (relative rows ra0 + q;
+// zero outside [rs, re)), staged once in LDS
+template <int SG>
+__device__ __forceinline__ void hemv_stage_v(const z2* A, i64 lda, i64 n, i64 j, i64 base, i64 c0,
+                                             i64 rs, i64 re, i64 ra0, z2 scale, z2* vJ, z2* vrow) {
   if (threadIdx.x < TS) {
     const i64 c = c0 + threadIdx.x;
     vJ[threadIdx.x] = c >= n ? mk(0.0, 0.0) : (c == base ? mk(1.0, 0.0) : A[c + lda * j] * scale);
@@ -354,30 +336,53 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     vrow[q] = (rr < rs || rr >= re) ? mk(0.0, 0.0)
                                     : (rr == 0 ? mk(1.0, 0.0) : A[(base + rr) + lda * j] * scale);
   }
+}
+
+// Layout 0 (FP64 matrix or FP32 copy).  Lane l of a wave covers row
+// (l mod 16) and the 8 columns 8 (l / 16) .. + 7 of a 16-row step, so one
+// load instruction reads four column segments of 16 rows (256 bytes in
+// FP64, 128 in FP32), a thread holds only 8 column partials, and the 4 waves
+// take interleaved 16-row steps.  The FP32 copy is read one step ahead
+// (8-byte loads alone leave too few bytes in flight per wave); the FP64
+// loads gain nothing from it.
+template <typename TM, int SG>
+__device__ __forceinline__ void hemv_strip16(i64 n, i64 j, const z2* A, i64 lda, const TM* M,
+                                             const double* npart, z2* P_, z2* P2_, i64 ntmax,
+                                             i64 msmax, int k, i64 s) {
+  const i64 base = j + 1, m = n - base, nt = nt_of(m);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const i64 J = (i64)blockIdx.x - TRD_PB;  // column strip
+  const i64 rs = TS * J + s * SG;  // first row (relative to base) of this block
+  if (J >= nt || rs >= m) return;
+  const i64 re = rs + SG < m ? rs + SG : m;
+  const Reflector R = reflector_of(A, lda, n, j, npart);
+  const i64 c0 = base + J * TS;
+  const int ncol = (int)((n - c0) < TS ? (n - c0) : TS);
+  __shared__ z2 vJ[TS];
+  __shared__ z2 red[4][TS];
+  __shared__ z2 vrow[SG + 32];
+  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+  hemv_stage_v<SG>(A, lda, n, j, base, c0, rs, re, ra0, R.scale, vJ, vrow);
   __syncthreads();
   const int rsub = lane & 15, cg = lane >> 4;  // row in the step, column group
   const int cb = cg * 8;                       // first column of this lane's group
-  // v of the strip's columns stays in LDS (a broadcast read per use), which
-  // keeps the kernel within 128 VGPRs
   z2 acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = mk(0.0, 0.0);
   z2* P = P_ + ((size_t)k * ntmax + J) * n;
   // uniform base (strip start) and 32-bit per-lane offsets: the loads use a
-  // scalar base, which saves a VGPR pair per column address
-  const TM* strip = M_ + k * sM + (size_t)lda * c0;
-  const unsigned lda32 = (unsigned)lda, cofs = (unsigned)cb * lda32;
-  // row steps start at an absolute multiple of 16 rows (ra0), so with lda a
+  // scalar base, which saves a VGPR pair per column address; with lda a
   // multiple of 16 every 16-row column chunk is one aligned 256-byte segment
-  // The FP32 copy is read one step ahead (8-byte loads alone leave too few
-  // bytes in flight per wave); the FP64 loads gain nothing from it.
+  const TM* strip = M + (size_t)lda * c0;
+  const unsigned lda32 = (unsigned)lda, cofs = (unsigned)cb * lda32;
   constexpr bool PF = sizeof(TM) == 8;
   TM nxt[8];
   if (PF) {
     const i64 rr = ra0 + 16 * wv + rsub;
     const bool ok = rr >= rs && rr < re;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rr) + lda32 * q] : TM{};
+    for (int q = 0; q < 8; ++q)
+      nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rr) + lda32 * q] : TM{};
   }
   for (i64 rr0 = ra0 + 16 * wv; rr0 < re; rr0 += 64) {
     const i64 rr = rr0 + rsub;  // row relative to base
@@ -392,7 +397,8 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
       const i64 rn = rr + 64;
       const bool ok = rn >= rs && rn < re;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rn) + lda32 * q] : TM{};
+      for (int q = 0; q < 8; ++q)
+        nxt[q] = (ok && cb + q < ncol) ? strip[cofs + (unsigned)(base + rn) + lda32 * q] : TM{};
     } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q)
@@ -439,6 +445,150 @@ __global__ void __launch_bounds__(256, 3) k_trd_hemv(i64 n, i64 j, i64 i0, int j
     z2 t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
     P2_[(((size_t)k * ntmax + J) * msmax + s) * TS + threadIdx.x] = t;
   }
+}
+
+// 16-byte load of rows rr, rr + 1 (relative to base) of one column of the
+// FP32 copy; rows outside [rs, re) and columns past the strip read as zero
+__device__ __forceinline__ float4 load_pair32(const float2* p, bool colok, bool ok0, bool ok1) {
+  if (colok && ok0 && ok1) return *(const float4*)p;
+  const float2 a = (colok && ok0) ? p[0] : make_float2(0.f, 0.f);
+  const float2 b = (colok && ok1) ? p[1] : make_float2(0.f, 0.f);
+  return make_float4(a.x, a.y, b.x, b.y);
+}
+
+// Layout 1 (FP32 copy).  Lane l of a wave covers the rows 2 (l mod 8), + 1
+// of a 16-row step and the 4 columns 4 (l / 8) .. + 3 of the strip: each load is 16
+// bytes (two rows of one column), 8 lanes read one 128-byte column segment,
+// and one load instruction reads 8 of them.  A lane holds 4 column partials,
+// and the loads of the next two steps are in flight while a step is summed.  Accumulation is FP64 against the FP64 v, as for the FP64
+// matrix.
+template <int SG>
+__device__ __forceinline__ void hemv_strip8(i64 n, i64 j, const z2* A, i64 lda, const float2* M,
+                                             const double* npart, z2* P_, z2* P2_, i64 ntmax,
+                                             i64 msmax, int k, i64 s) {
+  const i64 base = j + 1, m = n - base, nt = nt_of(m);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const i64 J = (i64)blockIdx.x - TRD_PB;  // column strip
+  const i64 rs = TS * J + s * SG;  // first row (relative to base) of this block
+  if (J >= nt || rs >= m) return;
+  const i64 re = rs + SG < m ? rs + SG : m;
+  const Reflector R = reflector_of(A, lda, n, j, npart);
+  const i64 c0 = base + J * TS;
+  const int ncol = (int)((n - c0) < TS ? (n - c0) : TS);
+  __shared__ z2 vJ[TS];
+  __shared__ z2 red[4][TS];
+  __shared__ z2 vrow[SG + 32];
+  // row steps start at an absolute multiple of 16 rows, so every row pair
+  // starts at an even row and (lda even) every 16-byte load is aligned
+  const i64 ra0 = ((base + rs) & ~(i64)15) - base;  // may be < rs (masked rows)
+  hemv_stage_v<SG>(A, lda, n, j, base, c0, rs, re, ra0, R.scale, vJ, vrow);
+  __syncthreads();
+  const int rp = lane & 7, cg = lane >> 3;  // row pair in the step, column group
+  const int cb = cg * 4;                    // first column of this lane's group
+  // the strip columns' v stays in LDS (a broadcast read per use)
+  const z2* vc = vJ + cb;
+  z2 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = mk(0.0, 0.0);
+  z2* P = P_ + ((size_t)k * ntmax + J) * n;
+  const float2* strip = M + (size_t)lda * c0;
+  const unsigned lda32 = (unsigned)lda, cofs = (unsigned)cb * lda32;
+  // loads of the lane's row pair at relative row rn into d
+  auto load_step = [&](i64 rn, float4* d) {
+    const bool ok0 = rn >= rs && rn < re, ok1 = rn + 1 >= rs && rn + 1 < re;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      d[q] = load_pair32(strip + cofs + (unsigned)(base + rn) + lda32 * q, cb + q < ncol, ok0, ok1);
+  };
+  // two steps in flight ahead of the one being summed
+  float4 pf0[4], pf1[4];
+  load_step(ra0 + 16 * wv + 2 * rp, pf0);
+  load_step(ra0 + 16 * wv + 2 * rp + 64, pf1);
+  for (i64 rr0 = ra0 + 16 * wv; rr0 < re; rr0 += 64) {
+    const i64 rr = rr0 + 2 * rp;  // first of the lane's two rows (relative to base)
+    float4 cur[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cur[q] = pf0[q];
+      pf0[q] = pf1[q];
+    }
+    load_step(rr + 128, pf1);
+    const z2 vr0 = vrow[rr - ra0], vr1 = vrow[rr + 1 - ra0];  // zero unless valid
+    const i64 dr = rr - TS * J;  // < 32 inside the diagonal tile
+    z2 y0 = mk(0.0, 0.0), y1 = mk(0.0, 0.0);
+    if (rr0 >= TS * J + TS) {
+      // below the diagonal tile (wave-uniform): every element counts
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const z2 a0 = mk((double)cur[q].x, (double)cur[q].y);
+        const z2 a1 = mk((double)cur[q].z, (double)cur[q].w);
+        y0 = y0 + a0 * vc[q];
+        y1 = y1 + a1 * vc[q];
+        acc[q] = acc[q] + cmul(a0, vr0) + cmul(a1, vr1);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const z2 zero = mk(0.0, 0.0);
+        const int c = cb + q;
+        const z2 a0 = mk((double)cur[q].x, (double)cur[q].y);
+        const z2 a1 = mk((double)cur[q].z, (double)cur[q].w);
+        y0 = y0 + ((c <= dr) ? a0 : zero) * vc[q];
+        y1 = y1 + ((c <= dr + 1) ? a1 : zero) * vc[q];
+        acc[q] = acc[q] + cmul((c < dr) ? a0 : zero, vr0) + cmul((c < dr + 1) ? a1 : zero, vr1);
+      }
+    }
+    // row sums over the 8 column groups (lanes l, l^8, ..., l^56)
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+      y0.x += __shfl_xor(y0.x, off, 64);
+      y0.y += __shfl_xor(y0.y, off, 64);
+      y1.x += __shfl_xor(y1.x, off, 64);
+      y1.y += __shfl_xor(y1.y, off, 64);
+    }
+    if (cg == 0) {
+      if (rr >= rs && rr < re) P[rr] = y0;
+      if (rr + 1 >= rs && rr + 1 < re) P[rr + 1] = y1;
+    }
+  }
+  // column partials: sum over the 8 row-pair lanes, then over the 4 waves
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double x = acc[q].x, yv = acc[q].y;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+      x += __shfl_xor(x, off, 64);
+      yv += __shfl_xor(yv, off, 64);
+    }
+    if (rp == 0) red[wv][cb + q] = mk(x, yv);
+  }
+  __syncthreads();
+  if (threadIdx.x < TS) {
+    z2 t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    P2_[(((size_t)k * ntmax + J) * msmax + s) * TS + threadIdx.x] = t;
+  }
+}
+
+// LAYOUT 0: hemv_strip16; 1: hemv_strip8 (FP32 copy only).  3 waves per
+// SIMD (at most 168 VGPRs, no spills): the loads in flight, not the
+// arithmetic, bound it.
+template <typename TM, int SG, int LAYOUT>
+__global__ void __launch_bounds__(256, 3) k_trd_hemv(
+    i64 n, i64 j, i64 i0, int jj, const z2* A_, size_t sA, i64 lda, const TM* M_, size_t sM,
+    const z2* W_, const double* npart_, z2* P_, z2* P2_, z2* tpart_, z2* scal_, i64 ntmax,
+    i64 msmax, i64 nbmax) {
+  const int k = blockIdx.z;
+  const i64 s = blockIdx.y;
+  const z2* A = A_ + k * sA;
+  const double* npart = npart_ + (size_t)k * nbmax;
+  if (blockIdx.x < TRD_PB) {
+    hemv_panel_block<SG>(n, j, i0, jj, A, lda, W_, npart, scal_, tpart_, msmax, k, s);
+    return;
+  }
+  if constexpr (LAYOUT == 1)
+    hemv_strip8<SG>(n, j, A, lda, M_ + k * sM, npart, P_, P2_, ntmax, msmax, k, s);
+  else
+    hemv_strip16<TM, SG>(n, j, A, lda, M_ + k * sM, npart, P_, P2_, ntmax, msmax, k, s);
 }
 
 // ---------------------------------------------------------------- K3: finalize w
@@ -700,18 +850,187 @@ static int read_pair(TridiagWork& w, i64 q, double* ms) {
 }
 
 // hemv launch; with events e0/e1 they take the dispatch's begin and end
-template <typename TM>
-static void launch_hemv(dim3 g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, i64 n, i64 j, i64 i0,
-                        int jj, const TridiagWork& w, const TM* M, size_t sM, i64 ntmax, i64 msmax,
-                        i64 nbmax) {
+template <typename TM, int SG, int LAYOUT>
+static void launch_hemv_v(dim3 g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, i64 n, i64 j,
+                          i64 i0, int jj, const TridiagWork& w, const TM* M, size_t sM, i64 ntmax,
+                          i64 msmax, i64 nbmax) {
   if (e0)
-    hipExtLaunchKernelGGL(k_trd_hemv<TM>, g, dim3(256), 0, s, e0, e1, 0, n, j, i0, jj,
+    hipExtLaunchKernelGGL(k_trd_hemv<TM, SG, LAYOUT>, g, dim3(256), 0, s, e0, e1, 0, n, j, i0, jj,
                           (const z2*)w.A, w.sA, w.lda, M, sM, (const z2*)w.W,
                           (const double*)w.npart, w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
   else
-    hipLaunchKernelGGL(k_trd_hemv<TM>, g, dim3(256), 0, s, n, j, i0, jj, (const z2*)w.A, w.sA,
-                       w.lda, M, sM, (const z2*)w.W, (const double*)w.npart, w.P, w.P2, w.tpart,
-                       w.scal, ntmax, msmax, nbmax);
+    hipLaunchKernelGGL((k_trd_hemv<TM, SG, LAYOUT>), g, dim3(256), 0, s, n, j, i0, jj,
+                       (const z2*)w.A, w.sA, w.lda, M, sM, (const z2*)w.W, (const double*)w.npart,
+                       w.P, w.P2, w.tpart, w.scal, ntmax, msmax, nbmax);
+}
+
+// a hemv variant: segment height and lane layout
+struct HemvVariant {
+  int sg, layout;
+};
+
+// the hemv of column j over the first Kj matrices with variant v
+static void launch_hemv(HemvVariant v, hipStream_t s, hipEvent_t e0, hipEvent_t e1, i64 j, i64 i0,
+                        int jj, int Kj, const TridiagWork& w) {
+  const i64 n = w.n, m = n - j - 1, nt = nt_of(m);
+  const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = npart_stride(n);
+  const dim3 g((unsigned)(nt + TRD_PB), (unsigned)nseg_of(m, 0, v.sg), Kj);
+  if (w.S) {
+    const float2* S = (const float2*)w.S;
+    if (v.layout == 1) {
+      if (v.sg == 1024) launch_hemv_v<float2, 1024, 1>(g, s, e0, e1, n, j, i0, jj, w, S, w.sS, ntmax, msmax, nbmax);
+      else launch_hemv_v<float2, 2048, 1>(g, s, e0, e1, n, j, i0, jj, w, S, w.sS, ntmax, msmax, nbmax);
+    } else {
+      if (v.sg == 1024) launch_hemv_v<float2, 1024, 0>(g, s, e0, e1, n, j, i0, jj, w, S, w.sS, ntmax, msmax, nbmax);
+      else launch_hemv_v<float2, 2048, 0>(g, s, e0, e1, n, j, i0, jj, w, S, w.sS, ntmax, msmax, nbmax);
+    }
+  } else {
+    const z2* A = (const z2*)w.A;
+    if (v.sg == 1024) launch_hemv_v<z2, 1024, 0>(g, s, e0, e1, n, j, i0, jj, w, A, w.sA, ntmax, msmax, nbmax);
+    else launch_hemv_v<z2, 2048, 0>(g, s, e0, e1, n, j, i0, jj, w, A, w.sA, ntmax, msmax, nbmax);
+  }
+}
+
+// ---------------------------------------------------------------- hemv tuner
+// The hemv variant per band of trailing orders, measured on first use for a
+// shape (device, n, K, FP64 or FP32 copy) and kept for the process:
+//   * the segment height (rows per strip block, 1024 or 2048): tall segments
+//     give each block more bytes to stream and fewer partials, short ones
+//     give more blocks, which fills the chip when the trailing matrix is
+//     small;
+//   * for the FP32 copy, the lane layout: 16 lanes per 16-row column chunk
+//     with 8-byte loads, or 8 lanes with 16-byte row-pair loads.
+// In each band the order reaches, the hemv of one column at a representative
+// trailing order (the band's middle, at most n - 1) is launched three times
+// with each candidate and the last two are timed with HIP events.  The hemv
+// writes only its own outputs (partials, reflector scalars), which the
+// reduction rewrites before it reads them.  FORGE_TUNE_LOG=1 prints the
+// timings.
+constexpr int HEMV_BANDS = 4;
+constexpr i64 HEMV_BAND_LO[HEMV_BANDS] = {0, 1536, 3072, 6144};
+constexpr i64 HEMV_BAND_HI[HEMV_BANDS] = {1536, 3072, 6144, (i64)1 << 40};
+constexpr HemvVariant HEMV_CAND[4] = {{1024, 0}, {2048, 0}, {1024, 1}, {2048, 1}};
+
+struct HemvPlan {
+  HemvVariant v[HEMV_BANDS];
+};
+
+static std::mutex g_hemv_mu;
+static std::map<std::tuple<int, i64, int, int>, HemvPlan> g_hemv_plan;
+
+static int hemv_band(i64 m) {
+  int b = 0;
+  while (b + 1 < HEMV_BANDS && m >= HEMV_BAND_LO[b + 1]) ++b;
+  return b;
+}
+
+static HemvPlan default_plan(bool fp32) {
+  HemvPlan p;
+  for (int b = 0; b < HEMV_BANDS; ++b) p.v[b] = fp32 ? HemvVariant{2048, 0} : HemvVariant{1024, 0};
+  return p;
+}
+
+static std::tuple<int, i64, int, int> plan_key(const TridiagWork& w) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return std::make_tuple(dev, w.n, w.K, w.S ? 1 : 0);
+}
+
+// FORGE_HEMV_FORCE="<segment>,<layout>" (1024 or 2048; 0 or 1, the latter for
+// the FP32 copy only) replaces the tuned plan by that one variant in every
+// band, for tests of each variant; 0 when unset or not applicable.
+static int forced_plan(bool fp32, HemvPlan* out) {
+  const char* f = getenv("FORGE_HEMV_FORCE");
+  if (!f) return 0;
+  int sg = 0, layout = -1;
+  if (sscanf(f, "%d,%d", &sg, &layout) != 2 || (sg != 1024 && sg != 2048) || layout < 0 ||
+      layout > 1 || (!fp32 && layout != 0))
+    return 0;
+  for (int b = 0; b < HEMV_BANDS; ++b) out->v[b] = HemvVariant{sg, layout};
+  return 1;
+}
+
+// the plan of w's shape; tuned here when absent and may_tune (never inside a
+// stream capture), else the default variants
+static int hemv_plan(TridiagWork& w, hipStream_t s, bool may_tune, HemvPlan* out) {
+  if (forced_plan(w.S != nullptr, out)) return 0;
+  const auto key = plan_key(w);
+  {
+    std::lock_guard<std::mutex> g(g_hemv_mu);
+    auto it = g_hemv_plan.find(key);
+    if (it != g_hemv_plan.end()) { *out = it->second; return 0; }
+  }
+  const bool fp32 = w.S != nullptr;
+  HemvPlan p = default_plan(fp32);
+  if (!may_tune) { *out = p; return 0; }
+  const i64 n = w.n, nbmax = npart_stride(n);
+  const bool log = getenv("FORGE_TUNE_LOG") != nullptr;
+  // the norm partials the reflector reads: cleared, so the timed launches
+  // compute with finite numbers
+  hipLaunchKernelGGL(k_trd_clear, dim3(64), dim3(256), 0, s, (size_t)w.K * nbmax / 2, (z2*)w.npart);
+  FORGE_LAUNCH();
+  hipEvent_t e0, e1;
+  FORGE_HIP(hipEventCreate(&e0));
+  FORGE_HIP(hipEventCreate(&e1));
+  const int ncand = fp32 ? 4 : 2;
+  for (int b = 0; b < HEMV_BANDS; ++b) {
+    if (n - 1 <= HEMV_BAND_LO[b] || n - 1 < 64) continue;
+    const i64 hi = std::min<i64>(HEMV_BAND_HI[b], n - 1);
+    const i64 m = (HEMV_BAND_LO[b] + hi) / 2;
+    if (m < 1) continue;
+    const i64 j = n - 1 - m;
+    float best = 0.f;
+    for (int c = 0; c < ncand; ++c) {
+      const HemvVariant v = HEMV_CAND[c];
+      launch_hemv(v, s, nullptr, nullptr, j, j, 0, w.K, w);
+      FORGE_LAUNCH();
+      FORGE_HIP(hipEventRecord(e0, s));
+      launch_hemv(v, s, nullptr, nullptr, j, j, 0, w.K, w);
+      launch_hemv(v, s, nullptr, nullptr, j, j, 0, w.K, w);
+      FORGE_LAUNCH();
+      FORGE_HIP(hipEventRecord(e1, s));
+      FORGE_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FORGE_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (log) {
+        const double gbs = (fp32 ? 4.0 : 8.0) * (double)m * (double)m * w.K * 2.0 / (ms * 1e-3) / 1e9;
+        fprintf(stderr, "[hemv tuner] n=%lld K=%d %s m=%lld segment %d layout %d: %.1f us, %.0f GB/s\n",
+                (long long)n, w.K, fp32 ? "fp32" : "fp64", (long long)m, v.sg, v.layout,
+                ms * 500.f, gbs);
+      }
+      if (c == 0 || ms < best) {
+        best = ms;
+        p.v[b] = v;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> g(g_hemv_mu);
+  g_hemv_plan[key] = p;
+  *out = p;
+  return 0;
+}
+
+int tridiag_hemv_plan(const TridiagWork& w, int* code_by_band, i64* band_lo) {
+  HemvPlan forced;
+  const bool fp32 = w.S != nullptr;
+  if (forced_plan(fp32, &forced)) {
+    for (int b = 0; b < HEMV_BANDS; ++b) {
+      code_by_band[b] = forced.v[b].sg / 1024 + 2 * forced.v[b].layout;
+      band_lo[b] = HEMV_BAND_LO[b];
+    }
+    return 1;
+  }
+  const auto key = plan_key(w);
+  std::lock_guard<std::mutex> g(g_hemv_mu);
+  auto it = g_hemv_plan.find(key);
+  const HemvPlan p = it != g_hemv_plan.end() ? it->second : default_plan(fp32);
+  for (int b = 0; b < HEMV_BANDS; ++b) {
+    code_by_band[b] = p.v[b].sg / 1024 + 2 * p.v[b].layout;
+    band_lo[b] = HEMV_BAND_LO[b];
+  }
+  return it != g_hemv_plan.end() ? 1 : 0;
 }
 
 // The launch sequence of one reduction.  capturing: inside a stream capture
@@ -747,6 +1066,11 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
     FORGE_LAUNCH();
     Kj = 0;
   }
+  HemvPlan plan;
+  {
+    const int r = hemv_plan(w, s, !capturing, &plan);
+    if (r) return r;
+  }
   if (w.S) {
     hipLaunchKernelGGL(k_trd_to_fp32, dim3((unsigned)cdiv(n, 256), (unsigned)std::min<i64>(n, 512), K),
                        dim3(256), 0, s, n, w.A, w.sA, w.lda, w.S, w.sS);
@@ -762,9 +1086,9 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
       hipLaunchKernelGGL(k_trd_col, dim3(cdiv(n - j, 256), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.d, w.tau, w.npart, w.dpart, nbmax);
       FORGE_LAUNCH();
-      const int segr = w.S ? segr_of<float2>() : segr_of<z2>();
-      const i64 m = n - j - 1, nt = nt_of(m), ms = nseg_of(m, 0, segr);
-      const dim3 hgrid((unsigned)(nt + TRD_PB), (unsigned)ms, Kj);
+      const i64 m = n - j - 1;
+      const HemvVariant hv = plan.v[hemv_band(m)];
+      const int segr = hv.sg;
       const bool tj = timed && (j % TIME_STRIDE == 0);
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tj) {
@@ -776,12 +1100,7 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
         // algorithmic bytes: the lower triangle of the m x m trailing matrix
         bytes += (w.S ? 4.0 : 8.0) * (double)m * (double)m * Kj;
       }
-      if (w.S)
-        launch_hemv<float2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const float2*)w.S, w.sS, ntmax,
-                            msmax, nbmax);
-      else
-        launch_hemv<z2>(hgrid, s, e0, e1, n, j, i0, jj, w, (const z2*)w.A, w.sA, ntmax, msmax,
-                        nbmax);
+      launch_hemv(hv, s, e0, e1, j, i0, jj, Kj, w);
       FORGE_LAUNCH();
       hipLaunchKernelGGL(k_trd_fin, dim3((unsigned)nfin_of(m), Kj), dim3(256), 0, s, n, j, i0, jj, w.A,
                          w.sA, w.lda, w.W, w.e, w.tau, w.scal, w.P, w.P2, w.tpart, w.dpart, ntmax,
@@ -849,6 +1168,12 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   auto* g = (TrdGraph*)w.graph;
   if (!g || !g->same(w)) {
     tridiag_release(w);
+    {
+      // the hemv plan is measured before the capture (it times with events)
+      HemvPlan plan;
+      const int r = hemv_plan(w, s, true, &plan);
+      if (r) return r;
+    }
     hipGraph_t graph = nullptr;
     FORGE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     const int rc = enqueue_reduction(w, s, nullptr, nullptr, nullptr, true);

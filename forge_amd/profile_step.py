@@ -56,7 +56,8 @@ def main():
             print("[profile_step]   stage ms: " + " ".join(
                 f"{name} {t[q]:.1f}" for q, name in enumerate(
                     ("assembly", "reduction", "tridiag_eig", "back_transform", "density",
-                     "energies_mixing"))) + f" (back-transform block width {t[13]:.0f})",
+                     "energies_mixing"))) + f" (back-transform block width {t[13]:.0f}, "
+                f"hemv variants {t[14]:04.0f} by band)",
                   flush=True)
     e.close()
     return 0

@@ -39,7 +39,7 @@ def main():
     with open(os.path.join(d, "run_kernel_trace.csv")) as f:
         for r in csv.DictReader(f):
             nm = r["Kernel_Name"]
-            if "k_trd_hemv<forge::z2>" in nm:
+            if "k_trd_hemv<forge::z2" in nm:
                 key = "z2"
             elif "k_trd_hemv<HIP_vector_type<float" in nm:
                 key = "f32"

@@ -336,7 +336,14 @@ int forge_sweep_run(forge_sweep* sweep, int64_t max_iter, double* traj, int64_t*
  * (every 17th column), [10] their summed time (ms), [11] their algorithmic
  * bytes (8 m^2 per matrix in FP64, 4 m^2 from the FP32 copy; m the trailing
  * order), [12] iterations run with precision 1 still in its mixed phase,
- * [13] block width of the last back-transformation (tuned per shape). */
+ * [13] block width of the last back-transformation (tuned per shape),
+ * [14] the Householder hemv's variants of the last reduction (tuned per
+ * shape), one decimal digit per band of trailing orders (thousands
+ * m >= 6144, hundreds 3072..6143, tens 1536..3071, units below 1536): 1 and
+ * 2 segments of 1024 and 2048 rows with 16 lanes per column chunk, 3 and 4
+ * the same with 8 lanes and 16-byte row-pair loads (FP32 copy only).
+ * FORGE_HEMV_FORCE="<segment>,<layout>" in the environment forces one
+ * variant (layout 0 or 1) in every band. */
 int forge_scf_timings(forge_ctx* ctx, double* out16);
 
 /* ---- MLP of the learned initial guess (no reference counterpart) ----
