@@ -90,6 +90,10 @@ def parse():
                     help="1: the learned start ends with a diagonalisation of H[guess]; 0: the "
                          "guess itself is the start state")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config0-full", action="store_true",
+                    help="also solve the reference's CPU-runnable case completely on both: 6.01 deg TBG "
+                         "(n=364), 6x6 k, nu=+1, kT=1 meV, linear mixing 0.3, seeded start, "
+                         "|dE| < 1e-6 eV -- the oracle on the host cores and the plain FP64 GPU path")
     ap.add_argument("--cpu-sample-ntheta", type=int, default=11)
     a = ap.parse_args()
     if a.full:
@@ -132,6 +136,46 @@ def cpu_baseline(args, iters: int, n_full: int):
                    f"n={n_full} and by the {iters} iterations of the plain solve with the same mixer "
                    f"(extrapolated)"),
     }
+
+
+def config0_full(args):
+    """The complete plain FP64 SCF of the reference's own CPU-runnable case on
+    the host (the oracle: OpenMP over the 36 k-points, MKL sequential per k,
+    the reference's structure) and on the GPU, same seeded start and options;
+    wall times from the start state to the stop."""
+    from forge_amd.hf import Engine, Setup
+    from forge_amd.seed import seeded_rho
+    from oracle.pyoracle import System
+
+    ntheta, nlayers, numk, nfill = WORKLOADS["tbg_6.01deg_n364_k6"]
+    opts = dict(occ_mode=2, kT=0.001, mix_mode=1, beta=0.3, etol=1e-6)
+    e = Engine(Setup(ntheta=ntheta, nlayers=nlayers, numk=numk, nfilling=nfill, xi_nm=40.0))
+    start = seeded_rho(e.n, e.nnc, nfill)
+    e.set_options(**opts)
+    e.scf_init(start)
+    e.scf_run(1)  # warm-up (code objects, tuning)
+    e.set_options(**opts)
+    t0 = time.perf_counter()
+    e.scf_init(start)
+    tg = e.scf_run(400)
+    gpu_s = time.perf_counter() - t0
+    e.close()
+    nthreads = min(os.cpu_count() or 1, numk * numk)
+    s = System(ntheta, numk, nfill, nlayers=nlayers, xi_nm=40.0)
+    s.longrange()
+    t0 = time.perf_counter()
+    its, tc, _, _ = s.scf(400, rho_start=start, occ_mode=2, kT=0.001, mix_mode=1, beta=0.3, etol=1e-6,
+                          nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    log(f"config 0 full SCF: GPU {len(tg)} iterations {gpu_s:.2f}s E {tg[-1, 1] - tg[-1, 8]:.9f}; "
+        f"CPU oracle ({nthreads} threads) {its} iterations {cpu_s:.2f}s E {tc[-1, 1] - tc[-1, 8]:.9f}")
+    return {"workload": "tbg_6.01deg_n364_k6 (n=364, 6x6 k, nu=+1, kT=1 meV, linear 0.3, seeded, 1e-6 eV)",
+            "gpu_s": round(gpu_s, 2), "gpu_iterations": len(tg),
+            "gpu_final_energy_eV": round(float(tg[-1, 1] - tg[-1, 8]), 9),
+            "cpu_s": round(cpu_s, 2), "cpu_iterations": int(its), "cpu_threads": nthreads,
+            "cpu_kind": "port (the oracle)",
+            "cpu_final_energy_eV": round(float(tc[-1, 1] - tc[-1, 8]), 9),
+            "gpu_over_cpu_speedup": round(cpu_s / gpu_s, 2)}
 
 
 def roofline(stage, kernel, bytes_per_elem, workload):
@@ -320,6 +364,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu_iters = plain_iters or iters
         out["cpu_baseline"] = cpu_baseline(args, cpu_iters, n)
+    if args.config0_full:
+        out["config"]["config0_full_scf"] = config0_full(args)
     print(json.dumps(out), flush=True)
 
 

@@ -5,9 +5,10 @@
 
 1. Data: for every cell index m and filling nu, the first diagonalisation
    (rho_TB) and a plain FP64 SCF from the seeded start (linear mixing 0.3,
-   Fermi smearing 1 meV, |dE| < 1e-6 eV), then the descriptors and targets of
-   every site (forge_amd.learned).  Cell index max(cells) is held out for
-   validation.
+   Fermi smearing 1 meV, |dE| < 1e-6 eV); the converged site densities and
+   the nearest-neighbour bond values of both densities are written, and the
+   descriptors and targets of every site (forge_amd.learned) are formed from
+   them when training.  Cell index max(cells) is held out for validation.
 2. Training: Adam on the library's MLP kernels (forward, backward and update
    on the GPU), standardised inputs and targets.
 3. Report: validation loss and the SCF iterations from the learned start
@@ -23,6 +24,7 @@ import time
 import numpy as np
 
 from . import learned as Lr
+from .geometry import build_geometry
 from .hf import Engine, Setup
 from .mlp import Mlp, loss_grad
 from .seed import seeded_rho
@@ -36,33 +38,61 @@ def say(*a):
 
 
 def solve_system(m: int, nu: float, max_iter: int = 250):
+    """SCF of cell m at filling nu; returns the raw fields the targets are
+    formed from (converged diagonal, converged and non-interacting bond
+    values in nn_bonds order) and a summary row."""
     e = Engine(Setup(ntheta=m, numk=3, nfilling=nu, xi_nm=40.0))
     e.set_options(**SCF)
     t0 = time.time()
+    bonds = Lr.nn_bonds(e.geom)
     e.scf_init(None)
-    rho_tb = e.density()
+    b_tb = Lr.bond_values(e.density(), bonds)
     e.scf_init(seeded_rho(e.n, e.nnc, nu))
     tr = e.scf_run(max_iter)
     rho = e.density()
-    X = Lr.descriptors(e.geom, nu)
-    Y, M = Lr.targets(e.geom, rho, rho_tb)
+    raw = (np.real(np.diagonal(rho[:, :, 0])).copy(), Lr.bond_values(rho, bonds), b_tb)
     info = (m, nu, e.n, len(tr), tr[-1, 1] - tr[-1, 8], tr[-1, 8], time.time() - t0)
     e.close()
-    return X, Y, M, info
+    return raw, info
 
 
 def generate(cells, fillings, path):
-    Xs, Ys, Ms, sys_id, infos = [], [], [], [], []
+    """Solves every (m, nu) and writes the raw fields (npz: info rows, and
+    per system its diagonal and bond values, concatenated)."""
+    diag, bconv, btb, infos = [], [], [], []
     for m in cells:
         for nu in fillings:
-            X, Y, M, info = solve_system(m, nu)
+            (d, bc, bt), info = solve_system(m, nu)
             say("  m=%2d nu=%+.0f n=%5d: %3d iterations, E=%.6f eV, last dE %+.1e (%.1fs)" % info)
-            Xs.append(X); Ys.append(Y); Ms.append(M)
-            sys_id.append(np.full(len(X), len(infos)))
+            diag.append(d); bconv.append(bc); btb.append(bt)
             infos.append(info)
-    np.savez(path, X=np.concatenate(Xs), Y=np.concatenate(Ys), M=np.concatenate(Ms),
-             sys=np.concatenate(sys_id), info=np.array(infos))
+    np.savez(path, info=np.array(infos), diag=np.concatenate(diag), bconv=np.concatenate(bconv),
+             btb=np.concatenate(btb))
     return path
+
+
+def load_data(paths):
+    """Descriptors and targets of every system in the raw data files:
+    (X, Y, M, system id per site, info rows)."""
+    Xs, Ys, Ms, sid, infos = [], [], [], [], []
+    for p in paths:
+        f = np.load(p)
+        od = ob = 0
+        for row in f["info"]:
+            m, nu, n = int(row[0]), float(row[1]), int(row[2])
+            g = build_geometry(m, 2, 3)
+            bonds = Lr.nn_bonds(g)
+            nb = len(bonds.i)
+            X = Lr.descriptors(g, nu, bonds)
+            Y, M = Lr.targets_from_values(g, f["diag"][od:od + n], f["bconv"][ob:ob + nb],
+                                          f["btb"][ob:ob + nb], bonds)
+            od += n
+            ob += nb
+            Xs.append(X); Ys.append(Y); Ms.append(M)
+            sid.append(np.full(n, len(infos)))
+            infos.append(row)
+    return (np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms), np.concatenate(sid),
+            np.array(infos))
 
 
 def iterations_from(e, start_fn, max_iter=250):
@@ -95,13 +125,7 @@ def main():
         generate(cells, fillings, paths[0])
         if a.data_only:
             return 0
-    parts = [np.load(p) for p in paths]
-    X = np.concatenate([f["X"] for f in parts])
-    Y = np.concatenate([f["Y"] for f in parts])
-    M = np.concatenate([f["M"] for f in parts])
-    offs = np.cumsum([0] + [len(f["info"]) for f in parts[:-1]])
-    sid = np.concatenate([f["sys"] + o for f, o in zip(parts, offs)])
-    info = np.concatenate([f["info"] for f in parts])
+    X, Y, M, sid, info = load_data(paths)
     held = info[:, 0] == cells[-1]
     val = np.isin(sid, np.nonzero(held)[0])
     say(f"[train] {int((~val).sum())} training sites, {int(val.sum())} validation sites "
@@ -120,15 +144,18 @@ def main():
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     model.save(a.out)
     say(f"[train] model written to {a.out}")
-    # the held-out cell at nu = +1: seeded start against the learned start
+    # the held-out cell at nu = +1: seeded start against the learned start,
+    # with linear mixing 0.3 and with optimal damping
     m = cells[-1]
     e = Engine(Setup(ntheta=m, numk=3, nfilling=1.0, xi_nm=40.0))
-    e.set_options(**SCF)
-    its_s = iterations_from(e, lambda: e.scf_init(seeded_rho(e.n, e.nnc, 1.0)))
-    its_tb = iterations_from(e, lambda: e.scf_init(None))
-    its_l = iterations_from(e, lambda: Lr.learned_start(e, model, 1.0))
-    for tag, r in (("seeded", its_s), ("first diagonalisation", its_tb), ("learned", its_l)):
-        say(f"[check] m={m} nu=+1 start {tag:22s}: {r[0]:3d} iterations, E={r[1]:.6f} eV ({r[2]:.1f}s)")
+    for mixer, opts in (("linear 0.3", SCF), ("optimal damping", dict(SCF, mix_mode=0))):
+        e.set_options(**opts)
+        its_s = iterations_from(e, lambda: e.scf_init(seeded_rho(e.n, e.nnc, 1.0)))
+        its_tb = iterations_from(e, lambda: e.scf_init(None))
+        its_l = iterations_from(e, lambda: Lr.learned_start(e, model, 1.0))
+        for tag, r in (("seeded", its_s), ("first diagonalisation", its_tb), ("learned", its_l)):
+            say(f"[check] m={m} nu=+1 {mixer}, start {tag:22s}: {r[0]:3d} iterations, "
+                f"E={r[1]:.6f} eV ({r[2]:.1f}s)")
     e.close()
     return 0
 

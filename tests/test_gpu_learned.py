@@ -1,21 +1,27 @@
 """What the trained network gains, judged by the plain SCF (GPU).
 
 The network (forge_amd/data/guess_mlp.npz) was trained on converged FP64
-solutions of TBG cells m = 6..14; cell m = 15 (n = 2884, 3 x 3 k, filling +1,
-kT = 1 meV) was held out.  On it, with the config's linear mixing 0.3, three
-starts are solved to |dE| < 1e-6 eV in FP64:
+solutions of TBG cells m = 6..14 (local stacking descriptors, forge_amd.learned);
+cell m = 15 (n = 2884, 3 x 3 k, filling +1, kT = 1 meV) was held out.  On it,
+three starts are solved to |dE| < 1e-6 eV in FP64:
 
   seeded              uniform + 1e-3 Gaussian (the method without the network)
   first diagonalisation  the reference's own initial state
   learned             first diagonalisation + the network's site charges and
                       bond shifts + one diagonalisation of H[proposal]
 
-The learned start must reach the seeded start's state (same energy within
-1e-5 eV) in at most 60 % of its iterations (measured when written: 25 vs 64).
-The network's site charges are judged against the converged solution: their
-coefficient of determination must exceed 0.8 (measured 0.96 on the held-out
-cell).  The start without the last diagonalisation (forge_scf_init_update)
-must reach the same state too.
+With the config's linear mixing 0.3 the learned start must reach the seeded
+start's state (same energy within 1e-5 eV) in at most 60 % of its iterations
+(measured when trained: 26 vs 64).  With optimal damping (OptimalStep, the
+benchmark's mixer) it must take fewer iterations than the seeded start and
+reach a state no higher (measured: 7 iterations against 10, and a state
+1.2 meV below the one the seeded start settles in).
+The surrogate's fields are judged against the converged solution: the site
+charges (the on-site Hartree field) with R^2 > 0.8 (measured 0.965 on the
+held-out cell) and the bond shift common to a site's three bonds (the
+symmetric bond-Fock field, Re b_0) with R^2 > 0.5 (measured 0.867).  The
+start without the last diagonalisation (forge_scf_init_update) must reach the
+same state too.
 """
 import os
 import sys
@@ -73,12 +79,33 @@ def test_learned_start_gains_over_seeded(eng):
     assert abs(El - Es) < 1e-5
     assert len(learned) <= 0.6 * len(seeded)
     # the surrogate judged by the converged SCF: site charges n (rho_ii - 1/2)
+    # and the symmetric bond shift Re b_0
     Y, _ = targets(eng.geom, rho_conv, rho_tb)
-    ss_res = float(np.sum((Yp[:, 0] - Y[:, 0]) ** 2))
-    ss_tot = float(np.sum((Y[:, 0] - Y[:, 0].mean()) ** 2))
-    r2 = 1.0 - ss_res / ss_tot
-    say(f"  site charges: R^2 of the network against the converged solution {r2:.3f}")
-    assert r2 > 0.8
+    r2 = []
+    for col in (0, 1):
+        ss_res = float(np.sum((Yp[:, col] - Y[:, col]) ** 2))
+        ss_tot = float(np.sum((Y[:, col] - Y[:, col].mean()) ** 2))
+        r2.append(1.0 - ss_res / ss_tot)
+    say(f"  R^2 of the network against the converged solution: site charges {r2[0]:.3f}, "
+        f"symmetric bond shift {r2[1]:.3f}")
+    assert r2[0] > 0.8
+    assert r2[1] > 0.5
+
+
+def test_learned_start_gains_with_optimal_damping(eng):
+    say("learned vs seeded start with optimal damping, m =", M_HELD_OUT)
+    model = Mlp.load(MODEL)
+    eng.set_options(occ_mode=2, mix_mode=0, kT=0.001, etol=1e-6)
+    eng.scf_init(seeded_rho(eng.n, eng.nnc, NU))
+    seeded = eng.scf_run(200)
+    eng.set_options(occ_mode=2, mix_mode=0, kT=0.001, etol=1e-6)
+    learned_start(eng, model, NU)
+    learned = eng.scf_run(200)
+    Es, El = (t[-1, 1] - t[-1, 8] for t in (seeded, learned))
+    say(f"  seeded {len(seeded)} its E {Es:.9f}; learned {len(learned)} its E {El:.9f}")
+    assert abs(seeded[-1, 8]) < 1e-6 and abs(learned[-1, 8]) < 1e-6
+    assert len(learned) < len(seeded)
+    assert El <= Es + 1e-5
 
 
 def test_learned_start_without_last_diagonalisation(eng):

@@ -77,11 +77,45 @@ def test_nearest_neighbour_bonds(m, nl):
 
 def test_descriptors_shape_and_registry():
     g = build_geometry(4, 2, 3)
-    X = Lr.descriptors(g, 1.0)
+    b = Lr.nn_bonds(g)
+    X = Lr.descriptors(g, 1.0, b)
+    layer, sub = Lr.site_labels(g)
     assert X.shape == (g.ndim, Lr.D_IN)
-    assert np.allclose(X[:, :2].sum(1), 1) and np.allclose(X[:, 2:4].sum(1), 1)
-    assert np.all(np.abs(X[:, 4:16]) <= 1 + 1e-6)
-    assert np.allclose(X[:, 16], 0.25) and np.allclose(X[:, 17], 2.5)
+    assert np.allclose(X[:, :2].sum(1), 1) and np.allclose(X[:, 2:5].sum(1), 1)
+    # bilayer: layer 0 is the bottom (only a layer above), layer 1 the top
+    assert np.all(X[layer == 0, 2] == 1) and np.all(X[layer == 1, 4] == 1)
+    assert np.all(X[layer == 0, 17:19] == [0, 1]) and np.all(X[layer == 1, 17:19] == [1, 0])
+    assert np.all(X[layer == 0, 5:11] == 0) and np.all(X[layer == 1, 11:17] == 0)
+    # cos^2 + sin^2 = 1 for every phase present
+    for col in (5, 11):
+        has = X[:, 17 + (col - 5) // 6] == 1
+        c, s = X[has, col:col + 6:2], X[has, col + 1:col + 6:2]
+        assert np.allclose(c * c + s * s, 1, atol=1e-6)
+    assert np.allclose(X[:, 19], 0.25) and np.allclose(X[:, 20], 2.5)
+    # the cell is centred on an AA point: some site sits on top of an A site of
+    # the other layer (all three phases 0)
+    assert np.max(X[layer == 0, 11:17:2].sum(1)) > 3 - 1e-6
+
+
+def test_layer_lattice_phases():
+    """G_k . (r - o) is a multiple of 2 pi on every A site of the layer itself
+    and takes the one value G_k . (d) on its B sites (d = r_B - r_A of a bond),
+    for both layers of a bilayer and every layer of a trilayer."""
+    for L in (2, 3):
+        g = build_geometry(3, L, 3)
+        b = Lr.nn_bonds(g)
+        layer, sub = Lr.site_labels(g)
+        C = np.asarray(g.Coords)[:, :2]
+        for l, (G, o) in enumerate(Lr.layer_lattices(g, b)):
+            ph = (C[(layer == l) & (sub == 0)] - o) @ G.T
+            assert np.allclose(np.cos(ph), 1, atol=1e-9)
+            phb = (C[(layer == l) & (sub == 1)] - o) @ G.T
+            assert np.allclose(np.cos(phb - phb[0]), 1, atol=1e-9)
+            assert not np.allclose(np.cos(phb[0]), 1)
+        X = Lr.descriptors(g, 0.0, b)
+        if L == 3:
+            mid = layer == 1
+            assert np.all(X[mid, 3] == 1) and np.all(X[mid, 17:19] == 1)
 
 
 def test_targets_guess_round_trip():
@@ -110,3 +144,19 @@ def test_targets_guess_round_trip():
     off = ~touched
     off[idx, idx, 0] = False
     assert np.array_equal(gr[off], tb[off])
+
+
+def test_bond_harmonics():
+    """b_0 is the shift common to the three bonds, b_1 / b_2 the patterns
+    w^-s / w^-2s over the angle slots s (w = exp(2 pi i / 3)); the transform
+    inverts exactly."""
+    w = np.exp(2j * np.pi / 3)
+    s = np.arange(3)
+    B = np.stack([np.full(3, 0.3 - 0.2j), 0.5 * w ** (-s), (0.1 + 0.4j) * w ** (-2 * s)])
+    H = Lr.to_harmonics(B)
+    np.testing.assert_allclose(H[0], [0.3, -0.2, 0, 0, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(H[1], [0, 0, 0.5, 0, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(H[2], [0, 0, 0, 0, 0.1, 0.4], atol=1e-15)
+    rng = np.random.default_rng(3)
+    R = rng.standard_normal((7, 3)) + 1j * rng.standard_normal((7, 3))
+    np.testing.assert_allclose(Lr.from_harmonics(Lr.to_harmonics(R)), R, atol=1e-14)
