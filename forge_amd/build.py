@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT, "libforge_hip.so")
-SOURCES = ["operators.hip", "tridiag.hip", "dc.hip", "backtr.hip", "gemm.hip", "density.hip",
+SOURCES = ["operators.hip", "tridiag.hip", "twostage.hip", "dc.hip", "backtr.hip", "gemm.hip", "density.hip",
            "c3.hip", "sym.hip", "valley.hip", "mlp.hip", "scf_device.hip", "capi.hip", "noalloc.hip"]
 FSRC = os.path.join(HERE, "fortran")
 # dependency order: the C-ABI binding, parameters, geometry, shims, driver

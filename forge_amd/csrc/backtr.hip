@@ -49,19 +49,21 @@ __global__ void k_bt_promote(size_t nm, const double* __restrict__ Z, size_t ldZ
   }
 }
 
-// V(r', q) for rows r = i0+1+r' of reflector j = i0+q: 1 at r = j+1, A(r, j) below.
+// V(r', q) for rows r = i0+voff+r' of reflector j = i0+q: 1 at r = j+voff, A(r, j)
+// below (voff = 1 after the one-stage reduction, the band width after the
+// first stage of the two-stage one).
 template <typename E>
-__global__ void k_bt_vclean(i64 n, i64 i0, int nb, const z2* __restrict__ A, size_t sA, i64 lda,
-                            E* __restrict__ V, size_t sV) {
+__global__ void k_bt_vclean(i64 n, i64 i0, int nb, int voff, const z2* __restrict__ A, size_t sA,
+                            i64 lda, E* __restrict__ V, size_t sV) {
   int k = blockIdx.y;
-  i64 mt = n - i0 - 1;
+  i64 mt = n - i0 - voff;
   for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < (size_t)mt * nb;
        q += (size_t)gridDim.x * 256) {
     i64 rr = (i64)(q % mt), c = (i64)(q / mt);
-    i64 r = i0 + 1 + rr, j = i0 + c;
+    i64 r = i0 + voff + rr, j = i0 + c;
     z2 v;
-    if (r == j + 1) v = mk(1.0, 0.0);
-    else if (r > j + 1) v = A[k * sA + r + lda * j];
+    if (r == j + voff) v = mk(1.0, 0.0);
+    else if (r > j + voff) v = A[k * sA + r + lda * j];
     else v = mk(0.0, 0.0);
     V[k * sV + rr + (size_t)n * c] = narrow<E>(v);
   }
@@ -176,25 +178,30 @@ static int apply_q(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ,
   FORGE_LAUNCH();
   if (promoted) FORGE_HIP(hipEventRecord(promoted, s));
   if (n < 2) return 0;
+  // two-stage reduction: the chase's reflectors first (Q = Q1 Q2)
+  if (t.two_stage && !first_only) CK_BT(twostage_apply_q2(t, m, C, ldC, sC, s));
+  const int voff = t.two_stage ? t.voff : 1;
+  const i64 nr = n - voff;  // reflectors of the (first-stage) reduction
+  if (nr < 1) return 0;
   const Op CT = Op::H, NO = Op::N;
-  const i64 nblocks = (n - 1 + BT - 1) / BT;
+  const i64 nblocks = (nr + BT - 1) / BT;
   for (i64 bq = first_only ? 0 : nblocks - 1; bq >= 0; --bq) {
     const i64 i0 = bq * BT;
-    const int nb = (int)((n - 1 - i0) < BT ? (n - 1 - i0) : BT);
-    const i64 mt = n - i0 - 1;
+    const int nb = (int)((nr - i0) < BT ? (nr - i0) : BT);
+    const i64 mt = n - i0 - voff;
     hipLaunchKernelGGL(k_bt_vclean<E>, dim3(cdiv(mt * nb, 256) < 4096 ? cdiv(mt * nb, 256) : 4096, K),
-                       dim3(256), 0, s, n, i0, nb, t.A, t.sA, t.lda, V, sV);
+                       dim3(256), 0, s, n, i0, nb, voff, t.A, t.sA, t.lda, V, sV);
     FORGE_LAUNCH();
     // G = V^H V
     CK_BT(gemm_sb(CT, NO, nb, nb, mt, 1.0, V, n, sV, V, n, sV, 0.0, G, BT, sG, K, s));
     hipLaunchKernelGGL((k_bt_tbuild<E, BT>), dim3(nb, K), dim3(64), 0, s, nb, i0, n, G, t.tau, T);
     FORGE_LAUNCH();
-    // Y = V^H C(i0+1:n, :)
-    CK_BT(gemm_sb(CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + 1, ldC, sC, 0.0, Y, BT, sY, K, s));
+    // Y = V^H C(i0+voff:n, :)
+    CK_BT(gemm_sb(CT, NO, nb, m, mt, 1.0, V, n, sV, C + i0 + voff, ldC, sC, 0.0, Y, BT, sY, K, s));
     // Y2 = T Y
     CK_BT(gemm_sb(NO, NO, nb, m, nb, 1.0, T, BT, sG, Y, BT, sY, 0.0, Y2, BT, sY, K, s));
-    // C(i0+1:n, :) -= V Y2
-    CK_BT(gemm_sb(NO, NO, mt, m, nb, -1.0, V, n, sV, Y2, BT, sY, 1.0, C + i0 + 1, ldC, sC, K, s));
+    // C(i0+voff:n, :) -= V Y2
+    CK_BT(gemm_sb(NO, NO, mt, m, nb, -1.0, V, n, sV, Y2, BT, sY, 1.0, C + i0 + voff, ldC, sC, K, s));
   }
   return 0;
 }
@@ -242,7 +249,7 @@ static int tuned_width(const TridiagWork& t, const double* Z, size_t ldZ, size_t
         FORGE_HIP(hipEventSynchronize(e1));
         FORGE_HIP(hipEventElapsedTime(&ms, e0, e1));
       }
-      const double nb = (double)std::min<i64>(bt, t.n - 1);
+      const double nb = (double)std::min<i64>(bt, t.n - (t.two_stage ? t.voff : 1));
       const double per = ms / nb;
       if (bt == BT_CAND[0] || per < best_per) { best_per = per; best = bt; }
     }

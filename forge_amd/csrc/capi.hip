@@ -129,6 +129,8 @@ struct forge_ctx {
   z2* H = nullptr;
   void* trd_ws = nullptr;
   TridiagWork trd{};
+  void* ts_ws = nullptr;  // two-stage reduction workspace (Kb matrices), when allocated
+  TwoStageWork ts;
   void* dc_ws = nullptr;
   double *Zq = nullptr, *evals = nullptr;
   z2* Cv = nullptr;
@@ -213,6 +215,19 @@ extern "C" int forge_device_count(void) {
 static int dalloc(void** p, size_t bytes) {
   FORGE_HIP(hipMalloc(p, bytes ? bytes : 256));
   return 0;
+}
+
+// Reduction to tridiagonal form: 1 one-stage (tridiag.hip), 2 two-stage
+// (twostage.hip).  FORGE_REDUCTION=1 or 2 in the environment overrides the
+// request; a request of 0 picks two-stage from order TS_MIN_N on, where its
+// matrix-core products outrun the one-stage reduction's matrix-vector
+// products.
+constexpr i64 TS_MIN_N = 1024;
+static int reduction_mode(int requested, i64 n) {
+  const char* env = getenv("FORGE_REDUCTION");
+  if (env && (env[0] == '1' || env[0] == '2') && env[1] == 0) return env[0] - '0';
+  if (requested == 1 || requested == 2) return requested;
+  return n >= TS_MIN_N ? 2 : 1;
 }
 
 extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, const double* tn,
@@ -336,8 +351,10 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
     size_t fr = 0, tot = 0;
     FORGE_HIP(hipMemGetInfo(&fr, &tot));
     const i64 nU = c->nU;
+    const bool two_stage = reduction_mode(0, n) == 2;
     size_t per_k = sizeof(z2) * pad_ld(n) * n + tridiag_workspace_bytes(n, 1) + dc_workspace_bytes(n, 1, nU) +
-                   sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU);
+                   sizeof(double) * (n * nU + n) + sizeof(z2) * n * nU + backtr_workspace_bytes(n, 1, nU) +
+                   (two_stage ? twostage_workspace_bytes(n, 1) : 0);
     size_t fixed_extra = sizeof(z2) * n * c->nb * Nk * 2 + ((size_t)64 << 20);
     size_t budget = fr > fixed_extra ? (size_t)((fr - fixed_extra) * 0.85) : 0;
     i64 Kb = (i64)(budget / per_k);
@@ -354,6 +371,10 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
     c->trd.A = c->H;
     c->trd.sA = (size_t)c->ldH * n;
     c->trd.lda = c->ldH;
+    if (two_stage) {
+      CK(dalloc(&c->ts_ws, twostage_workspace_bytes(n, Kb_)));
+      CK(twostage_bind(c->ts, c->ts_ws, n, Kb_));
+    }
     CK(dalloc(&c->dc_ws, dc_workspace_bytes(n, Kb_, nU)));
     CK(dalloc((void**)&c->Zq, sizeof(double) * n * nU * Kb_));
     CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
@@ -386,7 +407,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
   // teardown (also of a partly built context): errors here leave nothing
   // for the caller to act on
   if (c->s) (void)hipStreamSynchronize(c->s);
-  void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws,
+  void* ptrs[] = {c->X, c->V, c->T, c->L, c->bias, c->pot, c->rho, c->rho_new, c->H, c->trd_ws, c->ts_ws,
                   c->dc_ws, c->Zq, c->evals, c->Cv, c->bt_ws, c->Cwin, c->ph_asm, c->ph_cell,
                   c->Psi, c->wph, c->gidx, c->part, c->e5, c->dots, c->da, c->db, c->ya, c->yb,
                   c->S, c->c3_perm, c->c3_tab, c->c3_ph_asm, c->c3_dexpand, c->c3_dpsi,
@@ -478,6 +499,7 @@ static bool mixed_phase(const forge_ctx* c) {
 static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   const i64 n = c->n;
   c->trd.K = K;
+  c->trd.ts = (reduction_mode(c->opt.reduction, n) == 2 && c->ts_ws) ? &c->ts : nullptr;
   const bool mixed = mixed_phase(c);
   c->trd.S = mixed ? c->S : nullptr;
   c->trd.use_graph = c->opt.graph == 1;
@@ -536,12 +558,17 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   hipStream_t s = nullptr;
   FORGE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   z2* H = nullptr; void* tw = nullptr; void* dw = nullptr; double* Zq = nullptr; double* ev = nullptr;
-  z2* Cv = nullptr; void* bw = nullptr;
+  z2* Cv = nullptr; void* bw = nullptr; void* tsw = nullptr;
   TridiagWork t{};
+  TwoStageWork ts;
   const int rc = [&]() -> int {
     const i64 ld = pad_ld(n);
     CK(dalloc((void**)&H, sizeof(z2) * ld * n));
     CK(dalloc(&tw, tridiag_workspace_bytes(n, 1)));
+    if (reduction_mode(0, n) == 2) {
+      CK(dalloc(&tsw, twostage_workspace_bytes(n, 1)));
+      CK(twostage_bind(ts, tsw, n, 1));
+    }
     CK(dalloc(&dw, dc_workspace_bytes(n, 1, iu)));
     CK(dalloc((void**)&Zq, sizeof(double) * n * iu));
     CK(dalloc((void**)&ev, sizeof(double) * n));
@@ -551,6 +578,7 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
     t.A = H;
     t.sA = (size_t)ld * n;
     t.lda = ld;
+    if (tsw) t.ts = &ts;
     FORGE_HIP(hipMemcpy2DAsync(H, sizeof(z2) * ld, A, sizeof(z2) * n, sizeof(z2) * n, n,
                                hipMemcpyHostToDevice, s));
     CK(tridiag(t, s, nullptr, nullptr, nullptr));
@@ -568,7 +596,7 @@ extern "C" int forge_zdiagred(forge_z* A, int64_t n, double* eigenvalues, int64_
   }();
   // teardown: errors here leave nothing for the caller to act on
   (void)hipStreamSynchronize(s);
-  for (void* q : {(void*)H, tw, dw, (void*)Zq, (void*)ev, (void*)Cv, bw})
+  for (void* q : {(void*)H, tw, dw, (void*)Zq, (void*)ev, (void*)Cv, bw, tsw})
     if (q) (void)hipFree(q);
   tridiag_release(t);
   for (i64 q = 0; q < t.nev; ++q) (void)hipEventDestroy(t.evs[q]);
@@ -1473,6 +1501,21 @@ extern "C" int forge_scf_set_options(forge_ctx* c, const forge_scf_options* opt)
       FORGE_HIP(hipGraphExecDestroy(g));
       g = nullptr;
     }
+  if (opt->reduction < 0 || opt->reduction > 2) {
+    set_error("forge_scf_set_options: reduction must be 0 (by order), 1 (one-stage) or 2 (two-stage)");
+    return -2;
+  }
+  if (reduction_mode(opt->reduction, c->n) == 2 && !c->ts_ws) {
+    const size_t bytes = twostage_workspace_bytes(c->n, c->Kb);
+    size_t fr = 0, tot = 0;
+    FORGE_HIP(hipMemGetInfo(&fr, &tot));
+    if (bytes + ((size_t)256 << 20) > fr) {
+      set_error("forge_scf_set_options: not enough device memory for the two-stage reduction");
+      return -4;
+    }
+    CK(dalloc(&c->ts_ws, bytes));
+    CK(twostage_bind(c->ts, c->ts_ws, c->n, c->Kb));
+  }
   if (opt->precision == 1 && !c->S) {
     const size_t bytes = sizeof(float2) * (size_t)c->ldH * c->n * c->Kb;
     size_t fr = 0, tot = 0;

@@ -35,30 +35,9 @@
 // a contiguous range of tiles, and tiles are ordered in groups of 8 tile
 // rows, so the blocks resident on one XCD share operand panels in its L2.
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace forge {
-
-typedef double gf64x4 __attribute__((ext_vector_type(4)));
-typedef float gf32x4 __attribute__((ext_vector_type(4)));
-
-// MFMA 16 x 16 x 4 per scalar type: inputs one value per lane, A(row lane %
-// 16, k lane / 16), B(k lane / 16, column lane % 16); output four values per
-// lane in column lane % 16 at rows row(lane, i).
-template <typename S> struct Mfma;
-template <> struct Mfma<double> {
-  typedef gf64x4 acc;
-  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int i) { return (lane >> 4) + 4 * i; }
-};
-template <> struct Mfma<float> {
-  typedef gf32x4 acc;
-  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
-};
 
 // storage element of a scalar type, real or complex
 template <typename S, bool CPLX> struct Elem;

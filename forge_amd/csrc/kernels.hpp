@@ -1,5 +1,7 @@
 // Internal kernel entry points (host wrappers) of libforge_hip.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace forge {
@@ -88,6 +90,15 @@ struct TridiagWork {
   // replays it while the shape is unchanged (untimed calls only)
   int use_graph;
   void* graph;  // the captured graph and its shape; tridiag_release frees it
+  // Two-stage reduction (twostage.hip): non-null ts and no first_col make
+  // tridiag() reduce to a band of width TS_KD first and chase the band to
+  // tridiagonal form.  The reflectors of the first stage then have their
+  // unit element TS_KD rows below their column (voff = TS_KD; the one-stage
+  // reflectors: voff = 1), and the back-transformation applies the chase's
+  // reflectors (ts) before them.  Set by every reduction.
+  struct TwoStageWork* ts;
+  int voff;
+  int two_stage;
 };
 constexpr int TRD_NB = 32;  // panel width (a multiple of 8, at most 128; 64 measured slower)
 constexpr int TRD_TS = 32;     // hemv column-strip width
@@ -107,6 +118,47 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
 // synchronisation, no graph of its own)
 int tridiag_enqueue(TridiagWork& w, hipStream_t s);
 void tridiag_release(TridiagWork& w);
+
+// twostage.hip: the two-stage reduction.  Stage 1 (dense -> band of width
+// TS_KD): per panel of TS_KD columns a Householder QR of the panel below its
+// diagonal block, then A22 <- Q^H A22 Q as one matrix-core product of the
+// trailing matrix with the panel's reflectors (A22 V) and a rank-2 TS_KD
+// update.  Stage 2 (band -> tridiagonal): bulge chasing in band storage, one
+// launch per slot of independent steps.  Its reflectors are kept in groups
+// of TS_NB consecutive sweeps for the back-transformation.
+constexpr int TS_KD = 32;             // band width = panel width
+constexpr int TS_NB = 32;             // sweeps per group of the chase's back-transformation
+constexpr int TS_LDAB = 2 * TS_KD + 1;  // band storage: column c holds A(c .. c + 2 TS_KD, c)
+constexpr int TS_YSPLIT = 8;          // maximum column splits of the panel product A22 V
+struct TwoStageWork {
+  i64 n = 0;
+  int K = 0;
+  z2* V = nullptr;      // K x n x KD: the panel's reflectors (unit diagonal, zeros above), rows global
+  z2* X = nullptr;      // K x n x KD: A22 V T
+  z2* Y = nullptr;      // K x YSPLIT x n x KD: partial A22 V per column split
+  z2* qpart = nullptr;  // K x nbq x KD: v^H A partials of the panel QR
+  z2* gpart = nullptr;  // K x nbq x KD x KD: V^H V partials
+  double* npart = nullptr;  // K x nbq: column-norm partials
+  z2* T = nullptr;      // K x KD x KD: the panel's triangular factor
+  z2* Zm = nullptr;     // K x KD x KD: T^H V^H X
+  z2* mpart = nullptr;  // K x nbx x KD x KD: V^H X partials
+  z2* AB = nullptr;     // K x n x TS_LDAB: the band
+  z2* V2 = nullptr;     // K x nblk2 x NB x KD: chase reflectors by (group, step)
+  z2* T2 = nullptr;     // K x nblk2 x NB x NB: triangular factor of each block
+  z2* tau2 = nullptr;   // K x nblk2 x NB
+  i64* goff = nullptr;  // device: first block of each group (G + 1 entries)
+  std::vector<i64> hoff;  // host copy
+  i64 G = 0, nblk2 = 0;
+};
+size_t twostage_workspace_bytes(i64 n, int K);
+int twostage_bind(TwoStageWork& ts, void* base, i64 n, int K);
+// the reduction's launches (no host synchronisation while capturing)
+int twostage_reduce(TridiagWork& w, hipStream_t s, bool capturing);
+// its trailing update on the one-stage her2k kernel (tridiag.hip)
+int her2k_twostage(TridiagWork& w, i64 p, i64 i0, int nbp, hipStream_t s);
+// C <- Q2 C (the chase's reflectors), C complex FP64 or FP32
+int twostage_apply_q2(const TridiagWork& t, i64 m, z2* C, size_t ldC, size_t sC, hipStream_t s);
+int twostage_apply_q2(const TridiagWork& t, i64 m, float2* C, size_t ldC, size_t sC, hipStream_t s);
 
 // dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
 struct DCWork;

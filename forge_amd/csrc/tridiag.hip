@@ -115,6 +115,9 @@ int tridiag_bind(TridiagWork& w, void* base, i64 n, int K) {
   w.first_col = nullptr;
   w.use_graph = 0;
   w.graph = nullptr;
+  w.ts = nullptr;
+  w.voff = 1;
+  w.two_stage = 0;
   return 0;
 }
 
@@ -682,9 +685,11 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
 }
 
 // ---------------------------------------------------------------- her2k
-// A(p:n, p:n) -= V W^H + W V^H on the lower triangle, V = A(p:n, i0:i0+nbp),
-// W = W(p:n, 0:nbp), whose last column still lacks its -tau/2 (w^H v) v term
-// (added here while staging); 64x64 output tiles, lower tiles only.
+// A(p:n, p:n) -= V W^H + W V^H on the lower triangle, W = W(p:n, 0:nbp),
+// V = V_(p:n, 0:nbp) with leading dimension ldv (the one-stage reduction:
+// A(p:n, i0:i0+nbp), whose W's last column still lacks its -tau/2 (w^H v) v
+// term, added here while staging when corr is set; the two-stage reduction:
+// its panel's reflectors, corr = 0); 64x64 output tiles, lower tiles only.
 // On the FP64 matrix cores (v_mfma_f64_16x16x4f64).  With
 // X = [Vr Vi Wr Wi] for the tile's rows and, for its columns,
 // Yre = [Wr Wi Vr Vi], Yim = [-Wi Wr -Vi Vr]:
@@ -701,13 +706,16 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, int nbp, z2* A_,
                                                         size_t sA, i64 lda, float2* S_,
                                                         size_t sS, const z2* W_, const z2* tau_,
-                                                        const z2* dpart_, i64 nbmax) {
+                                                        const z2* dpart_, i64 nbmax,
+                                                        const z2* V_, size_t sV, i64 ldv, int corr) {
   const int k = blockIdx.y;
   z2* A = A_ + k * sA;
   float2* S = S_ ? S_ + k * sS : nullptr;
   const z2* W = W_ + (size_t)k * n * NB;
+  const z2* Vm = V_ + k * sV;
   const i64 jl = i0 + nbp - 1;  // last reflector of the panel
-  const z2 alpha = alpha_of(n, jl, tau_[(size_t)k * n + jl], dpart_ + (size_t)k * nbmax);
+  const z2 alpha =
+      corr ? alpha_of(n, jl, tau_[(size_t)k * n + jl], dpart_ + (size_t)k * nbmax) : mk(0.0, 0.0);
   i64 t = blockIdx.x;
   i64 I = (i64)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
@@ -730,8 +738,8 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
   // one step ahead so the loads overlap the previous step's MFMAs
   const int rr = threadIdx.x & 63, qq0 = threadIdx.x >> 6;  // e = tid: column qq0, e = tid + 256: qq0 + 4
   const bool rin = r0 + rr < n, cin = c0 + rr < n;
-  const z2* Ar = A + (r0 + rr) + lda * i0;
-  const z2* Ac = A + (c0 + rr) + lda * i0;
+  const z2* Ar = Vm + (r0 + rr);
+  const z2* Ac = Vm + (c0 + rr);
   const z2* Wr = W + (r0 + rr);
   const z2* Wc = W + (c0 + rr);
   z2 sv[2][4];
@@ -739,8 +747,8 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
   _Pragma("unroll") for (int h = 0; h < 2; ++h) {                             \
     const int q = (Q0) + qq0 + 4 * h;                                         \
     const bool qv = q < nbp;                                                  \
-    sv[h][0] = (qv && rin) ? Ar[lda * q] : zero;                              \
-    sv[h][1] = (qv && cin) ? Ac[lda * q] : zero;                              \
+    sv[h][0] = (qv && rin) ? Ar[ldv * q] : zero;                              \
+    sv[h][1] = (qv && cin) ? Ac[ldv * q] : zero;                              \
     sv[h][2] = (qv && rin) ? Wr[n * q] : zero;                                \
     sv[h][3] = (qv && cin) ? Wc[n * q] : zero;                                \
   }
@@ -752,7 +760,7 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
       const int qq = qq0 + 4 * h, q = q0 + qq;
       const z2 vr = sv[h][0], vc = sv[h][1];
       z2 wr = sv[h][2], wc = sv[h][3];
-      if (q == nbp - 1) {
+      if (corr && q == nbp - 1) {
         wr = wr + alpha * vr;
         wc = wc + alpha * vc;
       }
@@ -803,6 +811,18 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
       }
 }
 #undef HER2K_LOAD_STEP
+
+// The two-stage reduction's trailing update: A(p:n, p:n) -= V W^H + W V^H with
+// V its panel's reflectors (w.ts->V, leading dimension n) and W in w.W.
+int her2k_twostage(TridiagWork& w, i64 p, i64 i0, int nbp, hipStream_t s) {
+  const i64 mt = w.n - p, ntt = (mt + 63) / 64;
+  if (mt <= 0 || nbp <= 0) return 0;
+  hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), w.K), dim3(256), 0, s,
+                     w.n, p, i0, nbp, w.A, w.sA, w.lda, (float2*)nullptr, (size_t)0, w.W, w.tau,
+                     w.dpart, npart_stride(w.n), w.ts->V, (size_t)w.n * TS_KD, w.n, 0);
+  FORGE_LAUNCH();
+  return 0;
+}
 
 // S = fp32(A) on the lower triangle (column c, rows c..n-1); grid
 // (cdiv(n, 256), ny, K), columns blockIdx.y, blockIdx.y + ny, ...
@@ -1039,6 +1059,9 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
                              double* hemv_bytes, bool capturing) {
   const i64 n = w.n;
   const int K = w.K;
+  w.two_stage = 0;
+  w.voff = 1;
+  if (w.ts && !w.first_col) return twostage_reduce(w, s, capturing);
   const i64 ntmax = nt_of(n), msmax = nseg_of(n, 0), nbmax = npart_stride(n);
   const bool timed = hemv_ms != nullptr;
   if (timed && w.nev == 0) {
@@ -1110,7 +1133,8 @@ static int enqueue_reduction(TridiagWork& w, hipStream_t s, double* hemv_ms, lon
     const i64 p = i0 + nbp, mt = n - p, ntt = (mt + 63) / 64;
     if (Kj == 0) continue;
     hipLaunchKernelGGL(k_trd_her2k_mfma, dim3((unsigned)(ntt * (ntt + 1) / 2), Kj), dim3(256), 0, s,
-                       n, p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax);
+                       n, p, i0, nbp, w.A, w.sA, w.lda, w.S, w.sS, w.W, w.tau, w.dpart, nbmax,
+                       w.A + w.lda * i0, w.sA, w.lda, 1);
     FORGE_LAUNCH();
     // bound the launches in flight (about 400): the host otherwise runs up to
     // 3 n launches ahead, which rocprofv3's dispatch tracing does not survive
@@ -1137,11 +1161,11 @@ struct TrdGraph {
   hipGraphExec_t exec = nullptr;
   i64 n = 0, lda = 0;
   int K = 0;
-  const void *A = nullptr, *S = nullptr, *W = nullptr;
+  const void *A = nullptr, *S = nullptr, *W = nullptr, *ts = nullptr;
   size_t sA = 0, sS = 0;
   std::vector<i64> first;
   bool same(const TridiagWork& w) const {
-    if (n != w.n || K != w.K || lda != w.lda || A != w.A || S != w.S || W != w.W || sA != w.sA ||
+    if (n != w.n || K != w.K || lda != w.lda || A != w.A || S != w.S || W != w.W || ts != w.ts || sA != w.sA ||
         sS != w.sS || first.size() != (w.first_col ? (size_t)w.K : 0))
       return false;
     for (size_t q = 0; q < first.size(); ++q)
@@ -1168,7 +1192,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
   auto* g = (TrdGraph*)w.graph;
   if (!g || !g->same(w)) {
     tridiag_release(w);
-    {
+    if (!w.ts || w.first_col) {
       // the hemv plan is measured before the capture (it times with events)
       HemvPlan plan;
       const int r = hemv_plan(w, s, true, &plan);
@@ -1194,7 +1218,7 @@ int tridiag(TridiagWork& w, hipStream_t s, double* hemv_ms, long* hemv_launches,
       set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
       return -1;
     }
-    g->n = w.n; g->K = w.K; g->lda = w.lda; g->A = w.A; g->S = w.S; g->W = w.W;
+    g->n = w.n; g->K = w.K; g->lda = w.lda; g->A = w.A; g->S = w.S; g->W = w.W; g->ts = w.ts;
     g->sA = w.sA; g->sS = w.sS;
     if (w.first_col) g->first.assign(w.first_col, w.first_col + w.K);
     w.graph = g;

@@ -39,7 +39,7 @@ module forge_c
     integer(c_int32_t) :: precision = 0, timing = 0
     real(c_double) :: switch_de = 0.0_c_double
     integer(c_int32_t) :: graph = 0
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: reduction = 0
   end type forge_scf_options_t
 
   interface

@@ -268,8 +268,9 @@ class Engine:
 
     # -- device-resident SCF -----------------------------------------------
     def set_options(self, occ_mode=0, mix_mode=0, kT=0.0, beta=0.3, etol=0.0, precision=0,
-                    timing=0, switch_de=0.0, graph=0):
-        o = L.ScfOptions(occ_mode, mix_mode, kT, beta, etol, precision, timing, switch_de, graph, 0)
+                    timing=0, switch_de=0.0, graph=0, reduction=0):
+        o = L.ScfOptions(occ_mode, mix_mode, kT, beta, etol, precision, timing, switch_de, graph,
+                         reduction)
         L.check(self.lib.forge_scf_set_options(self.ctx, C.byref(o)), "forge_scf_set_options")
 
     def scf_init(self, rho_start=None):

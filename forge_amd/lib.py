@@ -33,7 +33,7 @@ class ForgeParams(C.Structure):
 class ScfOptions(C.Structure):
     _fields_ = [("occ_mode", i32), ("mix_mode", i32), ("kT", dbl), ("beta", dbl),
                 ("etol", dbl), ("precision", i32), ("timing", i32), ("switch_de", dbl),
-                ("graph", i32), ("reserved", i32)]
+                ("graph", i32), ("reduction", i32)]
 
 
 _SIGS = {

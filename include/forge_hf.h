@@ -62,7 +62,11 @@ typedef struct {
                          valley projection or symmetry enforcement, every k-point in one
                          eigensolver batch, nb * numk^2 <= 4096) without timing; the first
                          iteration of each phase of a solve runs on the host path. */
-  int32_t reserved;   /* 0 */
+  int32_t reduction;  /* reduction to tridiagonal form: 0 by order (two-stage from n = 1024 on),
+                         1 one-stage (tridiag.hip: a Householder matrix-vector product of the
+                         trailing matrix per column), 2 two-stage (twostage.hip: band of width
+                         32 by matrix-core products per panel, then bulge chasing); the
+                         environment variable FORGE_REDUCTION=1 or 2 overrides it */
 } forge_scf_options;
 
 typedef struct forge_ctx forge_ctx;
