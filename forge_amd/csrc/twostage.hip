@@ -160,11 +160,26 @@ __device__ inline double ts_block_sum256(double v, double* sh) {
   return r;
 }
 
-// A22(i, j) of the trailing matrix at r0 (local indices), from the lower triangle
-__device__ inline z2 herm_at(const z2* A, i64 lda, i64 r0, i64 i, i64 j) {
-  if (i > j) return A[(r0 + i) + lda * (r0 + j)];
-  if (i == j) return mk(A[(r0 + i) + lda * (r0 + i)].x, 0.0);
-  return conjz(A[(r0 + j) + lda * (r0 + i)]);
+// Loads below are issued unconditionally -- from `safe`, an element known to
+// hold a finite value, when the wanted one does not exist -- and the value is
+// scaled by 1 or 0 afterwards.  A load behind a per-element branch (or a
+// select the compiler turns back into one) makes it wait for each load before
+// issuing the next.
+__device__ inline z2 ld_or_zero(const z2* p, size_t off, bool ok, size_t safe) {
+  const z2 v = p[ok ? off : safe];
+  const double f = ok ? 1.0 : 0.0;
+  return mk(f * v.x, f * v.y);
+}
+
+// A22(i, j) of the trailing matrix at r0 (local indices), from the lower
+// triangle (mirrored and conjugated above the diagonal); zero unless ok
+__device__ inline z2 herm_at(const z2* A, i64 lda, i64 r0, i64 i, i64 j, bool ok) {
+  const bool up = i < j;
+  const i64 a = up ? j : i, b = up ? i : j;
+  z2 v = ld_or_zero(A, (size_t)((r0 + a) + lda * (r0 + b)), ok, (size_t)(r0 + lda * r0));
+  if (up) v.y = -v.y;
+  if (i == j) v.y = 0.0;
+  return v;
 }
 
 // ---------------------------------------------------------------- panel QR
@@ -176,8 +191,8 @@ __global__ void __launch_bounds__(256) k_ts_qnorm(i64 n, i64 i0, const z2* A_, s
   const i64 r0 = i0 + KD, mp = n - r0;
   const i64 rr = (i64)blockIdx.x * QROWS + threadIdx.x;
   __shared__ double sh[4];
-  double v = 0.0;
-  if (rr >= 1 && rr < mp) v = abs2(A[(r0 + rr) + lda * i0]);
+  double v = abs2(ld_or_zero(A, (size_t)((r0 + rr) + lda * i0), rr >= 1 && rr < mp,
+                             (size_t)(r0 + lda * i0)));
   v = ts_block_sum256(v, sh);
   if (threadIdx.x == 0) npart_[(size_t)k * nbqmax + blockIdx.x] = v;
 }
@@ -204,10 +219,10 @@ __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2
   const TsRefl R = Rs;
   const i64 rr = (i64)blockIdx.x * QROWS + threadIdx.x, r = r0 + rr;
   const bool live = rr < mp;
-  z2 v = mk(0.0, 0.0);
+  const i64 rs = live ? r : r0;  // a row inside the matrix for the unconditional loads
+  z2 v = ld_or_zero(A, (size_t)(rs + lda * c), live && rr > jj, (size_t)(r0 + lda * c)) * R.scale;
+  if (live && rr == jj) v = mk(1.0, 0.0);
   if (live) {
-    if (rr == jj) v = mk(1.0, 0.0);
-    else if (rr > jj) v = A[r + lda * c] * R.scale;
     V[r + n * jj] = v;
     if (rr > jj) A[r + lda * c] = v;
     if (rr == jj) {
@@ -223,11 +238,10 @@ __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll 4
   for (int q = 0; q < KD; ++q) {
-    z2 p = mk(0.0, 0.0);
-    if (live && q != jj) {
-      if (q < jj) p = cmul(V[r + n * q], v);
-      else if (rr >= jj) p = cmul(v, A[r + lda * (i0 + q)]);
-    }
+    // q < jj: V(r, q); q > jj: A(r, i0 + q) (v is zero on rows above jj)
+    const z2 a = q < jj ? V[rs + n * q] : A[rs + lda * (i0 + q)];
+    z2 p = q < jj ? cmul(a, v) : cmul(v, a);
+    if (!live || q == jj) p = mk(0.0, 0.0);
     double px = ts_wave_sum(p.x), py = ts_wave_sum(p.y);
     if (lane == 0) red[wv][q] = mk(px, py);
   }
@@ -270,12 +284,17 @@ __global__ void __launch_bounds__(256) k_ts_qb(i64 n, i64 i0, int jj, int nq, z2
   const z2 tb = conjz(tau_[(size_t)k * n + c]);
   const i64 rr = (i64)blockIdx.x * QROWS + threadIdx.x, r = r0 + rr;
   double nrm = 0.0;
-  if (rr < mp && rr >= jj) {
-    const z2 tv = tb * V[r + n * jj];
-#pragma unroll 4
+  const bool act = rr < mp && rr >= jj;
+  const i64 rs = act ? r : r0 + jj;
+  const z2 tv = tb * V[rs + n * jj];
+  z2 col[KD];
+#pragma unroll
+  for (int q = 0; q < KD; ++q) col[q] = A[rs + lda * (i0 + (q > jj ? q : jj))];
+  if (act) {
+#pragma unroll
     for (int q = 0; q < KD; ++q)
       if (q > jj) {
-        const z2 a = A[r + lda * (i0 + q)] - tv * wl[q];
+        const z2 a = col[q] - tv * wl[q];
         A[r + lda * (i0 + q)] = a;
         if (q == jj + 1 && rr >= jj + 2) nrm = abs2(a);
       }
@@ -349,13 +368,13 @@ __global__ void __launch_bounds__(256) k_ts_hemm(i64 n, i64 r0, i64 mp, int nq, 
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const i64 col = c0 + 4 * ks + lk;
-      af[ks] = (row < mp && col < ke) ? herm_at(A, lda, r0, row, col) : mk(0.0, 0.0);
+      af[ks] = herm_at(A, lda, r0, row, col, row < mp && col < ke);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int e = tid + 256 * h, vr = e % 16, vc = e / 16;
       const i64 gr = c0 + vr;
-      vf[h] = (gr < ke && vc < nq) ? V[(r0 + gr) + n * vc] : mk(0.0, 0.0);
+      vf[h] = ld_or_zero(V, (size_t)((r0 + gr) + n * vc), gr < ke && vc < nq, (size_t)r0);
     }
   };
   auto store_v = [&](int buf) {
@@ -420,10 +439,10 @@ __global__ void __launch_bounds__(256) k_ts_x(i64 n, i64 r0, i64 mp, int nsplit,
   for (int c = 0; c < 8; ++c) {
     const int col = 8 * cq + c;
     z2 y = mk(0.0, 0.0);
-    if (live)
-      for (int q = 0; q < nsplit; ++q)
-        y = y + Y_[((size_t)k * TS_YSPLIT + q) * n * KD + (r0 + r) + (size_t)n * col];
-    Bf[lr][col] = y;
+    const i64 rs = live ? r : 0;
+    for (int q = 0; q < nsplit; ++q)
+      y = y + Y_[((size_t)k * TS_YSPLIT + q) * n * KD + (r0 + rs) + (size_t)n * col];
+    Bf[lr][col] = live ? y : mk(0.0, 0.0);
   }
   __syncthreads();
   z2* X = X_ + (size_t)k * n * KD;
@@ -442,7 +461,7 @@ __global__ void __launch_bounds__(256) k_ts_x(i64 n, i64 r0, i64 mp, int nsplit,
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int col = 8 * cq + c;
-    Bf[lr][col] = live ? V[(r0 + r) + (size_t)n * col] : mk(0.0, 0.0);
+    Bf[lr][col] = ld_or_zero(V, (size_t)((r0 + r) + (size_t)n * col), live, (size_t)r0);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < KD * KD; e += 256) {
@@ -492,7 +511,7 @@ __global__ void __launch_bounds__(256) k_ts_w(i64 n, i64 r0, i64 mp, const z2* X
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int col = 8 * cq + c;
-    Vr[lr][col] = live ? V[(r0 + r) + (size_t)n * col] : mk(0.0, 0.0);
+    Vr[lr][col] = ld_or_zero(V, (size_t)((r0 + r) + (size_t)n * col), live, (size_t)r0);
   }
   __syncthreads();
   if (!live) return;
@@ -544,14 +563,16 @@ __global__ void __launch_bounds__(64) k_ts_chase(i64 n, i64 slot, i64 s_lo, i64 
 #pragma unroll
   for (int h = 0; h < 16; ++h) {
     const int e = l + 64 * h, i = e % KD, j = e / KD;
-    dv[h] = (i < L && j < L && i >= j) ? AB[(st + j) * LDAB + (i - j)] : mk(0.0, 0.0);
-    bv[h] = (i < lm && j < L) ? AB[(st + j) * LDAB + (KD + i - j)] : mk(0.0, 0.0);
+    dv[h] = ld_or_zero(AB, (size_t)((st + j) * LDAB + (i - j)), i < L && j < L && i >= j,
+                       (size_t)(st * LDAB));
+    bv[h] = ld_or_zero(AB, (size_t)((st + j) * LDAB + (KD + i - j)), i < lm && j < L,
+                       (size_t)(st * LDAB));
   }
   z2* V2 = V2_ + ((size_t)k * nblk2 + goff[g]) * NBG * KD;
   z2* tau2 = tau2_ + ((size_t)k * nblk2 + goff[g]) * NBG;
   if (t == 0) {
     // H_{s,0} from column s, rows st .. st + L - 1 (band offsets 1 .. L)
-    const z2 x = (l >= 1 && l < L) ? AB[s * LDAB + 1 + l] : mk(0.0, 0.0);
+    const z2 x = ld_or_zero(AB, (size_t)(s * LDAB + 1 + l), l >= 1 && l < L, (size_t)(s * LDAB));
     const double xn2 = ts_wave_sum(abs2(x));
     const z2 alpha = AB[s * LDAB + 1];
     const TsRefl R = ts_reflector(alpha, xn2);
@@ -732,9 +753,13 @@ __global__ void __launch_bounds__(64) k_ts_t2(i64 n, i64 G, const i64* goff, con
 //   C -= V W'
 // run on the accumulators directly, skipping the 16 x 16 blocks where V or T
 // is zero.  After each block the top 32 rows are final for the group and are
-// stored; the window moves down by KD.
+// stored; the window moves down by KD.  The next block's reflectors, its
+// triangular factor and the 32 rows of C it adds are loaded into registers
+// before the current block's products, so their latency overlaps them.  V's
+// structural zeros (entries outside the parallelogram) are written to LDS
+// once; each block overwrites only the KD x NB entries of its reflectors.
 template <typename S, typename E>
-__global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i64 m, E* C_,
+__global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i64 m, E* C_,
                                                size_t ldC, size_t sC, const z2* V2_, const z2* T2_,
                                                i64 nblk2) {
   typedef Mfma<S> MF;
@@ -746,16 +771,55 @@ __global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i6
   const bool cin = col < m;
   __shared__ S Vs[2][64][NBG + 1];
   __shared__ S Ts[2][NBG][NBG + 1];
+  for (int e = tid; e < 64 * NBG; e += 256) {
+    Vs[0][e % 64][e / 64] = (S)0;
+    Vs[1][e % 64][e / 64] = (S)0;
+  }
   ACC cre[4], cim[4];
   i64 base = g * NBG + 1;
-  auto load_rb = [&](int rb) {
+  // prefetch registers: 4 reflector entries, 4 entries of T, 2 x 4 elements of C
+  z2 pv[4], pt[4];
+  bool pok[4];
+  E pc[2][4];
+  auto fetch_vt = [&](i64 tt) {
+    const z2* V2 = V2_ + ((size_t)k * nblk2 + b0 + tt) * NBG * KD;
+    const z2* T2 = T2_ + ((size_t)k * nblk2 + b0 + tt) * NBG * NBG;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int e = tid + 256 * h, i = e / KD;  // reflector i, entry e % KD
+      const i64 s = g * NBG + i;
+      pok[h] = (s <= n - 2) & (tt < ts_nsteps(n, s));
+      pv[h] = V2[pok[h] ? e : 0];
+      pt[h] = T2[e];
+    }
+  };
+  auto stage_vt = [&]() {
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int e = tid + 256 * h, i = e / KD, r = i + e % KD;
+      const double f = pok[h] ? 1.0 : 0.0;
+      Vs[0][r][i] = (S)(f * pv[h].x);
+      Vs[1][r][i] = (S)(f * pv[h].y);
+      Ts[0][e % NBG][e / NBG] = (S)pt[h].x;
+      Ts[1][e % NBG][e / NBG] = (S)pt[h].y;
+    }
+  };
+  // rows of row block rb of the window at bs (out-of-range elements read C[0]
+  // and are zeroed when taken into the accumulators)
+  auto fetch_rows = [&](E* dst, i64 bs, int rb) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const i64 r = base + 16 * rb + MF::row(lane, i);
-      z2 v = mk(0.0, 0.0);
-      if (cin && r < n) v = widen(C[r + ldC * col]);
-      cre[rb][i] = (S)v.x;
-      cim[rb][i] = (S)v.y;
+      const i64 r = bs + 16 * rb + MF::row(lane, i);
+      dst[i] = C[(cin && r < n) ? r + ldC * col : 0];
+    }
+  };
+  auto take_rows = [&](const E* src, i64 bs, int rb) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const i64 r = bs + 16 * rb + MF::row(lane, i);
+      const S f = (cin && r < n) ? (S)1 : (S)0;
+      cre[rb][i] = f * (S)src[i].x;
+      cim[rb][i] = f * (S)src[i].y;
     }
   };
   auto store_rb = [&](int rb) {
@@ -765,34 +829,24 @@ __global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i6
       if (cin && r < n) C[r + ldC * col] = narrow<E>(mk((double)cre[rb][i], (double)cim[rb][i]));
     }
   };
+  fetch_vt(0);
+  {
+    E w0[4][4];
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb) load_rb(rb);
+    for (int rb = 0; rb < 4; ++rb) fetch_rows(w0[rb], base, rb);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) take_rows(w0[rb], base, rb);
+  }
+  __syncthreads();  // the zeros of Vs are written
+  stage_vt();
+  __syncthreads();
   for (i64 t = 0; t < cnt; ++t, base += KD) {
-    if (t > 0) {
-      cre[0] = cre[2]; cim[0] = cim[2];
-      cre[1] = cre[3]; cim[1] = cim[3];
-      load_rb(2);
-      load_rb(3);
+    const bool more = t + 1 < cnt;
+    if (more) {
+      fetch_vt(t + 1);
+      fetch_rows(pc[0], base + KD, 2);
+      fetch_rows(pc[1], base + KD, 3);
     }
-    const z2* V2 = V2_ + ((size_t)k * nblk2 + b0 + t) * NBG * KD;
-    const z2* T2 = T2_ + ((size_t)k * nblk2 + b0 + t) * NBG * NBG;
-#pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      const int e = tid + 256 * h, r = e % 64, i = e / 64;
-      const i64 s = g * NBG + i;
-      z2 v = mk(0.0, 0.0);
-      if (r >= i && r - i < KD && s <= n - 2 && t < ts_nsteps(n, s)) v = V2[i * KD + (r - i)];
-      Vs[0][r][i] = (S)v.x;
-      Vs[1][r][i] = (S)v.y;
-    }
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int e = tid + 256 * h;
-      const z2 v = T2[e];
-      Ts[0][e % NBG][e / NBG] = (S)v.x;
-      Ts[1][e % NBG][e / NBG] = (S)v.y;
-    }
-    __syncthreads();
     // W = V^H C: reflector block mb meets row blocks mb .. mb + 2
     ACC wre[2], wim[2];
 #pragma unroll
@@ -808,8 +862,8 @@ __global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i6
           const S ar = Vs[0][rr][16 * mb + l16], ai = -Vs[1][rr][16 * mb + l16];
           const S br = cre[rb][i], bi = cim[rb][i];
           wre[mb] = MF::mma(ar, br, wre[mb]);
-          wre[mb] = MF::mma(-ai, bi, wre[mb]);
           wim[mb] = MF::mma(ar, bi, wim[mb]);
+          wre[mb] = MF::mma(-ai, bi, wre[mb]);
           wim[mb] = MF::mma(ai, br, wim[mb]);
         }
       }
@@ -828,8 +882,8 @@ __global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i6
           const S ar = Ts[0][16 * mo + l16][kk], ai = Ts[1][16 * mo + l16][kk];
           const S br = wre[mi][i], bi = wim[mi][i];
           xre[mo] = MF::mma(ar, br, xre[mo]);
-          xre[mo] = MF::mma(-ai, bi, xre[mo]);
           xim[mo] = MF::mma(ar, bi, xim[mo]);
+          xre[mo] = MF::mma(-ai, bi, xre[mo]);
           xim[mo] = MF::mma(ai, br, xim[mo]);
         }
     }
@@ -845,15 +899,22 @@ __global__ void __launch_bounds__(256) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0, i6
           const S ar = -Vs[0][16 * rb + l16][kk], ai = -Vs[1][16 * rb + l16][kk];
           const S br = xre[mb][i], bi = xim[mb][i];
           cre[rb] = MF::mma(ar, br, cre[rb]);
-          cre[rb] = MF::mma(-ai, bi, cre[rb]);
           cim[rb] = MF::mma(ar, bi, cim[rb]);
+          cre[rb] = MF::mma(-ai, bi, cre[rb]);
           cim[rb] = MF::mma(ai, br, cim[rb]);
         }
       }
-    __syncthreads();
+    __syncthreads();  // every wave is done with this block's V and T
     store_rb(0);
     store_rb(1);
-    if (t == cnt - 1) {
+    if (more) {
+      stage_vt();
+      cre[0] = cre[2]; cim[0] = cim[2];
+      cre[1] = cre[3]; cim[1] = cim[3];
+      take_rows(pc[0], base + KD, 2);
+      take_rows(pc[1], base + KD, 3);
+      __syncthreads();
+    } else {
       store_rb(2);
       store_rb(3);
     }

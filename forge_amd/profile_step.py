@@ -3,6 +3,7 @@ start and N SCF iterations of one system, with no timing of its own.
 
     python -m forge_amd.profile_step [--ntheta 31] [--nlayers 2] [--numk 3]
         [--nfilling 1] [--iters 1] [--precision 0|1] [--stages] [--graph] [--graph-mode N] [--c3]
+        [--reduction 0|1|2]
 
 Run under rocprofv3 (kernel trace, counters) to attribute device time and
 HBM traffic to kernels for exactly the work of N iterations.
@@ -32,6 +33,8 @@ def main():
     ap.add_argument("--graph-mode", type=int, default=None,
                     help="forge_scf_options.graph: 1 reduction graph, 2 device iteration, "
                          "3 iteration graph")
+    ap.add_argument("--reduction", type=int, default=0,
+                    help="forge_scf_options.reduction: 0 by order, 1 one-stage, 2 two-stage")
     ap.add_argument("--c3", action="store_true",
                     help="C3-reduced k-set (nenforceC3 = 1): one eigensolve per C3 orbit")
     a = ap.parse_args()
@@ -40,7 +43,7 @@ def main():
                      xi_nm=40.0))
     gm = a.graph_mode if a.graph_mode is not None else (1 if a.graph else 0)
     e.set_options(occ_mode=2, mix_mode=1, kT=0.001, beta=0.3, precision=a.precision,
-                  switch_de=1e-12, timing=1 if a.stages else 0, graph=gm)
+                  switch_de=1e-12, timing=1 if a.stages else 0, graph=gm, reduction=a.reduction)
     if a.c3:
         e.set_c3(True)
     e.scf_init(seeded_rho(e.n, e.nnc, a.nfilling))
