@@ -54,6 +54,12 @@ PLAIN_RECORD = {
 # measured on tbg_1.05deg_n11908_k3 for the FP64 matrix (16 B per element) and
 # the FP32 copy (8 B per element)
 HEMV_TRAFFIC_RATIO = {("tbg_1.05deg_n11908_k3", 16): 1.034, ("tbg_1.05deg_n11908_k3", 8): 1.087}
+# HBM traffic of k_ts_q2 (the chase's back-transformation) per launch over the
+# bytes of the eigenvector rows a launch passes over, read and written once
+# (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel of known size),
+# for complex FP64 (16 B) and complex FP32 (8 B) eigenvectors
+Q2_TRAFFIC_RATIO = {}
+MFMA_PEAK_TFS = {16: 78.6, 8: 157.3}  # dense FP64 and FP32 matrix-core peaks of MI355X
 MODEL = os.path.join(REPO, "forge_amd", "data", "guess_mlp.npz")
 
 WORKLOADS = {
@@ -197,6 +203,28 @@ def roofline(stage, kernel, bytes_per_elem, workload):
             "bytes_per_matrix_element": bytes_per_elem}
 
 
+def roofline_q2(stage, elem_bytes, workload):
+    """Sampled launches of k_ts_q2 (every 4th group of sweeps, HIP events on the
+    dispatch): their summed algorithmic flops (16 per reflector element per
+    eigenvector column) over their summed duration, against the matrix-core
+    peak of the precision the eigenvectors are held in."""
+    launches, ms, flops, byts = int(stage[15]), stage[16], stage[17], stage[18]
+    if launches == 0:
+        return None
+    achieved = flops / (ms * 1e-3) / 1e12
+    peak = MFMA_PEAK_TFS[elem_bytes]
+    ratio = Q2_TRAFFIC_RATIO.get((workload, elem_bytes))
+    return {"kernel": "k_ts_q2<%s> (the chase's reflectors applied to the eigenvectors)"
+                      % ("FP32" if elem_bytes == 8 else "FP64"),
+            "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": round(ratio * byts / launches) if ratio else None,
+            "traffic_over_eigenvector_bytes": ratio,
+            "timed_launches": launches, "avg_launch_us": round(ms / launches * 1e3, 2),
+            "algorithmic_flops_per_launch": round(flops / launches),
+            "eigenvector_bytes_per_launch": round(byts / launches)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -226,7 +254,7 @@ def main():
     def iterate(tag, count, stop_on_conv):
         """count single-iteration runs; stage timings summed separately for
         mixed-phase and FP64 iterations"""
-        rows, tm = [], {"mixed": np.zeros(16), "fp64": np.zeros(16)}
+        rows, tm = [], {"mixed": np.zeros(24), "fp64": np.zeros(24)}
         for _ in range(count):
             t0 = time.perf_counter()
             r = eng.scf_run(1)
@@ -319,13 +347,21 @@ def main():
         stage_ms[ph] = {name: round(st[q] / nph[ph], 1) for q, name in enumerate(
             ("assembly", "tridiagonal_reduction", "tridiagonal_eigensolver", "back_transformation",
              "density_products", "energies_mixing"))}
+    def dominant(ph):
+        # two-stage reduction (the default from order 1024 on): the chase's
+        # back-transformation; one-stage: the Householder hemv
+        if stage[ph][15] > 0:
+            return roofline_q2(stage[ph], 8 if ph == "mixed" else 16, args.workload)
+        if ph == "mixed":
+            return roofline(stage[ph], "k_trd_hemv<FP32 copy> (mixed-phase iterations of the solve)",
+                            8, args.workload)
+        return roofline(stage[ph], "k_trd_hemv<FP64>", 16, args.workload)
+
     if nph["mixed"] > 0:
-        rf = roofline(stage["mixed"], "k_trd_hemv<FP32 copy> (mixed-phase iterations of the solve)", 8,
-                      args.workload)
-        extra["roofline_fp64_iterations"] = roofline(stage["fp64"], "k_trd_hemv<FP64>", 16,
-                                                     args.workload) if nph["fp64"] else None
+        rf = dominant("mixed")
+        extra["roofline_fp64_iterations"] = dominant("fp64") if nph["fp64"] else None
     else:
-        rf = roofline(stage["fp64"], "k_trd_hemv<FP64>", 16, args.workload)
+        rf = dominant("fp64")
     out = {
         "metric": "wall-time to SCF converged 1e-6 eV, 1.05deg TBG; mixed+learned vs plain FP64 GPU",
         "value": round(wall, 3),

@@ -153,7 +153,7 @@ struct forge_ctx {
   double e[4] = {0, 0, 0, 0}, Etot = 0;
   std::vector<double> h_da;
   int it = 0;
-  double tm[16] = {};
+  double tm[FORGE_NTIMINGS] = {};
   hipEvent_t ev[16] = {};
   // C3-reduced k-set (nenforceC3 = 1): tables built by c3_configure from
   // RotateLayers and nC3pairs; c3 = the SCF loop uses Solve_C3 / GetFock_C3
@@ -418,6 +418,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   tridiag_release(c->trd);
+  twostage_release_events(c->ts);
   for (i64 q = 0; q < c->trd.nev; ++q) (void)hipEventDestroy(c->trd.evs[q]);
   delete[] c->trd.evs;
   for (int q = 0; q < 16; ++q)
@@ -519,8 +520,11 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
   // mixed phase: the FP32 copy is free once the reduction is done and holds
   // the FP32 eigenvectors (n x mout x K <= ldH x n x K)
   int width = 0;
-  CK(backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n, (size_t)n * c->nU, c->bt_ws,
-                   c->s, mixed ? c->S : nullptr, &width));
+  if (c->trd.ts) c->trd.ts->q2_timed = timed && c->trd.two_stage;
+  const int btr = backtransform(c->trd, c->Zq, n, (size_t)n * mout, mout, c->Cv, n,
+                                (size_t)n * c->nU, c->bt_ws, c->s, mixed ? c->S : nullptr, &width);
+  if (c->trd.ts) c->trd.ts->q2_timed = false;
+  CK(btr);
   c->tm[13] = width;
   {
     // the hemv variants of the last reduction, one decimal digit per band of
@@ -543,6 +547,15 @@ static int eig_batch(forge_ctx* c, int K, i64 mout, bool timed) {
     c->tm[9] += hemv_n;
     c->tm[10] += hemv_ms;
     c->tm[11] += hemv_bytes;
+    if (c->trd.ts && c->trd.two_stage) {
+      long q2n = 0;
+      double q2ms = 0.0, q2fl = 0.0, q2b = 0.0;
+      CK(twostage_q2_timing(*c->trd.ts, &q2n, &q2ms, &q2fl, &q2b));
+      c->tm[15] += q2n;
+      c->tm[16] += q2ms;
+      c->tm[17] += q2fl;
+      c->tm[18] += q2b;
+    }
   }
   return fails;
 }
@@ -1644,7 +1657,7 @@ extern "C" int forge_scf_init(forge_ctx* c, const forge_z* rho_start) {
   if (!c) { set_error("forge_scf_init: null context"); return -2; }
   const i64 n = c->n, nnc = c->nnc;
   const size_t nn = (size_t)n * n;
-  std::fill(c->tm, c->tm + 16, 0.0);
+  std::fill(c->tm, c->tm + FORGE_NTIMINGS, 0.0);
   c->mixed_done = false;
   int fails = 0;
   if (rho_start) {
@@ -1728,7 +1741,7 @@ static int apply_guess(forge_ctx* c, const char* who, const double* diag, int64_
       set_error(std::string(who) + ": entry out of range");
       return -2;
     }
-  std::fill(c->tm, c->tm + 16, 0.0);
+  std::fill(c->tm, c->tm + FORGE_NTIMINGS, 0.0);
   c->mixed_done = false;
   double* dd = nullptr;
   i64* ix = nullptr;
@@ -1777,7 +1790,7 @@ extern "C" int forge_scf_init_guess(forge_ctx* c, const forge_z* rho_guess) {
   if (!c || !rho_guess) { set_error("forge_scf_init_guess: null argument"); return -2; }
   const i64 n = c->n, nnc = c->nnc;
   const size_t nn = (size_t)n * n;
-  std::fill(c->tm, c->tm + 16, 0.0);
+  std::fill(c->tm, c->tm + FORGE_NTIMINGS, 0.0);
   c->mixed_done = false;
   // H[rho_guess]: its potential, then one diagonalisation at every k
   FORGE_HIP(hipMemcpyAsync(c->rho, rho_guess, sizeof(z2) * nnc * nn, hipMemcpyHostToDevice, c->s));
@@ -2013,7 +2026,7 @@ static int scf_step_dev(forge_ctx* c, double* traj_row, bool* stop) {
 extern "C" int forge_scf_run(forge_ctx* c, int64_t max_iter, double* traj, int64_t* iters_done) {
   if (!c || max_iter < 0) { set_error("forge_scf_run: invalid argument"); return -2; }
   if (!c->scf_ready) { set_error("forge_scf_run: call forge_scf_init first"); return -5; }
-  std::fill(c->tm, c->tm + 16, 0.0);
+  std::fill(c->tm, c->tm + FORGE_NTIMINGS, 0.0);
   const bool timed = c->opt.timing != 0;
   int fails = 0;
   i64 done = 0;
@@ -2049,9 +2062,9 @@ extern "C" int forge_scf_get_density(forge_ctx* c, forge_z* rho) {
   return 0;
 }
 
-extern "C" int forge_scf_timings(forge_ctx* c, double* out16) {
-  if (!c || !out16) { set_error("forge_scf_timings: null argument"); return -2; }
-  for (int q = 0; q < 16; ++q) out16[q] = c->tm[q];
+extern "C" int forge_scf_timings(forge_ctx* c, double* out24) {
+  if (!c || !out24) { set_error("forge_scf_timings: null argument"); return -2; }
+  for (int q = 0; q < FORGE_NTIMINGS; ++q) out24[q] = c->tm[q];
   return 0;
 }
 
@@ -2243,7 +2256,7 @@ static int scf2_init_state(forge_scf2* p) {
   for (forge_ctx* c : p->c) {
     c->scf_ready = false;  // the contexts' own loop does not continue the pair's state
     c->mixed_done = false;
-    std::fill(c->tm, c->tm + 16, 0.0);
+    std::fill(c->tm, c->tm + FORGE_NTIMINGS, 0.0);
   }
   return 0;
 }
@@ -2618,8 +2631,13 @@ static int sweep_solve(forge_sweep* w, const std::vector<int>& act) {
     int f = dc_solve(c->trd.d, c->trd.e, n, K, c->nU, c->evals, c->Zq, c->dc_ws, w->s);
     if (f < 0) return f;
     fails += f;
-    // reflectors of this system: its slots' trailing corners, compacted tau
+    // reflectors of this system: its slots' trailing corners, compacted tau;
+    // the batch's reduction is one-stage, whatever the system's own last
+    // reduction was (its two-stage flags would apply stale chase reflectors)
     TridiagWork v = c->trd;
+    v.ts = nullptr;
+    v.two_stage = w->trd.two_stage;
+    v.voff = w->trd.voff;
     v.n = n;
     v.K = K;
     v.A = w->H + q0 * sH + s0 + w->ldH * s0;
@@ -2655,7 +2673,7 @@ extern "C" int forge_sweep_run(forge_sweep* w, int64_t max_iter, double* traj, i
   for (int i = 0; i < ns; ++i) {
     own[i] = w->sys[i]->s;
     w->sys[i]->s = w->s;
-    std::fill(w->sys[i]->tm, w->sys[i]->tm + 16, 0.0);
+    std::fill(w->sys[i]->tm, w->sys[i]->tm + FORGE_NTIMINGS, 0.0);
   }
   std::vector<i64> done(ns, 0);
   std::vector<bool> stopped(ns, false);

@@ -149,7 +149,16 @@ struct TwoStageWork {
   i64* goff = nullptr;  // device: first block of each group (G + 1 entries)
   std::vector<i64> hoff;  // host copy
   i64 G = 0, nblk2 = 0;
+  // Timing of the chase's back-transformation (k_ts_q2): while q2_timed is
+  // set, every TS_Q2_STRIDE-th group launch is bracketed by an event pair on
+  // the dispatch; twostage_q2_timing reads and clears the records.
+  bool q2_timed = false;
+  mutable std::vector<hipEvent_t> q2ev;
+  mutable std::vector<double> q2flops;  // algorithmic flops of each recorded launch
+  mutable std::vector<double> q2bytes;  // its eigenvector rows' bytes, read and written once
+  mutable size_t q2rec = 0;
 };
+constexpr int TS_Q2_STRIDE = 4;
 size_t twostage_workspace_bytes(i64 n, int K);
 int twostage_bind(TwoStageWork& ts, void* base, i64 n, int K);
 // the reduction's launches (no host synchronisation while capturing)
@@ -159,6 +168,13 @@ int her2k_twostage(TridiagWork& w, i64 p, i64 i0, int nbp, hipStream_t s);
 // C <- Q2 C (the chase's reflectors), C complex FP64 or FP32
 int twostage_apply_q2(const TridiagWork& t, i64 m, z2* C, size_t ldC, size_t sC, hipStream_t s);
 int twostage_apply_q2(const TridiagWork& t, i64 m, float2* C, size_t ldC, size_t sC, hipStream_t s);
+// the recorded k_ts_q2 launches since the last call: count, summed time (ms),
+// algorithmic flops (16 per reflector element per column: v^H c and
+// c -= tau v w as complex multiply-adds) and the bytes of the eigenvector
+// rows they pass over (read and written once); waits for the last one
+int twostage_q2_timing(const TwoStageWork& ts, long* launches, double* ms, double* flops,
+                       double* bytes);
+void twostage_release_events(TwoStageWork& ts);
 
 // dc.hip: batched divide-and-conquer for real symmetric tridiagonals.
 struct DCWork;

@@ -37,6 +37,8 @@
 // and inside a group step by step, t = 0, 1, ...: every block as
 // C <- C - V T (V^H C) with its own triangular factor T (k_ts_t2, k_ts_q2).
 // That order respects every overlap of rows between the reflectors.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -1009,13 +1011,62 @@ static int apply_q2_t(const TridiagWork& t, i64 m, E* C, size_t ldC, size_t sC, 
   const TwoStageWork& ts = *t.ts;
   if (m <= 0 || t.n < 2) return 0;
   const unsigned nbc = cdiv(m, 64);
+  const i64 n = t.n;
   for (i64 g = ts.G - 1; g >= 0; --g) {
     const i64 cnt = ts.hoff[g + 1] - ts.hoff[g];
-    hipLaunchKernelGGL((k_ts_q2<S, E>), dim3(nbc, t.K), dim3(256), 0, s, t.n, g, cnt, ts.hoff[g], m,
-                       C, ldC, sC, ts.V2, ts.T2, ts.nblk2);
+    if (ts.q2_timed && g % TS_Q2_STRIDE == 0) {
+      const size_t q = ts.q2rec++;
+      while (ts.q2ev.size() < 2 * (q + 1)) {
+        hipEvent_t e;
+        FORGE_HIP(hipEventCreate(&e));
+        ts.q2ev.push_back(e);
+      }
+      // reflector elements of the group: sweep s's reflectors cover rows s + 1 .. n - 1
+      double elems = 0.0;
+      for (i64 sw = g * NBG; sw < std::min<i64>(g * NBG + NBG, n - 1); ++sw) elems += (double)(n - 1 - sw);
+      if (ts.q2flops.size() <= q) ts.q2flops.resize(q + 1);
+      if (ts.q2bytes.size() <= q) ts.q2bytes.resize(q + 1);
+      ts.q2flops[q] = 16.0 * elems * (double)m * t.K;
+      // the eigenvector rows the group's steps pass over, read and written once
+      ts.q2bytes[q] = 2.0 * (double)(n - 1 - g * NBG) * (double)m * t.K * sizeof(E);
+      hipExtLaunchKernelGGL((k_ts_q2<S, E>), dim3(nbc, t.K), dim3(256), 0, s, ts.q2ev[2 * q],
+                            ts.q2ev[2 * q + 1], 0, n, g, cnt, ts.hoff[g], m, C, ldC, sC, ts.V2, ts.T2,
+                            ts.nblk2);
+    } else {
+      hipLaunchKernelGGL((k_ts_q2<S, E>), dim3(nbc, t.K), dim3(256), 0, s, n, g, cnt, ts.hoff[g], m,
+                         C, ldC, sC, ts.V2, ts.T2, ts.nblk2);
+    }
     FORGE_LAUNCH();
   }
   return 0;
+}
+
+int twostage_q2_timing(const TwoStageWork& ts, long* launches, double* ms, double* flops,
+                       double* bytes) {
+  *launches = 0;
+  *ms = 0.0;
+  *flops = 0.0;
+  *bytes = 0.0;
+  if (ts.q2rec == 0) return 0;
+  FORGE_HIP(hipEventSynchronize(ts.q2ev[2 * ts.q2rec - 1]));
+  for (size_t q = 0; q < ts.q2rec; ++q) {
+    float a = 0.f;
+    FORGE_HIP(hipEventElapsedTime(&a, ts.q2ev[2 * q], ts.q2ev[2 * q + 1]));
+    *ms += a;
+    *flops += ts.q2flops[q];
+    *bytes += ts.q2bytes[q];
+  }
+  *launches = (long)ts.q2rec;
+  ts.q2rec = 0;
+  return 0;
+}
+
+void twostage_release_events(TwoStageWork& ts) {
+  for (hipEvent_t e : ts.q2ev) (void)hipEventDestroy(e);
+  ts.q2ev.clear();
+  ts.q2flops.clear();
+  ts.q2bytes.clear();
+  ts.q2rec = 0;
 }
 
 int twostage_apply_q2(const TridiagWork& t, i64 m, z2* C, size_t ldC, size_t sC, hipStream_t s) {

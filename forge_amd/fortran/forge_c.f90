@@ -426,10 +426,10 @@ module forge_c
       integer(c_int) :: r
     end function forge_scf_get_density
 
-    function forge_scf_timings(ctx, out16) bind(C, name="forge_scf_timings") result(r)
+    function forge_scf_timings(ctx, out24) bind(C, name="forge_scf_timings") result(r)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: out16(16)
+      real(c_double), intent(out) :: out24(24)
       integer(c_int) :: r
     end function forge_scf_timings
   end interface

@@ -327,7 +327,7 @@ class Engine:
         return rho
 
     def timings(self):
-        out = np.zeros(16)
+        out = np.zeros(24)
         L.check(self.lib.forge_scf_timings(self.ctx, _p(out)), "forge_scf_timings")
         return out
 

@@ -347,8 +347,14 @@ int forge_sweep_run(forge_sweep* sweep, int64_t max_iter, double* traj, int64_t*
  * 2 segments of 1024 and 2048 rows with 16 lanes per column chunk, 3 and 4
  * the same with 8 lanes and 16-byte row-pair loads (FP32 copy only).
  * FORGE_HEMV_FORCE="<segment>,<layout>" in the environment forces one
- * variant (layout 0 or 1) in every band. */
-int forge_scf_timings(forge_ctx* ctx, double* out16);
+ * variant (layout 0 or 1) in every band.  With the two-stage reduction:
+ * [15] timed launches of the chase's back-transformation kernel (every 4th
+ * group of sweeps), [16] their summed time (ms), [17] their algorithmic flops
+ * (16 per reflector element per eigenvector column), [18] the bytes of the
+ * eigenvector rows they pass over, read and written once.  [19..23] are
+ * zero. */
+#define FORGE_NTIMINGS 24
+int forge_scf_timings(forge_ctx* ctx, double* out24);
 
 /* ---- MLP of the learned initial guess (no reference counterpart) ----
  * 3 layers of width 128 with ReLU: Y = relu(relu(X W1 + b1) W2 + b2) W3 + b3,
