@@ -28,4 +28,61 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
 
+// A complex 16 x 16 product accumulated on the real instructions, operands
+// (ar + i ai)(br + i bi) one value per lane as above.  M3 = false: four real
+// products into re and im.  M3 = true: Gauss's three, P1 = ar br, P2 = ai bi,
+// P3 = (ar + ai)(br + bi), with re = P1 - P2 and im = P3 - P1 - P2 formed at
+// the end -- a quarter fewer matrix-core operations for an imaginary part
+// whose rounding is bounded by (|ar| + |ai|)(|br| + |bi|) instead of
+// |ar||bi| + |ai||br|.
+//   begin(cr, ci)  start a sum that ends added to (cr, ci): the four-product
+//                  form accumulates into them directly
+//   end(cr, ci)    (cr, ci) += the sum
+template <typename S, bool M3> struct CAcc;
+template <typename S> struct CAcc<S, false> {
+  typedef Mfma<S> MF;
+  typename MF::acc r, m;
+  __device__ __forceinline__ void zero() { r = typename MF::acc{}; m = typename MF::acc{}; }
+  __device__ __forceinline__ void begin(const typename MF::acc& cr, const typename MF::acc& ci) {
+    r = cr;
+    m = ci;
+  }
+  __device__ __forceinline__ void mma(S ar, S ai, S br, S bi) {
+    r = MF::mma(ar, br, r);
+    m = MF::mma(ar, bi, m);
+    r = MF::mma(-ai, bi, r);
+    m = MF::mma(ai, br, m);
+  }
+  __device__ __forceinline__ S re(int i) const { return r[i]; }
+  __device__ __forceinline__ S im(int i) const { return m[i]; }
+  __device__ __forceinline__ void end(typename MF::acc& cr, typename MF::acc& ci) const {
+    cr = r;
+    ci = m;
+  }
+};
+template <typename S> struct CAcc<S, true> {
+  typedef Mfma<S> MF;
+  typename MF::acc p1, p2, p3;
+  __device__ __forceinline__ void zero() {
+    p1 = typename MF::acc{};
+    p2 = typename MF::acc{};
+    p3 = typename MF::acc{};
+  }
+  __device__ __forceinline__ void begin(const typename MF::acc&, const typename MF::acc&) { zero(); }
+  __device__ __forceinline__ void mma(S ar, S ai, S br, S bi) {
+    p1 = MF::mma(ar, br, p1);
+    p2 = MF::mma(ai, bi, p2);
+    p3 = MF::mma(ar + ai, br + bi, p3);
+  }
+  __device__ __forceinline__ S re(int i) const { return p1[i] - p2[i]; }
+  __device__ __forceinline__ S im(int i) const { return p3[i] - p1[i] - p2[i]; }
+  __device__ __forceinline__ void end(typename MF::acc& cr, typename MF::acc& ci) const {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cr[i] += re(i);
+      ci[i] += im(i);
+    }
+  }
+};
+
 }  // namespace forge

@@ -703,7 +703,7 @@ __global__ void __launch_bounds__(256) k_trd_fin(i64 n, i64 j, i64 i0, int jj, z
 // column lane % 16 for output i = 0..3.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, int nbp, z2* A_,
+__global__ void __launch_bounds__(256, 2) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, int nbp, z2* A_,
                                                         size_t sA, i64 lda, float2* S_,
                                                         size_t sS, const z2* W_, const z2* tau_,
                                                         const z2* dpart_, i64 nbmax,
@@ -733,6 +733,20 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
 #pragma unroll
     for (int v = 0; v < 2; ++v) re[u][v] = im[u][v] = f64x4{0.0, 0.0, 0.0, 0.0};
   const z2 zero = mk(0.0, 0.0);
+  // the tile elements this lane updates, loaded before the products so their
+  // latency is hidden behind them (outside the lower triangle or the matrix:
+  // the always-valid element A(p, p), discarded)
+  z2 aold[2][2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const i64 r = r0 + qr * 32 + u * 16 + lk + 4 * i, c = c0 + qc * 32 + v * 16 + l16;
+        const bool ok = r < n && c < n && r >= c;
+        aold[u][v][i] = A[ok ? r + lda * c : p + lda * p];
+      }
   // staging of 8 panel columns: this thread's two elements e = tid, tid + 256
   // (panel column e / 64, tile row / column e % 64), loaded into registers
   // one step ahead so the loads overlap the previous step's MFMAs
@@ -804,7 +818,7 @@ __global__ void __launch_bounds__(256) k_trd_her2k_mfma(i64 n, i64 p, i64 i0, in
       for (int i = 0; i < 4; ++i) {
         const i64 r = r0 + qr * 32 + u * 16 + lk + 4 * i, c = c0 + qc * 32 + v * 16 + l16;
         if (r < n && c < n && r >= c) {
-          const z2 a = A[r + lda * c] - mk(re[u][v][i], im[u][v][i]);
+          const z2 a = aold[u][v][i] - mk(re[u][v][i], im[u][v][i]);
           A[r + lda * c] = a;
           if (S) S[r + lda * c] = make_float2((float)a.x, (float)a.y);
         }

@@ -344,14 +344,18 @@ __global__ void __launch_bounds__(256) k_ts_tbuild(i64 n, i64 i0, int nq, const 
 // ---------------------------------------------------------------- Y = A22 V
 // Block: 64 rows of Y (tile) over the column range [kb, ke) of A22 (split q).
 // Wave w owns rows 16 w .. 16 w + 15 and all 32 columns of V: two 16 x 16
-// FP64 MFMA blocks per real and imaginary part.  The A operand (one element
-// per lane per k-step: row lane % 16, column lane / 16) is read straight from
-// the lower triangle (mirrored and conjugated above the diagonal); V's stage
-// of 16 rows goes through LDS, double-buffered.
+// FP64 MFMA blocks (complex, three real products each).  The A operand (one
+// element per lane per k-step: row lane % 16, column lane / 16) is read
+// straight from the lower triangle (mirrored and conjugated above the
+// diagonal), a stage of HS columns ahead of its use; V's stage of HS rows goes
+// through LDS, double-buffered.  HS = 32: eight loads per lane in flight
+// across each barrier.
+constexpr int HS = 32;
 __global__ void __launch_bounds__(256) k_ts_hemm(i64 n, i64 r0, i64 mp, int nq, const z2* A_,
                                                  size_t sA, i64 lda, const z2* V_, z2* Y_,
                                                  i64 ntile, i64 KS) {
   typedef Mfma<double> MF;
+  typedef CAcc<double, true> CA;
   const int k = blockIdx.y;
   const z2* A = A_ + k * sA;
   const z2* V = V_ + (size_t)k * n * KD;
@@ -359,30 +363,31 @@ __global__ void __launch_bounds__(256) k_ts_hemm(i64 n, i64 r0, i64 mp, int nq, 
   const i64 kb = q * KS, ke = std::min<i64>(mp, kb + KS);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
   const i64 row = 64 * tile + 16 * wv + l16;
-  __shared__ double Vs[2][2][16][KD + 1];
-  gf64x4 re[2], im[2];
-#pragma unroll
-  for (int cb = 0; cb < 2; ++cb) re[cb] = im[cb] = gf64x4{0.0, 0.0, 0.0, 0.0};
-  z2 af[4], vf[2];
-  const i64 nst = (ke - kb + 15) / 16;
+  __shared__ double Vs[2][2][HS][KD + 1];
+  CA acc[2];
+  acc[0].zero();
+  acc[1].zero();
+  constexpr int NKS = HS / 4, NV = HS * KD / 256;
+  z2 af[NKS], vf[NV];
+  const i64 nst = (ke - kb + HS - 1) / HS;
   auto load = [&](i64 st) {
-    const i64 c0 = kb + 16 * st;
+    const i64 c0 = kb + HS * st;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+    for (int ks = 0; ks < NKS; ++ks) {
       const i64 col = c0 + 4 * ks + lk;
       af[ks] = herm_at(A, lda, r0, row, col, row < mp && col < ke);
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = tid + 256 * h, vr = e % 16, vc = e / 16;
+    for (int h = 0; h < NV; ++h) {
+      const int e = tid + 256 * h, vr = e % HS, vc = e / HS;
       const i64 gr = c0 + vr;
       vf[h] = ld_or_zero(V, (size_t)((r0 + gr) + n * vc), gr < ke && vc < nq, (size_t)r0);
     }
   };
   auto store_v = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = tid + 256 * h, vr = e % 16, vc = e / 16;
+    for (int h = 0; h < NV; ++h) {
+      const int e = tid + 256 * h, vr = e % HS, vc = e / HS;
       Vs[buf][0][vr][vc] = vf[h].x;
       Vs[buf][1][vr][vc] = vf[h].y;
     }
@@ -394,22 +399,16 @@ __global__ void __launch_bounds__(256) k_ts_hemm(i64 n, i64 r0, i64 mp, int nq, 
   __syncthreads();
   for (i64 st = 0; st < nst; ++st) {
     const int buf = (int)(st & 1);
-    z2 cur[4];
+    z2 cur[NKS];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) cur[ks] = af[ks];
+    for (int ks = 0; ks < NKS; ++ks) cur[ks] = af[ks];
     if (st + 1 < nst) load(st + 1);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const double ar = cur[ks].x, ai = cur[ks].y;
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int kk = 4 * ks + lk;
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        const double br = Vs[buf][0][4 * ks + lk][16 * cb + l16];
-        const double bi = Vs[buf][1][4 * ks + lk][16 * cb + l16];
-        re[cb] = MF::mma(ar, br, re[cb]);
-        re[cb] = MF::mma(-ai, bi, re[cb]);
-        im[cb] = MF::mma(ar, bi, im[cb]);
-        im[cb] = MF::mma(ai, br, im[cb]);
-      }
+      for (int cb = 0; cb < 2; ++cb)
+        acc[cb].mma(cur[ks].x, cur[ks].y, Vs[buf][0][kk][16 * cb + l16], Vs[buf][1][kk][16 * cb + l16]);
     }
     if (st + 1 < nst) store_v(buf ^ 1);
     __syncthreads();
@@ -420,7 +419,7 @@ __global__ void __launch_bounds__(256) k_ts_hemm(i64 n, i64 r0, i64 mp, int nq, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const i64 r = 64 * tile + 16 * wv + MF::row(lane, i);
-      if (r < mp) Y[(r0 + r) + n * (16 * cb + l16)] = mk(re[cb][i], im[cb][i]);
+      if (r < mp) Y[(r0 + r) + n * (16 * cb + l16)] = mk(acc[cb].re(i), acc[cb].im(i));
     }
 }
 
@@ -766,6 +765,10 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
                                                i64 nblk2) {
   typedef Mfma<S> MF;
   typedef typename MF::acc ACC;
+  // FP64: Gauss's three-product complex form (a quarter fewer matrix-core
+  // operations); FP32 keeps four products (its rounding is what bounds the
+  // mixed path)
+  typedef CAcc<S, sizeof(S) == 8> CA;
   const int k = blockIdx.y;
   E* C = C_ + k * sC;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15;
@@ -850,62 +853,61 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
       fetch_rows(pc[1], base + KD, 3);
     }
     // W = V^H C: reflector block mb meets row blocks mb .. mb + 2
-    ACC wre[2], wim[2];
+    S wr[2][4], wi[2][4];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-      wre[mb] = ACC{};
-      wim[mb] = ACC{};
+      CA w;
+      w.zero();
 #pragma unroll
       for (int dr = 0; dr < 3; ++dr) {
         const int rb = mb + dr;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int rr = 16 * rb + MF::row(lane, i);
-          const S ar = Vs[0][rr][16 * mb + l16], ai = -Vs[1][rr][16 * mb + l16];
-          const S br = cre[rb][i], bi = cim[rb][i];
-          wre[mb] = MF::mma(ar, br, wre[mb]);
-          wim[mb] = MF::mma(ar, bi, wim[mb]);
-          wre[mb] = MF::mma(-ai, bi, wre[mb]);
-          wim[mb] = MF::mma(ai, br, wim[mb]);
+          w.mma(Vs[0][rr][16 * mb + l16], -Vs[1][rr][16 * mb + l16], cre[rb][i], cim[rb][i]);
         }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        wr[mb][i] = w.re(i);
+        wi[mb][i] = w.im(i);
       }
     }
     // W' = T W (T upper triangular)
-    ACC xre[2], xim[2];
+    S xr[2][4], xi[2][4];
 #pragma unroll
     for (int mo = 0; mo < 2; ++mo) {
-      xre[mo] = ACC{};
-      xim[mo] = ACC{};
+      CA x;
+      x.zero();
 #pragma unroll
       for (int mi = mo; mi < 2; ++mi)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int kk = 16 * mi + MF::row(lane, i);
-          const S ar = Ts[0][16 * mo + l16][kk], ai = Ts[1][16 * mo + l16][kk];
-          const S br = wre[mi][i], bi = wim[mi][i];
-          xre[mo] = MF::mma(ar, br, xre[mo]);
-          xim[mo] = MF::mma(ar, bi, xim[mo]);
-          xre[mo] = MF::mma(-ai, bi, xre[mo]);
-          xim[mo] = MF::mma(ai, br, xim[mo]);
+          x.mma(Ts[0][16 * mo + l16][kk], Ts[1][16 * mo + l16][kk], wr[mi][i], wi[mi][i]);
         }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        xr[mo][i] = x.re(i);
+        xi[mo][i] = x.im(i);
+      }
     }
     // C -= V W': row block rb meets reflector blocks rb - 2 .. rb (within 0..1)
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int rb = 0; rb < 4; ++rb) {
+      CA u;
+      u.begin(cre[rb], cim[rb]);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
         if (rb < mb || rb > mb + 2) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int kk = 16 * mb + MF::row(lane, i);
-          const S ar = -Vs[0][16 * rb + l16][kk], ai = -Vs[1][16 * rb + l16][kk];
-          const S br = xre[mb][i], bi = xim[mb][i];
-          cre[rb] = MF::mma(ar, br, cre[rb]);
-          cim[rb] = MF::mma(ar, bi, cim[rb]);
-          cre[rb] = MF::mma(-ai, bi, cre[rb]);
-          cim[rb] = MF::mma(ai, br, cim[rb]);
+          u.mma(-Vs[0][16 * rb + l16][kk], -Vs[1][16 * rb + l16][kk], xr[mb][i], xi[mb][i]);
         }
       }
+      u.end(cre[rb], cim[rb]);
+    }
     __syncthreads();  // every wave is done with this block's V and T
     store_rb(0);
     store_rb(1);
@@ -962,7 +964,7 @@ int twostage_reduce(TridiagWork& w, hipStream_t s, bool capturing) {
     // column splits of A22 V: enough blocks to fill the chip several times
     const i64 ntile = (mp + 63) / 64;
     i64 nsplit = std::max<i64>(1, std::min<i64>(TS_YSPLIT, 2048 / std::max<i64>(1, ntile * K)));
-    i64 KS = ((mp + nsplit - 1) / nsplit + 15) / 16 * 16;
+    i64 KS = ((mp + nsplit - 1) / nsplit + HS - 1) / HS * HS;
     nsplit = (mp + KS - 1) / KS;
     hipLaunchKernelGGL(k_ts_hemm, dim3((unsigned)(ntile * nsplit), K), dim3(256), 0, s, n, r0, mp,
                        nq, w.A, w.sA, w.lda, ts.V, ts.Y, ntile, KS);
