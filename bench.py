@@ -7,9 +7,10 @@ the same mixer (the reference's optimal damping, OptimalStep, by default).
 
   mixed+learned   learned start (first diagonalisation, MLP site charges and
                   bond shifts on the matrix cores, one diagonalisation of the
-                  proposed Hamiltonian); SCF iterations with the FP32-copy
-                  hemv and the FP32 back-transformation while |dE| > 2e-6 eV,
-                  then FP64 iterations to convergence
+                  proposed Hamiltonian); SCF iterations with the eigenvectors
+                  back-transformed in FP32 (both stages of the two-stage
+                  reduction's reflectors) while |dE| > 2e-6 eV, then FP64
+                  iterations to convergence
   plain FP64      seeded start (uniform + 1e-3 Gaussian), every step FP64
 
 Default run (about 5 minutes): a step is one SCF iteration.  W untimed
@@ -22,10 +23,13 @@ more than ten minutes, so the default line carries the record of its
 complete run with the same mixer (PLAIN_RECORD, profiles/); `--full` runs it
 in the same process, and `--path plain` runs it alone.
 
-The JSON line carries the roofline of the dominant kernel, the hemv of the
-Householder reduction streaming the FP32 copy (HBM bound; HIP events on the
-library's stream inside the timed region), and a CPU baseline: the oracle
-(restatement of the reference's OpenMP/MKL path) timed on a bounded sample.
+The JSON line carries the roofline of the dominant kernel -- at this order the
+eigensolver's two-stage reduction runs, and its dominant kernel is k_ts_q2,
+the chase's reflectors applied to the eigenvectors on the matrix cores (HIP
+events on the library's stream inside the timed region; with the one-stage
+reduction, below order 1024, the Householder hemv) -- and a CPU baseline: the
+oracle (restatement of the reference's OpenMP/MKL path) timed on a bounded
+sample.
 Inputs are synthetic; nothing is read from disk except the trained network.
 """
 from __future__ import annotations
@@ -44,10 +48,11 @@ HBM_PEAK_GBS = 8000.0
 # one per mixer (bench.py --full --path plain --mixer M); the default line
 # compares the measured mixed+learned solve with the record of its mixer
 PLAIN_RECORD = {
-    "oda": {"s": 841.6, "iterations": 74, "final_energy_eV": -52523.3655272, "converged": True,
-            "profile": "profiles/scf_convergence_plain_fp64_oda_n11908_k3.txt"},
+    "oda": {"s": 540.2, "iterations": 75, "final_energy_eV": -52523.3655285, "converged": True,
+            "reduction": "two-stage", "profile": "profiles/scf_convergence_plain_fp64_oda_n11908_k3.txt"},
+    # an earlier build (one-stage reduction, 12.3 s per FP64 iteration); not converged in 85 iterations
     "linear": {"s": 1046.7, "iterations": 85, "final_energy_eV": -52523.3651114, "converged": False,
-               "profile": "profiles/scf_convergence_plain_fp64_n11908_k3.txt"},
+               "reduction": "one-stage", "profile": "profiles/scf_convergence_plain_fp64_n11908_k3.txt"},
 }
 # HBM traffic of k_trd_hemv per launch over algorithmic bytes (read + write,
 # rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated on a copy kernel of known size),
@@ -58,7 +63,7 @@ HEMV_TRAFFIC_RATIO = {("tbg_1.05deg_n11908_k3", 16): 1.034, ("tbg_1.05deg_n11908
 # bytes of the eigenvector rows a launch passes over, read and written once
 # (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel of known size),
 # for complex FP64 (16 B) and complex FP32 (8 B) eigenvectors
-Q2_TRAFFIC_RATIO = {}
+Q2_TRAFFIC_RATIO = {("tbg_1.05deg_n11908_k3", 16): 1.292, ("tbg_1.05deg_n11908_k3", 8): 1.397}
 MFMA_PEAK_TFS = {16: 78.6, 8: 157.3}  # dense FP64 and FP32 matrix-core peaks of MI355X
 MODEL = os.path.join(REPO, "forge_amd", "data", "guess_mlp.npz")
 
@@ -378,7 +383,7 @@ def main():
         "config": {
             "workload": args.workload,
             "ndim": n, "k_grid": f"{numk}x{numk}", "nfilling": nfill, "layers": nlayers,
-            "path": ("mixed precision (FP32-copy hemv and FP32 back-transformation until |dE| < %g eV, "
+            "path": ("mixed precision (FP32 back-transformation of the eigenvectors until |dE| < %g eV, "
                      "then FP64) + learned start" % args.switch_de) if key == "mixed_learned"
                     else "plain FP64 from the seeded start",
             "value_method": "measured",

@@ -78,13 +78,6 @@ template <typename T> struct GemmArgs {
 
 enum { EPI_GEMM = 0, EPI_LOWER = 1, EPI_CELLS = 2 };
 
-// blocks dispatched to one XCD (id % 8) get a contiguous range of tiles
-__device__ __forceinline__ i64 xcd_remap(i64 pid, i64 nb) {
-  const i64 q = nb / 8;
-  if (pid >= q * 8) return pid;
-  return (pid % 8) * q + pid / 8;
-}
-
 template <typename S, bool CPLX, bool TA, bool TB, int BM, int BN, int NWM, int NWN, int EPI>
 __global__ void __launch_bounds__(NWM * NWN * 64)
     k_gemm(GemmArgs<typename Elem<S, CPLX>::T> g) {
@@ -95,7 +88,10 @@ __global__ void __launch_bounds__(NWM * NWN * 64)
   constexpr int BK = 16;
   constexpr int WM = BM / NWM, WN = BN / NWN, UM = WM / 16, UN = WN / 16;
   constexpr int NP = CPLX ? 2 : 1;
-  constexpr int LDA_S = BM + 1, LDB_S = BN + 1;  // padded LDS rows
+  // padded LDS rows: one 8-byte element for FP64; 16 words for 4-byte
+  // elements, so the four k-rows of an MFMA operand start 16 banks apart
+  constexpr int PAD = sizeof(S) == 4 ? 16 : 1;
+  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
   constexpr int EA = BM * BK / NT, EB = BN * BK / NT;
   static_assert(BM * BK % NT == 0 && BN * BK % NT == 0, "tile / thread count");
   __shared__ S As[2][NP][BK][LDA_S];

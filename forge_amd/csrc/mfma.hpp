@@ -5,6 +5,16 @@
 
 namespace forge {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8), each
+// with its own L2.  xcd_remap(pid, nb) maps the linear id to a logical one so
+// that every XCD works on a contiguous range of the nb logical blocks, and
+// neighbouring blocks that read the same operands share one L2.
+__device__ __forceinline__ i64 xcd_remap(i64 pid, i64 nb) {
+  const i64 q = nb / 8;
+  if (pid >= q * 8) return pid;
+  return (pid % 8) * q + pid / 8;
+}
+
 typedef double gf64x4 __attribute__((ext_vector_type(4)));
 typedef float gf32x4 __attribute__((ext_vector_type(4)));
 
