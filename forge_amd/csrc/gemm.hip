@@ -88,10 +88,7 @@ __global__ void __launch_bounds__(NWM * NWN * 64)
   constexpr int BK = 16;
   constexpr int WM = BM / NWM, WN = BN / NWN, UM = WM / 16, UN = WN / 16;
   constexpr int NP = CPLX ? 2 : 1;
-  // padded LDS rows: one 8-byte element for FP64; 16 words for 4-byte
-  // elements, so the four k-rows of an MFMA operand start 16 banks apart
-  constexpr int PAD = sizeof(S) == 4 ? 16 : 1;
-  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  constexpr int LDA_S = BM + 1, LDB_S = BN + 1;  // padded LDS rows
   constexpr int EA = BM * BK / NT, EB = BN * BK / NT;
   static_assert(BM * BK % NT == 0 && BN * BK % NT == 0, "tile / thread count");
   __shared__ S As[2][NP][BK][LDA_S];

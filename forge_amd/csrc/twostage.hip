@@ -774,24 +774,11 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15;
   const i64 col = (i64)blockIdx.x * 64 + 16 * wv + l16;
   const bool cin = col < m;
-  // LDS: V for W = V^H C, read along reflectors; T for W' = T W and V for
-  // C -= V W', both read along their second index.  FP64 (8-byte words): one
-  // copy of each, rows padded by one element.  FP32: the rows padded to 36
-  // words, and T and a second copy of V stored transposed (rows of 36 and 68
-  // words), so the 16 lanes of each k-row and the four k-rows of an MFMA
-  // operand land on 64 distinct banks in all three products.
-  constexpr bool F32 = sizeof(S) == 4;
-  constexpr int PV = F32 ? 36 : NBG + 1;
-  __shared__ S Vs[2][64][PV];
-  __shared__ S Vt[2][F32 ? NBG : 1][F32 ? 68 : 1];
-  __shared__ S Ts[2][NBG][PV];
+  __shared__ S Vs[2][64][NBG + 1];
+  __shared__ S Ts[2][NBG][NBG + 1];
   for (int e = tid; e < 64 * NBG; e += 256) {
     Vs[0][e % 64][e / 64] = (S)0;
     Vs[1][e % 64][e / 64] = (S)0;
-    if constexpr (F32) {
-      Vt[0][e / 64][e % 64] = (S)0;
-      Vt[1][e / 64][e % 64] = (S)0;
-    }
   }
   ACC cre[4], cim[4];
   i64 base = g * NBG + 1;
@@ -816,18 +803,10 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
     for (int h = 0; h < 4; ++h) {
       const int e = tid + 256 * h, i = e / KD, r = i + e % KD;
       const double f = pok[h] ? 1.0 : 0.0;
-      const S vx = (S)(f * pv[h].x), vy = (S)(f * pv[h].y);
-      Vs[0][r][i] = vx;
-      Vs[1][r][i] = vy;
-      if constexpr (F32) {
-        Vt[0][i][r] = vx;
-        Vt[1][i][r] = vy;
-        Ts[0][e / NBG][e % NBG] = (S)pt[h].x;  // T(e % NBG, e / NBG), transposed
-        Ts[1][e / NBG][e % NBG] = (S)pt[h].y;
-      } else {
-        Ts[0][e % NBG][e / NBG] = (S)pt[h].x;
-        Ts[1][e % NBG][e / NBG] = (S)pt[h].y;
-      }
+      Vs[0][r][i] = (S)(f * pv[h].x);
+      Vs[1][r][i] = (S)(f * pv[h].y);
+      Ts[0][e % NBG][e / NBG] = (S)pt[h].x;
+      Ts[1][e % NBG][e / NBG] = (S)pt[h].y;
     }
   };
   // rows of row block rb of the window at bs (out-of-range elements read C[0]
@@ -905,10 +884,7 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int kk = 16 * mi + MF::row(lane, i);
-          if constexpr (F32)
-            x.mma(Ts[0][kk][16 * mo + l16], Ts[1][kk][16 * mo + l16], wr[mi][i], wi[mi][i]);
-          else
-            x.mma(Ts[0][16 * mo + l16][kk], Ts[1][16 * mo + l16][kk], wr[mi][i], wi[mi][i]);
+          x.mma(Ts[0][16 * mo + l16][kk], Ts[1][16 * mo + l16][kk], wr[mi][i], wi[mi][i]);
         }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -927,10 +903,7 @@ __global__ void __launch_bounds__(256, 2) k_ts_q2(i64 n, i64 g, i64 cnt, i64 b0,
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int kk = 16 * mb + MF::row(lane, i);
-          if constexpr (F32)
-            u.mma(-Vt[0][kk][16 * rb + l16], -Vt[1][kk][16 * rb + l16], xr[mb][i], xi[mb][i]);
-          else
-            u.mma(-Vs[0][16 * rb + l16][kk], -Vs[1][16 * rb + l16][kk], xr[mb][i], xi[mb][i]);
+          u.mma(-Vs[0][16 * rb + l16][kk], -Vs[1][16 * rb + l16][kk], xr[mb][i], xi[mb][i]);
         }
       }
       u.end(cre[rb], cim[rb]);
