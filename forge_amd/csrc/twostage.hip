@@ -236,16 +236,39 @@ __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2
   }
   if (jj == 0 && blockIdx.x == 0 && threadIdx.x >= nq && threadIdx.x < KD)
     tau_[(size_t)k * n + i0 + threadIdx.x] = mk(0.0, 0.0);
-  // slot q: the Gram entry for q < jj, the product with column q for q > jj
+  // slot q: the Gram entry for q < jj, the product with column q for q > jj.
+  // All 32 loads are issued first; the 32 sums over the wave's 64 lanes are
+  // then formed by a transposing butterfly: at each of the offsets 32, 16, 8,
+  // 4, 2 a lane keeps half of its slots and adds its partner's copy of them
+  // (16 + 8 + 4 + 2 + 1 exchanges), a last exchange at offset 1 completes
+  // the sum, and lane l holds slot l / 2.
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll 4
+  z2 pq[KD];
+#pragma unroll
   for (int q = 0; q < KD; ++q) {
     // q < jj: V(r, q); q > jj: A(r, i0 + q) (v is zero on rows above jj)
-    const z2 a = q < jj ? V[rs + n * q] : A[rs + lda * (i0 + q)];
-    z2 p = q < jj ? cmul(a, v) : cmul(v, a);
+    const z2* src = q < jj ? V + (rs + n * q) : A + (rs + lda * (i0 + q));
+    pq[q] = *src;
+  }
+#pragma unroll
+  for (int q = 0; q < KD; ++q) {
+    z2 p = q < jj ? cmul(pq[q], v) : cmul(v, pq[q]);
     if (!live || q == jj) p = mk(0.0, 0.0);
-    double px = ts_wave_sum(p.x), py = ts_wave_sum(p.y);
-    if (lane == 0) red[wv][q] = mk(px, py);
+    pq[q] = p;
+  }
+#pragma unroll
+  for (int half = KD / 2; half >= 1; half >>= 1) {
+    const bool up = (lane & (2 * half)) != 0;  // offsets 32, 16, 8, 4, 2
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const z2 keep = up ? pq[j + half] : pq[j];
+      const z2 send = up ? pq[j] : pq[j + half];
+      pq[j] = mk(keep.x + __shfl_xor(send.x, 2 * half, 64), keep.y + __shfl_xor(send.y, 2 * half, 64));
+    }
+  }
+  {
+    const z2 tot = mk(pq[0].x + __shfl_xor(pq[0].x, 1, 64), pq[0].y + __shfl_xor(pq[0].y, 1, 64));
+    if ((lane & 1) == 0) red[wv][lane >> 1] = tot;
   }
   __syncthreads();
   if (threadIdx.x < KD) {

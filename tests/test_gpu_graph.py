@@ -43,11 +43,15 @@ def _traj(e, nfill, iters, **opt):
     return e.scf_run(iters)
 
 
+@pytest.mark.parametrize("reduction", [1, 2])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replay_matches_eager(precision):
-    say("graph replay, precision", precision)
+def test_graph_replay_matches_eager(precision, reduction):
+    """reduction 2: the two-stage reduction's launches (panel QR, products,
+    bulge chasing) captured instead of the one-stage ones."""
+    say("graph replay, precision", precision, "reduction", reduction)
     e = Engine(Setup(ntheta=5, numk=3, nfilling=1.0, xi_nm=40.0))  # n = 364
     sw = dict(precision=precision, switch_de=1e-3) if precision else {}
+    sw["reduction"] = reduction
     eager = _traj(e, 1.0, 12, **sw)
     graph = _traj(e, 1.0, 12, graph=1, **sw)
     again = _traj(e, 1.0, 12, graph=1, **sw)  # replays of the graph kept from the last run
@@ -132,11 +136,12 @@ def test_device_iteration_matches_reference_trajectory(case):
     e.close()
 
 
+@pytest.mark.parametrize("reduction", [1, 2])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_iteration_graph_replays_the_device_iteration_bitwise(precision):
-    say("iteration graph vs eager device iteration, precision", precision)
+def test_iteration_graph_replays_the_device_iteration_bitwise(precision, reduction):
+    say("iteration graph vs eager device iteration, precision", precision, "reduction", reduction)
     e = Engine(Setup(ntheta=5, numk=3, nfilling=1.0, xi_nm=40.0))  # n = 364
-    opts = dict(occ_mode=2, mix_mode=0, kT=0.001)
+    opts = dict(occ_mode=2, mix_mode=0, kT=0.001, reduction=reduction)
     if precision:
         opts.update(precision=1, switch_de=1e-3)
     eager, rho_e = _dev_run(e, 12, 2, opts)
