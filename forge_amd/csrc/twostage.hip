@@ -628,13 +628,17 @@ __global__ void __launch_bounds__(64) k_ts_chase(i64 n, i64 slot, i64 s_lo, i64 
   }
   __syncthreads();
   const z2 tau = tau_s;
+  // The products below use both halves of the wave: lane l works on row (or
+  // column) r = l % 32 over the half hh = l / 32 of the summation index, and
+  // the two halves are added across lanes l and l + 32.
+  const int r = l & (KD - 1), hh = l >> 5, q0 = 16 * hh;
   // D <- H^H D H:  y = D v, w = tau y - |tau|^2 (v^H y) v / 2, D -= w v^H + v w^H
   z2 y = mk(0.0, 0.0);
-  if (l < KD)
-#pragma unroll 8
-    for (int j = 0; j < KD; ++j) y = y + M[l][j] * vs[j];
-  const double sig = ts_wave_sum(l < KD ? (vs[l].x * y.x + vs[l].y * y.y) : 0.0);
-  if (l < KD) ws[l] = tau * y - (0.5 * abs2(tau) * sig) * vs[l];
+#pragma unroll
+  for (int j = q0; j < q0 + 16; ++j) y = y + M[r][j] * vs[j];
+  y = mk(y.x + __shfl_xor(y.x, 32, 64), y.y + __shfl_xor(y.y, 32, 64));
+  const double sig = ts_wave_sum(hh == 0 ? (vs[r].x * y.x + vs[r].y * y.y) : 0.0);
+  if (hh == 0) ws[r] = tau * y - (0.5 * abs2(tau) * sig) * vs[r];
   __syncthreads();
 #pragma unroll
   for (int h = 0; h < 16; ++h) {
@@ -655,11 +659,12 @@ __global__ void __launch_bounds__(64) k_ts_chase(i64 n, i64 slot, i64 s_lo, i64 
     M[e % KD][e / KD] = bv[h];
   }
   __syncthreads();
-  if (l < KD) {
+  {
     z2 u = mk(0.0, 0.0);
-#pragma unroll 8
-    for (int j = 0; j < KD; ++j) u = u + M[l][j] * vs[j];
-    us[l] = tau * u;
+#pragma unroll
+    for (int j = q0; j < q0 + 16; ++j) u = u + M[r][j] * vs[j];
+    u = mk(u.x + __shfl_xor(u.x, 32, 64), u.y + __shfl_xor(u.y, 32, 64));
+    if (hh == 0) us[r] = tau * u;
   }
   __syncthreads();
 #pragma unroll
@@ -680,17 +685,21 @@ __global__ void __launch_bounds__(64) k_ts_chase(i64 n, i64 slot, i64 s_lo, i64 
   if (l == 0) tau2[(size_t)(t + 1) * NBG + ii] = R.tau;
   __syncthreads();
   const z2 tb = conjz(R.tau);
-  if (l == 0) {
-    M[0][0] = mk(R.beta, 0.0);
-    for (int i = 1; i < KD; ++i) M[i][0] = mk(0.0, 0.0);
-  } else if (l < L) {
-    // column l: q = v2^H B(:, l), B(:, l) -= conj(tau2) v2 q
+  {
+    // column r: q = v2^H B(:, r), B(:, r) -= conj(tau2) v2 q; column 0
+    // becomes (beta, 0, ..., 0)
     z2 qv = mk(0.0, 0.0);
-#pragma unroll 8
-    for (int i = 0; i < KD; ++i) qv = qv + cmul(vs[i], M[i][l]);
+#pragma unroll
+    for (int i = q0; i < q0 + 16; ++i) qv = qv + cmul(vs[i], M[i][r]);
+    qv = mk(qv.x + __shfl_xor(qv.x, 32, 64), qv.y + __shfl_xor(qv.y, 32, 64));
     const z2 f = tb * qv;
-#pragma unroll 8
-    for (int i = 0; i < KD; ++i) M[i][l] = M[i][l] - vs[i] * f;
+    if (r == 0) {
+#pragma unroll
+      for (int i = q0; i < q0 + 16; ++i) M[i][0] = i == 0 ? mk(R.beta, 0.0) : mk(0.0, 0.0);
+    } else if (r < L) {
+#pragma unroll
+      for (int i = q0; i < q0 + 16; ++i) M[i][r] = M[i][r] - vs[i] * f;
+    }
   }
   __syncthreads();
 #pragma unroll
