@@ -1,14 +1,18 @@
 """Training of the learned-guess MLP on SCF solutions of this engine (GPU).
 
-    python -m forge_amd.guess_train [--cells 6-15] [--fillings=-2,-1,0,1,2]
-        [--data PATH] [--out forge_amd/data/guess_mlp.npz] [--steps 4000]
+    python -m forge_amd.guess_train [--cells 6-15] [--tri-cells 6-10]
+        [--fillings=-2,-1,0,1,2] [--data PATH] [--out forge_amd/data/guess_mlp.npz]
+        [--steps 4000]
 
-1. Data: for every cell index m and filling nu, the first diagonalisation
-   (rho_TB) and a plain FP64 SCF from the seeded start (linear mixing 0.3,
-   Fermi smearing 1 meV, |dE| < 1e-6 eV); the converged site densities and
-   the nearest-neighbour bond values of both densities are written, and the
+1. Data: for every cell index m and filling nu -- twisted bilayers (--cells)
+   and alternating-twist trilayers (--tri-cells, so that the middle layer's
+   descriptors are seen in training) -- the first diagonalisation (rho_TB)
+   and a plain FP64 SCF from the seeded start (linear mixing 0.3, Fermi
+   smearing 1 meV, |dE| < 1e-6 eV); the converged site densities and the
+   nearest-neighbour bond values of both densities are written, and the
    descriptors and targets of every site (forge_amd.learned) are formed from
-   them when training.  Cell index max(cells) is held out for validation.
+   them when training.  The largest cell index of each layer count is held
+   out for validation.
 2. Training: Adam on the library's MLP kernels (forward, backward and update
    on the GPU), standardised inputs and targets.
 3. Report: validation loss and the SCF iterations from the learned start
@@ -37,11 +41,11 @@ def say(*a):
     print(*a, flush=True)
 
 
-def solve_system(m: int, nu: float, max_iter: int = 250):
-    """SCF of cell m at filling nu; returns the raw fields the targets are
-    formed from (converged diagonal, converged and non-interacting bond
-    values in nn_bonds order) and a summary row."""
-    e = Engine(Setup(ntheta=m, numk=3, nfilling=nu, xi_nm=40.0))
+def solve_system(m: int, nu: float, nlayers: int = 2, max_iter: int = 250):
+    """SCF of cell m (nlayers layers) at filling nu; returns the raw fields
+    the targets are formed from (converged diagonal, converged and
+    non-interacting bond values in nn_bonds order) and a summary row."""
+    e = Engine(Setup(ntheta=m, nlayers=nlayers, numk=3, nfilling=nu, xi_nm=40.0))
     e.set_options(**SCF)
     t0 = time.time()
     bonds = Lr.nn_bonds(e.geom)
@@ -51,19 +55,20 @@ def solve_system(m: int, nu: float, max_iter: int = 250):
     tr = e.scf_run(max_iter)
     rho = e.density()
     raw = (np.real(np.diagonal(rho[:, :, 0])).copy(), Lr.bond_values(rho, bonds), b_tb)
-    info = (m, nu, e.n, len(tr), tr[-1, 1] - tr[-1, 8], tr[-1, 8], time.time() - t0)
+    info = (m, nu, e.n, len(tr), tr[-1, 1] - tr[-1, 8], tr[-1, 8], time.time() - t0, nlayers)
     e.close()
     return raw, info
 
 
-def generate(cells, fillings, path):
-    """Solves every (m, nu) and writes the raw fields (npz: info rows, and
-    per system its diagonal and bond values, concatenated)."""
+def generate(systems, fillings, path):
+    """Solves every (layers, m) of systems at every filling nu and writes the
+    raw fields (npz: info rows, and per system its diagonal and bond values,
+    concatenated)."""
     diag, bconv, btb, infos = [], [], [], []
-    for m in cells:
+    for nl, m in systems:
         for nu in fillings:
-            (d, bc, bt), info = solve_system(m, nu)
-            say("  m=%2d nu=%+.0f n=%5d: %3d iterations, E=%.6f eV, last dE %+.1e (%.1fs)" % info)
+            (d, bc, bt), info = solve_system(m, nu, nl)
+            say("  m=%2d nu=%+.0f n=%5d: %3d iterations, E=%.6f eV, last dE %+.1e (%.1fs), %d layers" % info)
             diag.append(d); bconv.append(bc); btb.append(bt)
             infos.append(info)
     np.savez(path, info=np.array(infos), diag=np.concatenate(diag), bconv=np.concatenate(bconv),
@@ -80,7 +85,8 @@ def load_data(paths):
         od = ob = 0
         for row in f["info"]:
             m, nu, n = int(row[0]), float(row[1]), int(row[2])
-            g = build_geometry(m, 2, 3)
+            nl = int(row[7]) if len(row) > 7 else 2
+            g = build_geometry(m, nl, 3)
             bonds = Lr.nn_bonds(g)
             nb = len(bonds.i)
             X = Lr.descriptors(g, nu, bonds)
@@ -104,7 +110,8 @@ def iterations_from(e, start_fn, max_iter=250):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--cells", default="6-15")
+    ap.add_argument("--cells", default="6-15", help="bilayer cell indices")
+    ap.add_argument("--tri-cells", default="", help="trilayer cell indices (e.g. 6-10)")
     ap.add_argument("--fillings", default="-2,-1,0,1,2")
     ap.add_argument("--data", default="guess_data.npz")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "guess_mlp.npz"))
@@ -115,21 +122,31 @@ def main():
                     help="train on existing data files (comma-separated --data)")
     ap.add_argument("--data-only", action="store_true", help="generate the data and stop")
     a = ap.parse_args()
-    lo, hi = (int(x) for x in a.cells.split("-"))
-    cells = list(range(lo, hi + 1))
+    def span(text):
+        if not text:
+            return []
+        lo, hi = (int(x) for x in text.split("-"))
+        return list(range(lo, hi + 1))
+
+    cells, tri = span(a.cells), span(a.tri_cells)
     fillings = [float(x) for x in a.fillings.split(",")]
     paths = a.data.split(",")
     if not a.reuse_data:
         os.makedirs(os.path.dirname(os.path.abspath(paths[0])), exist_ok=True)
-        say(f"[data] {len(cells)} cells x {len(fillings)} fillings")
-        generate(cells, fillings, paths[0])
+        say(f"[data] {len(cells)} bilayer + {len(tri)} trilayer cells x {len(fillings)} fillings")
+        generate([(2, m) for m in cells] + [(3, m) for m in tri], fillings, paths[0])
         if a.data_only:
             return 0
     X, Y, M, sid, info = load_data(paths)
-    held = info[:, 0] == cells[-1]
+    nls = info[:, 7].astype(int) if info.shape[1] > 7 else np.full(len(info), 2)
+    held = np.zeros(len(info), bool)
+    for nl in np.unique(nls):
+        sel = nls == nl
+        held |= sel & (info[:, 0] == info[sel, 0].max())
     val = np.isin(sid, np.nonzero(held)[0])
     say(f"[train] {int((~val).sum())} training sites, {int(val.sum())} validation sites "
-        f"(m = {cells[-1]} held out)")
+        f"(held out: " + ", ".join(f"m = {int(info[held & (nls == nl), 0].max())}, {nl} layers"
+                                    for nl in np.unique(nls)) + ")")
     t0 = time.time()
     model, hist = Mlp.fit(X[~val], Y[~val], M[~val], steps=a.steps, batch=a.batch, lr=a.lr)
     say(f"[train] {a.steps} Adam steps in {time.time() - t0:.1f}s; batch loss "
@@ -144,19 +161,23 @@ def main():
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     model.save(a.out)
     say(f"[train] model written to {a.out}")
-    # the held-out cell at nu = +1: seeded start against the learned start,
-    # with linear mixing 0.3 and with optimal damping
-    m = cells[-1]
-    e = Engine(Setup(ntheta=m, numk=3, nfilling=1.0, xi_nm=40.0))
-    for mixer, opts in (("linear 0.3", SCF), ("optimal damping", dict(SCF, mix_mode=0))):
-        e.set_options(**opts)
-        its_s = iterations_from(e, lambda: e.scf_init(seeded_rho(e.n, e.nnc, 1.0)))
-        its_tb = iterations_from(e, lambda: e.scf_init(None))
-        its_l = iterations_from(e, lambda: Lr.learned_start(e, model, 1.0))
-        for tag, r in (("seeded", its_s), ("first diagonalisation", its_tb), ("learned", its_l)):
-            say(f"[check] m={m} nu=+1 {mixer}, start {tag:22s}: {r[0]:3d} iterations, "
-                f"E={r[1]:.6f} eV ({r[2]:.1f}s)")
-    e.close()
+    # each held-out cell (bilayer at nu = +1, trilayer at nu = +2): seeded
+    # start against the learned start, with linear mixing 0.3 and with
+    # optimal damping
+    checks = [(2, cells[-1], 1.0)] if cells else []
+    if tri:
+        checks.append((3, tri[-1], 2.0))
+    for nl, m, nu in checks:
+        e = Engine(Setup(ntheta=m, nlayers=nl, numk=3, nfilling=nu, xi_nm=40.0))
+        for mixer, opts in (("linear 0.3", SCF), ("optimal damping", dict(SCF, mix_mode=0))):
+            e.set_options(**opts)
+            its_s = iterations_from(e, lambda: e.scf_init(seeded_rho(e.n, e.nnc, nu)))
+            its_tb = iterations_from(e, lambda: e.scf_init(None))
+            its_l = iterations_from(e, lambda: Lr.learned_start(e, model, nu))
+            for tag, r in (("seeded", its_s), ("first diagonalisation", its_tb), ("learned", its_l)):
+                say(f"[check] {nl} layers m={m} nu={nu:+.0f} {mixer}, start {tag:22s}: {r[0]:3d} "
+                    f"iterations, E={r[1]:.6f} eV ({r[2]:.1f}s)")
+        e.close()
     return 0
 
 
