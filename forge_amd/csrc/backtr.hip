@@ -22,6 +22,7 @@
 // the promoted eigenvectors twice; the second run is timed with HIP events
 // and the width with the least time per reflector is kept for the shape for
 // the rest of the process.
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -226,6 +227,15 @@ static std::map<std::tuple<int, i64, i64, int, int>, int> g_bt_width;  // (devic
 template <typename E>
 static int tuned_width(const TridiagWork& t, const double* Z, size_t ldZ, size_t sZ, i64 m, E* C,
                        size_t ldC, size_t sC, void* work, hipStream_t s, int* width) {
+  // FORGE_BT_WIDTH=64|128|256 in the environment forces the width (runs that
+  // compare with a fixed width)
+  if (const char* f = getenv("FORGE_BT_WIDTH")) {
+    const int w = atoi(f);
+    if (w == 64 || w == 128 || w == 256) {
+      *width = w;
+      return 0;
+    }
+  }
   int dev = 0;
   FORGE_HIP(hipGetDevice(&dev));
   const auto key = std::make_tuple(dev, t.n, m, t.K, (int)(sizeof(E) == sizeof(float2)));
