@@ -113,6 +113,10 @@ __global__ void k_copy_window(i64 n, i64 nb, i64 col0, int K, i64 k0, const z2* 
 
 using namespace forge;
 
+// pinned readback entries beyond the eigenvalues and site densities (5 energy
+// sums, 10 dot products)
+constexpr i64 HPIN_SMALL = 64;
+
 struct forge_ctx {
   forge_params P;
   PairParams pp;
@@ -146,6 +150,10 @@ struct forge_ctx {
   // FP32 copy of the eigensolver batch for the mixed-precision hemv
   // (allocated on the first forge_scf_set_options with precision 1)
   float2* S = nullptr;
+  // pinned host memory for the readbacks of every iteration (eigenvalues
+  // n x Kb, the two site densities, energy sums): the copies land by DMA
+  // and are complete when the stream is synchronised
+  double* hpin = nullptr;
   // SCF state
   forge_scf_options opt{};
   bool scf_ready = false;
@@ -380,6 +388,8 @@ extern "C" int forge_ctx_create(const forge_params* p, const double* Coords, con
     CK(dalloc((void**)&c->evals, sizeof(double) * n * Kb_));
     CK(dalloc((void**)&c->Cv, sizeof(z2) * n * nU * Kb_));
     CK(dalloc(&c->bt_ws, backtr_workspace_bytes(n, Kb_, nU)));
+    FORGE_HIP(hipHostMalloc((void**)&c->hpin, sizeof(double) * (n * Kb_ + 2 * n + HPIN_SMALL),
+                            hipHostMallocDefault));
     CK(dalloc((void**)&c->Cwin, sizeof(z2) * n * c->nb * Nk));
     CK(dalloc((void**)&c->Psi, sizeof(z2) * n * c->nb * Nk));
     CK(dalloc((void**)&c->wph, sizeof(double) * c->nb * Nk * nnc * 2));
@@ -417,6 +427,7 @@ extern "C" int forge_ctx_destroy(forge_ctx* c) {
     if (g) (void)hipGraphExecDestroy(g);
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (c->hpin) (void)hipHostFree(c->hpin);
   tridiag_release(c->trd);
   twostage_release_events(c->ts);
   for (i64 q = 0; q < c->trd.nev; ++q) (void)hipEventDestroy(c->trd.evs[q]);
@@ -892,7 +903,7 @@ static int solve_device_c3(forge_ctx* c, const z2* rho_dev, const double* pot_de
   const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nU = c->nU, nb = c->nb;
   const i64 R = (i64)c->c3_reps.size();
   hband.assign(nb * Nk, 0.0);
-  std::vector<double> hev(n * c->Kb);
+  double* hev = c->hpin;  // n x Kb
   int fails = 0;
   for (i64 r0 = 0; r0 < R; r0 += c->Kb) {
     const int K = (int)std::min<i64>(c->Kb, R - r0);
@@ -927,7 +938,7 @@ static int solve_device_c3(forge_ctx* c, const z2* rho_dev, const double* pot_de
       FORGE_HIP(hipEventElapsedTime(&a, c->ev[6], c->ev[7]));
       c->tm[4] += a;
     }
-    FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
+    FORGE_HIP(hipMemcpyAsync(hev, c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
     FORGE_HIP(hipStreamSynchronize(c->s));
     for (int kb = 0; kb < K; ++kb) {
       const i64 k = c->c3_reps[r0 + kb];
@@ -955,7 +966,7 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
   const i64 n = c->n, nnc = c->nnc, Nk = c->Nk, nU = c->nU, nb = c->nb;
   const size_t nn = (size_t)n * n;
   hband.assign(nb * Nk, 0.0);
-  std::vector<double> hev(n * c->Kb);
+  double* hev = c->hpin;  // n x Kb
   int fails = 0;
   for (i64 k0 = 0; k0 < Nk; k0 += c->Kb) {
     const int K = (int)std::min<i64>(c->Kb, Nk - k0);
@@ -999,7 +1010,7 @@ static int solve_device(forge_ctx* c, const z2* rho_dev, const double* pot_dev,
       CK(band_gather(n, nb, c->nL - 1, K, k0, Nk, c->evals, c->dband, c->s));
       continue;
     }
-    FORGE_HIP(hipMemcpyAsync(hev.data(), c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
+    FORGE_HIP(hipMemcpyAsync(hev, c->evals, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->s));
     FORGE_HIP(hipStreamSynchronize(c->s));
     for (int kb = 0; kb < K; ++kb)
       for (i64 b = 0; b < nb; ++b) hband[b + nb * (k0 + kb)] = hev[kb * n + (c->nL - 1) + b];
@@ -1434,9 +1445,10 @@ static int energy_pieces(forge_ctx* c, const z2* a, const z2* b, double* pieces)
   hipLaunchKernelGGL(k_bias_dot, dim3(1), dim3(256), 0, c->s, n, c->da, b ? c->db : nullptr, c->bias,
                      c->dots + 8);
   FORGE_LAUNCH();
-  double h5[5], hd[10];
-  FORGE_HIP(hipMemcpyAsync(h5, c->e5, sizeof(h5), hipMemcpyDeviceToHost, c->s));
-  FORGE_HIP(hipMemcpyAsync(hd, c->dots, sizeof(hd), hipMemcpyDeviceToHost, c->s));
+  double* h5 = c->hpin + n * c->Kb + 2 * n;  // 5 energy sums, then 10 dot products
+  double* hd = h5 + 5;
+  FORGE_HIP(hipMemcpyAsync(h5, c->e5, sizeof(double) * 5, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(hd, c->dots, sizeof(double) * 10, hipMemcpyDeviceToHost, c->s));
   FORGE_HIP(hipStreamSynchronize(c->s));
   for (int q = 0; q < 5; ++q) pieces[q] = h5[q];
   pieces[5] = hd[0];
@@ -1805,7 +1817,8 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   const i64 n = c->n, nnc = c->nnc;
   const size_t nn = (size_t)n * n;
   *stop = false;
-  std::vector<double> hda(n), hdb(n);
+  double* hda = c->hpin + n * c->Kb;  // the site densities of rho and rho_new
+  double* hdb = hda + n;
   std::vector<double> band;
   const bool mixed = c->opt.precision == 1 && !c->mixed_done;
   if (mixed) c->tm[12] += 1;
@@ -1844,8 +1857,8 @@ static int scf_step(forge_ctx* c, double* traj_row, bool timed, bool eig_ready, 
   energies_at(pc, step, c->e);
   c->Etot = total_energy(c->P, c->e);
   // DensityConvergence = max |D_mixed - D_in| (Main.f90:564-571)
-  FORGE_HIP(hipMemcpyAsync(hda.data(), c->da, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
-  FORGE_HIP(hipMemcpyAsync(hdb.data(), c->db, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(hda, c->da, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  FORGE_HIP(hipMemcpyAsync(hdb, c->db, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
   CK(mix(nnc * nn, c->rho, c->rho_new, step, c->s));
   hipLaunchKernelGGL(k_potential, dim3(cdiv(n, 256)), dim3(256), 0, c->s, n, c->ya, c->yb, c->da,
                      c->db, step, c->P.alphaH, c->P.U, c->pot, c->da);

@@ -139,16 +139,25 @@ __global__ void __launch_bounds__(256) k_assemble(i64 n, int nnc, int K,
     double v0 = V[o], t0 = T[o];
     z2 w0 = mk(t0 - alpha * v0 * r0.x, alpha * v0 * r0.y);
     if (i == j) w0.x += pot[i] + bias[i];
-    z2 a[MAXCELL], b[MAXCELL];
-    for (int c = 1; c < nnc; ++c) {
-      double tv = T[c * nn + o], vv = V[c * nn + o];
-      z2 r = rho ? rho[c * nn + o] : mk(0.0, 0.0);
-      a[c] = mk(tv - alpha * vv * r.x, alpha * vv * r.y);
-      b[c] = conjz(tr[c - 1][cc][tx]);  // conj(A_c(j, i))
+    // the image cells in registers: fixed-size arrays under fully unrolled
+    // loops (a runtime bound would index them dynamically, in scratch memory)
+    z2 a[ASM_CELLS], b[ASM_CELLS];
+#pragma unroll
+    for (int c = 1; c < ASM_CELLS; ++c) {
+      a[c] = mk(0.0, 0.0);
+      b[c] = mk(0.0, 0.0);
+      if (c < nnc) {
+        double tv = T[c * nn + o], vv = V[c * nn + o];
+        z2 r = rho ? rho[c * nn + o] : mk(0.0, 0.0);
+        a[c] = mk(tv - alpha * vv * r.x, alpha * vv * r.y);
+        b[c] = conjz(tr[c - 1][cc][tx]);  // conj(A_c(j, i))
+      }
     }
     for (int k = 0; k < K; ++k) {
       z2 h = w0;
-      for (int c = 1; c < nnc; ++c) {
+#pragma unroll
+      for (int c = 1; c < ASM_CELLS; ++c) {
+        if (c >= nnc) continue;
         double co = phase[(k * nnc + c) * 2], si = phase[(k * nnc + c) * 2 + 1];
         // a e^{-i th} + b e^{+i th}
         h.x += a[c].x * co + a[c].y * si + b[c].x * co - b[c].y * si;
