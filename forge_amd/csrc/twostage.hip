@@ -199,8 +199,9 @@ __global__ void __launch_bounds__(256) k_ts_qnorm(i64 n, i64 i0, const z2* A_, s
   if (threadIdx.x == 0) npart_[(size_t)k * nbqmax + blockIdx.x] = v;
 }
 
-// Reflector of panel column jj from the norm partials; v written to V (and its
-// tail to A, beta to A's diagonal of R); partial sums per block of
+// Reflector of panel column jj from the norm partials and the pivot (read by
+// every block, so left unchanged here: k_ts_qb writes beta to it); v written
+// to V and its tail to A; partial sums per block of
 //   w_q = v^H A(:, i0 + q)  (q > jj)   and   G(q, jj) = V(:, q)^H v  (q < jj).
 __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2* A_, size_t sA,
                                                i64 lda, z2* V_, z2* tau_, const double* npart_,
@@ -228,7 +229,10 @@ __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2
     V[r + n * jj] = v;
     if (rr > jj) A[r + lda * c] = v;
     if (rr == jj) {
-      A[r + lda * c] = mk(R.beta, 0.0);
+      // beta goes to the pivot in k_ts_qb: every block of this launch reads
+      // the pivot for its reflector, so it must not change before they have;
+      // it is parked in slot 0 of block 0's partials (qpart holds q > jj only)
+      qpart_[((size_t)k * nbqmax) * KD] = mk(R.beta, 0.0);
       tau_[(size_t)k * n + c] = R.tau;
     }
     if (jj == 0)
@@ -279,8 +283,8 @@ __global__ void __launch_bounds__(256) k_ts_qa(i64 n, i64 i0, int jj, int nq, z2
   }
 }
 
-// A(:, i0 + q) -= conj(tau) v w_q for q > jj (own rows), norm partials of
-// column jj + 1 below its first panel row
+// A(:, i0 + q) -= conj(tau) v w_q for q > jj (own rows), beta to the pivot
+// of column jj, norm partials of column jj + 1 below its first panel row
 __global__ void __launch_bounds__(256) k_ts_qb(i64 n, i64 i0, int jj, int nq, z2* A_, size_t sA,
                                                i64 lda, const z2* V_, const z2* tau_,
                                                const z2* qpart_, double* npart_, i64 nbqmax) {
@@ -323,6 +327,9 @@ __global__ void __launch_bounds__(256) k_ts_qb(i64 n, i64 i0, int jj, int nq, z2
         A[r + lda * (i0 + q)] = a;
         if (q == jj + 1 && rr >= jj + 2) nrm = abs2(a);
       }
+    // the pivot of column jj becomes beta (k_ts_qa parked it); no other
+    // thread of this launch reads it
+    if (rr == jj) A[r + lda * c] = qpart_[((size_t)k * nbqmax) * KD];
   }
   if (jj + 1 < nq) {
     nrm = ts_block_sum256(nrm, sh);
