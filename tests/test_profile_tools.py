@@ -4,7 +4,6 @@ totals, the copy-kernel calibration of FETCH_SIZE / WRITE_SIZE, the matrix-core
 busy fraction, and the k_ts_q2 rate from a kernel trace."""
 import csv
 import json
-import os
 import sqlite3
 import sys
 
