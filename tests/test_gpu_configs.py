@@ -168,7 +168,7 @@ def test_config_tbg_n11908_k3_learned_and_mixed():
     "was intermittent at this size (about one process in five: an iteration 0.02-2.3 eV off, "
     "Fermi level moved, from an identical start) through a race on the panel QR's pivot, which "
     "block 0 of k_ts_qa overwrote while later blocks could still read it; fixed by writing it "
-    "in k_ts_qb, and kept non-strict until more runs than two confirm the fix"))
+    "in k_ts_qb, and kept non-strict until more runs than three confirm the fix"))
 def test_config_ttg_n17862_k3_mixed_vs_fp64():
     e = Engine(Setup(ntheta=31, nlayers=3, numk=3, nfilling=2.0, xi_nm=40.0))
     say("trilayer n17862 k3: n =", e.n)
